@@ -1,0 +1,218 @@
+/*
+ * omm_hip.h -- C-ABI of the MI355X (gfx950) OMM hot-path library, libommhip.so.
+ *
+ * The library is the device half of a MatrixSwitch storage format that keeps matrices resident
+ * in HBM ("g" storage labels, operation label "hip") and of libOMM's CG loop driven over it.
+ * It is called from the Fortran host (MatrixSwitch and omm() drop-ins, through ISO_C_BINDING
+ * interfaces) and, for tests and benchmarks, from Python through ctypes.  The signatures use
+ * only plain C types: device pointers are void*, sizes are int / int64_t, scalars are double.
+ *
+ * Conventions
+ *   - Every function returns 0 (OMX_OK) or a negative OMX_ERR_* code; omx_last_error() gives
+ *     the message.  Invalid shapes or leading dimensions are rejected before any launch.
+ *   - Dense matrices are column-major with a leading dimension, as MatrixSwitch's dval(:,:)
+ *     (MatrixSwitch/src/MatrixSwitch_ops.F90:84).  Element types are OMX_F64, OMX_F32,
+ *     OMX_BF16, OMX_C128, OMX_C64.
+ *   - Work is enqueued on the context's stream and is asynchronous, except functions that
+ *     return a host scalar (traces, line-search coefficients, dots), which synchronise.
+ *   - A context is used from one host thread at a time; the reference is non-reentrant too
+ *     (libOMM/src/omm.F90:62-106 keeps module SAVE state).
+ */
+#ifndef OMM_HIP_H
+#define OMM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define OMX_OK 0
+#define OMX_ERR_ARG (-1)         /* invalid argument / incompatible shapes */
+#define OMX_ERR_HIP (-2)         /* HIP runtime error */
+#define OMX_ERR_NOMEM (-3)       /* device allocation failed */
+#define OMX_ERR_UNSUPPORTED (-4) /* combination not implemented */
+#define OMX_ERR_STATE (-5)       /* call sequence error */
+
+/* ---- element types and product precisions ------------------------------------------------ */
+#define OMX_F64 0
+#define OMX_F32 1
+#define OMX_BF16 2
+#define OMX_C128 3
+#define OMX_C64 4
+
+#define OMX_PREC_F64 0  /* FP64 MFMA, FP64 accumulate (the oracle-grade path) */
+#define OMX_PREC_F32 1  /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
+#define OMX_PREC_BF16 2 /* operands rounded to BF16, BF16 MFMA, FP32 accumulate */
+
+/* ---- counter-based generator streams (lattice inputs) ------------------------------------- */
+#define OMX_STREAM_ONSITE 1
+#define OMX_STREAM_HOP 2
+#define OMX_STREAM_OVL 3
+#define OMX_STREAM_C0 4
+#define OMX_STREAM_THETA 5
+
+/* ---- kernel classes for event timing (omx_prof_*) ----------------------------------------- */
+#define OMX_K_SPMM 0
+#define OMX_K_GEMM 1
+#define OMX_K_COEFF 2
+#define OMX_K_ELEM 3
+#define OMX_K_DOT 4
+#define OMX_K_COUNT 5
+
+typedef struct omx_ctx_s *omx_ctx;
+typedef struct omx_bsr_s *omx_bsr;
+
+/* ==== context, memory ====================================================================== */
+
+/* Number of visible HIP devices (0 without a GPU; does not fail). */
+int omx_device_count(int *count);
+/* Create a context on `device` with its own (default-priority) stream. */
+int omx_init(int device, omx_ctx *ctx);
+int omx_finalize(omx_ctx ctx);
+/* Wait for all work enqueued on the context's stream. */
+int omx_sync(omx_ctx ctx);
+/* Message of the last error on this host thread ("" if none). */
+const char *omx_last_error(void);
+/* hipStream_t of the context, as void*, for callers that enqueue their own work. */
+void *omx_stream(omx_ctx ctx);
+
+/* Device allocation, zero-filled, as m_allocate (MatrixSwitch/src/MatrixSwitch.F90:394-415). */
+int omx_malloc(omx_ctx ctx, size_t bytes, void **dptr);
+int omx_free(omx_ctx ctx, void *dptr);
+/* Zero `bytes` of device memory with a fill kernel (safe under stream capture). */
+int omx_zero(omx_ctx ctx, void *dptr, size_t bytes);
+int omx_h2d(omx_ctx ctx, void *dst, const void *src, size_t bytes);
+int omx_d2h(omx_ctx ctx, void *dst, const void *src, size_t bytes);
+int omx_d2d(omx_ctx ctx, void *dst, const void *src, size_t bytes);
+/* Strided 2-D host<->device copies of an m x n column-major matrix of element size esize. */
+int omx_h2d_2d(omx_ctx ctx, void *dst, int64_t ldd, const void *src, int64_t lds, int m, int n, int esize);
+int omx_d2h_2d(omx_ctx ctx, void *dst, int64_t ldd, const void *src, int64_t lds, int m, int n, int esize);
+/* Element-type conversion of an m x n matrix (any real type pair). */
+int omx_convert(omx_ctx ctx, int m, int n, const void *src, int dt_src, int64_t lds, void *dst, int dt_dst,
+                int64_t ldd);
+
+/* ==== BSR-4 sparse operators (H, S) ======================================================== */
+
+/*
+ * Upload a basis-size nb operator given as 0-based BSR-4 block columns (block-CSC: bcolptr has
+ * ceil(nb/4)+1 entries, browidx block rows ascending, 16 column-major values per block).
+ * Replaces the CSC registration m_register_pcsc / psp_register_spm
+ * (MatrixSwitch/src/MatrixSwitch_m_register.F90:394-443, pspBLAS/src/pspVariable/pspMat.F90:87-140):
+ * like pspBLAS, the data are copied, here into HBM.
+ */
+int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval, omx_bsr *out);
+/*
+ * Build the BSR-4 operator from a CSC matrix (index_base 0 or 1; colptr 0- or 1-based with
+ * the same base).  Accepts both MatrixSwitch's 'sdcsc' (1-based rows iaux3, 0-based offsets
+ * iaux4: pass rowbase=1, ptrbase=0; MatrixSwitch/src/MatrixSwitch_m_copy.F90:574-599) and
+ * pspBLAS's pdcsc (both 1-based; pspMat.F90:31-60).  Blocks that hold no entry are not stored.
+ */
+int omx_bsr_from_csc(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
+                     int rowbase, omx_bsr *out);
+int omx_bsr_destroy(omx_bsr a);
+int omx_bsr_info(omx_bsr a, int *nb, int64_t *nnzb, int *is_complex);
+/* Replace the block values (same pattern), e.g. after a parameter change. */
+int omx_bsr_update_values(omx_bsr a, const double *bval);
+
+/* ==== MatrixSwitch operations on device matrices =========================================== */
+
+/*
+ * Y := alpha*X*A + beta*Y, X and Y no x nb dense (element type dt: OMX_F64, OMX_F32, OMX_C128,
+ * OMX_C64), A a BSR-4 operator.  The dense x sparse product mm_multiply(C,'n',H,'n',HC)
+ * (libOMM/src/omm_ops.F90:61 -> MatrixSwitch/src/MatrixSwitch_mm_multiply.F90:263-327 'sdcsc',
+ * pspBLAS/src/pspLevel3/pspMspm_nn.F90:58-212 'pdcsc').
+ */
+int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, int64_t ldx, omx_bsr A, double beta,
+             void *Y, int64_t ldy);
+
+/*
+ * C := alpha*op(A)*op(B) + beta*C  (mm_multiply on dense operands,
+ * MatrixSwitch/src/MatrixSwitch.F90:1046-1390; 'ref' semantics of
+ * MatrixSwitch_mm_multiply.F90:19-79).  op = 'n', 't' or 'c' ('c' = 't' for real data).
+ * A, B and C share the element type dt (OMX_F64 / OMX_F32); prec selects the MFMA path.
+ * beta == 0 overwrites C (BLAS convention).
+ */
+int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+             int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+/* C := alpha*op(A) + beta*C  (m_add, MatrixSwitch.F90:1731-1902; MatrixSwitch_m_add.F90:95-175). */
+int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const void *A, int64_t lda, double beta,
+            void *C, int64_t ldc);
+/* C := beta*C  (m_scale, MatrixSwitch.F90:2496-2621). */
+int omx_scale(omx_ctx ctx, int dt, int m, int n, double beta, void *C, int64_t ldc);
+/* C_ij := alpha (i /= j), beta (i == j) on 'a'll, 'u'pper or 'l'ower part
+ * (m_set, MatrixSwitch.F90:2747-2827; MatrixSwitch_m_set.F90:19-75). */
+int omx_set(omx_ctx ctx, int dt, char seC, int m, int n, double alpha, double beta, void *C, int64_t ldc);
+/* alpha := tr(A), FP64 accumulation (m_trace, MatrixSwitch.F90:2060-2156). Synchronises. */
+int omx_trace(omx_ctx ctx, int dt, int n, const void *A, int64_t lda, double *out);
+/* alpha := sum_ij A_ij*B_ij = tr(A^T B), FP64 accumulation (mm_trace, MatrixSwitch.F90:2253-2377;
+ * complex: sum conj(A_ij)*B_ij, :2471, out = (re, im)). Synchronises. */
+int omx_mm_trace(omx_ctx ctx, int dt, int m, int n, const void *A, int64_t lda, const void *B, int64_t ldb,
+                 double *out);
+/* C_ij := alpha + beta*C_ij for one element, 1-based (m_set_element, MatrixSwitch.F90:2917-3086). */
+int omx_set_element(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha, double beta);
+/* alpha := C_ij, 1-based (m_get_element, MatrixSwitch.F90:3309-3417). Synchronises. */
+int omx_get_element(omx_ctx ctx, int dt, const void *C, int64_t ldc, int i, int j, double *out);
+
+/* ==== fused libOMM kernels ================================================================== */
+
+/*
+ * The five quartic line-search coefficients from one fused FP64 reduction over HW, SW, HWd,
+ * SWd, HWdd, SWdd (n x n, element type dt, leading dimension ld): replaces calc_coeff's three
+ * m_trace, ten mm_trace and the SWd^T copy (libOMM/src/omm_ops.F90:156-219).  The reduction
+ * is two-stage and deterministic.  Synchronises and returns coeff[0..4] (coeff[0] is the energy).
+ */
+int omx_omm_coeff(omx_ctx ctx, int dt, int n, int64_t ld, const void *HW, const void *SW, const void *HWd,
+                  const void *SWd, const void *HWdd, const void *SWdd, double *coeff);
+
+/*
+ * G := 4*HC - 2*SW*HC - 2*HW*SC in one MFMA product with a K = 2n contraction
+ * (calc_G, libOMM/src/omm_ops.F90:17-39).  HW, SW: n x n; HC, SC, G: n x m.
+ */
+int omx_omm_calc_G(omx_ctx ctx, int prec, int dt, int n, int m, const void *HW, const void *SW, int64_t ldw,
+                   const void *HC, const void *SC, void *G, int64_t ldc);
+
+/* out[0] = <G, G - G_p>, out[1] = <G_p, G_p> in FP64 (omm.F90:536-546, Polak-Ribiere). Synchronises. */
+int omx_omm_pr_dots(omx_ctx ctx, int dt, int n, int m, const void *G, const void *G_p, int64_t ld, double *out);
+
+/* D := G + lambda*D and G_p := G in one pass (omm.F90:440-445). */
+int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const void *G, void *D, void *G_p,
+                         int64_t ld);
+
+/* W := W + x*(Wd^T + Wd) + x^2*Wdd in one pass (omm.F90:496-502 and 511-517). */
+int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const void *Wd, const void *Wdd, void *W);
+
+/* ==== event timing ========================================================================= */
+
+/* Start (enable=1) or stop accumulating HIP-event durations of launches of each kernel class. */
+int omx_prof_enable(omx_ctx ctx, int enable);
+/* Total milliseconds and launch count accumulated for a kernel class (synchronises). */
+int omx_prof_read(omx_ctx ctx, int kclass, double *ms, int64_t *launches);
+int omx_prof_reset(omx_ctx ctx);
+
+/* ==== synthetic inputs ====================================================================== */
+
+/*
+ * Periodic Lx x Ly x Lz simple-cubic lattice, 4 orbitals per site, nearest-neighbour blocks.
+ * which = 0: H (onsite diag(-5,1,1,1) + sym. U(-0.1,0.1) noise, hopping U(-1,1) blocks);
+ * which = 1: S (onsite I, bonds 0.03*U(-1,1)).  First call with bcolptr == NULL returns the
+ * block count in *nnzb; second call fills bcolptr (Lx*Ly*Lz+1), browidx (nnzb), bval (16*nnzb).
+ */
+int omx_lattice_bsr(int Lx, int Ly, int Lz, uint64_t seed, int which, int64_t *nnzb, int *bcolptr, int *browidx,
+                    double *bval);
+/* H(theta): one hopping block theta[0..15] (column-major) on every bond, onsite diag(theta[16..19]). */
+int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *nnzb, int *bcolptr, int *browidx,
+                          double *bval);
+/* Initial parameters: hopping U(-1,1) from the seed, onsite levels (-5, 1, 1, 1). */
+int omx_lattice_theta0(uint64_t seed, double *theta);
+/* C0(i,k) = U(-0.5,0.5)/sqrt(nb), no x nb column-major. */
+int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc);
+/* The underlying counter-based uniform variate in [0,1). */
+double omx_u01(uint64_t seed, uint64_t stream, uint64_t idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMM_HIP_H */

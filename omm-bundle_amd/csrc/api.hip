@@ -1,0 +1,488 @@
+// Context, memory, sparse-operator upload, event timing and element access of libommhip.so.
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <utility>
+
+#include "omx_internal.h"
+
+namespace omx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+int ensure_ws(omx_ctx ctx, size_t bytes) {
+    if (ctx->ws_bytes >= bytes) return OMX_OK;
+    if (ctx->ws) {
+        OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        OMX_CHECK_HIP(hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&ctx->ws, want) != hipSuccess) {
+        set_error("workspace allocation of %zu bytes failed", want);
+        return OMX_ERR_NOMEM;
+    }
+    ctx->ws_bytes = want;
+    return OMX_OK;
+}
+
+static hipEvent_t pool_get(omx_ctx ctx) {
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_begin(omx_ctx ctx, int kclass, hipEvent_t *ev_start) {
+    *ev_start = nullptr;
+    if (!ctx->prof_on) return;
+    hipEvent_t e = pool_get(ctx);
+    if (!e) return;
+    (void)hipEventRecord(e, ctx->stream);
+    *ev_start = e;
+}
+
+static void prof_drain(omx_ctx ctx) {
+    for (auto &p : ctx->prof_pending) {
+        (void)hipEventSynchronize(p.stop);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.start, p.stop);
+        ctx->prof_ms[p.kclass] += ms;
+        ctx->prof_count[p.kclass] += 1;
+        ctx->prof_pool.push_back(p.start);
+        ctx->prof_pool.push_back(p.stop);
+    }
+    ctx->prof_pending.clear();
+}
+
+void prof_end(omx_ctx ctx, int kclass, hipEvent_t ev_start) {
+    if (!ctx->prof_on || !ev_start) return;
+    hipEvent_t e = pool_get(ctx);
+    if (!e) return;
+    (void)hipEventRecord(e, ctx->stream);
+    ctx->prof_pending.push_back({ev_start, e, kclass});
+    if (ctx->prof_pending.size() > 4096) prof_drain(ctx);
+}
+
+int read_scalars(omx_ctx ctx, int n) {
+    OMX_CHECK_HIP(hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+// Grid-stride zero fill (used instead of hipMemsetAsync so that the call is capture-safe).
+__global__ void k_zero_bytes(uint8_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    // 16-byte stores only when the base is 16-byte aligned; bytes otherwise.
+    size_t n16 = (((uintptr_t)p & 15) == 0) ? n / 16 : 0;
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    for (size_t t = i; t < n16; t += stride) q[t] = make_uint4(0, 0, 0, 0);
+    for (size_t t = n16 * 16 + i; t < n; t += stride) p[t] = 0;
+}
+
+template <typename TS, typename TD>
+__device__ inline TD cvt(TS v) {
+    return (TD)(float)v;
+}
+template <>
+__device__ inline double cvt<double, double>(double v) { return v; }
+template <>
+__device__ inline float cvt<double, float>(double v) { return (float)v; }
+template <>
+__device__ inline double cvt<float, double>(float v) { return (double)v; }
+template <>
+__device__ inline float cvt<float, float>(float v) { return v; }
+template <>
+__device__ inline __hip_bfloat16 cvt<double, __hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+template <>
+__device__ inline __hip_bfloat16 cvt<float, __hip_bfloat16>(float v) { return __float2bfloat16(v); }
+template <>
+__device__ inline double cvt<__hip_bfloat16, double>(__hip_bfloat16 v) { return (double)__bfloat162float(v); }
+template <>
+__device__ inline float cvt<__hip_bfloat16, float>(__hip_bfloat16 v) { return __bfloat162float(v); }
+template <>
+__device__ inline __hip_bfloat16 cvt<__hip_bfloat16, __hip_bfloat16>(__hip_bfloat16 v) { return v; }
+
+template <typename TS, typename TD>
+__global__ void k_convert(int m, int n, const TS *__restrict__ src, int64_t lds, TD *__restrict__ dst, int64_t ldd) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = t % m, j = t / m;
+        dst[i + j * ldd] = cvt<TS, TD>(src[i + j * lds]);
+    }
+}
+
+template <typename TS>
+static int convert_from(omx_ctx ctx, int m, int n, const TS *src, int64_t lds, void *dst, int dt_dst, int64_t ldd) {
+    int64_t total = (int64_t)m * n;
+    int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    if (grid < 1) return OMX_OK;
+    switch (dt_dst) {
+        case OMX_F64:
+            hipLaunchKernelGGL((k_convert<TS, double>), dim3(grid), dim3(256), 0, ctx->stream, m, n, src, lds,
+                               (double *)dst, ldd);
+            break;
+        case OMX_F32:
+            hipLaunchKernelGGL((k_convert<TS, float>), dim3(grid), dim3(256), 0, ctx->stream, m, n, src, lds,
+                               (float *)dst, ldd);
+            break;
+        case OMX_BF16:
+            hipLaunchKernelGGL((k_convert<TS, __hip_bfloat16>), dim3(grid), dim3(256), 0, ctx->stream, m, n, src,
+                               lds, (__hip_bfloat16 *)dst, ldd);
+            break;
+        default:
+            set_error("omx_convert: unsupported destination type %d", dt_dst);
+            return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+// Element read/write kernels (single thread; used by m_set_element / m_get_element).
+template <typename T>
+__global__ void k_set_element(T *C, int64_t off, double alpha, double beta) {
+    C[off] = (T)(alpha + beta * (double)C[off]);
+}
+template <typename T>
+__global__ void k_get_element(const T *C, int64_t off, double *out) {
+    out[0] = (double)C[off];
+}
+
+}  // namespace omx
+
+using namespace omx;
+
+extern "C" {
+
+const char *omx_last_error(void) { return g_err; }
+
+int omx_device_count(int *count) {
+    if (!count) return OMX_ERR_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return OMX_OK;
+}
+
+int omx_init(int device, omx_ctx *out) {
+    clear_error();
+    OMX_REQUIRE(out != nullptr, "omx_init: null output");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("omx_init: no HIP device available");
+        return OMX_ERR_HIP;
+    }
+    OMX_REQUIRE(device >= 0 && device < count, "omx_init: device %d out of range (%d devices)", device, count);
+    OMX_CHECK_HIP(hipSetDevice(device));
+    omx_ctx ctx = new omx_ctx_s();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        set_error("omx_init: stream creation failed");
+        return OMX_ERR_HIP;
+    }
+    if (hipMalloc(&ctx->d_scalars, sizeof(double) * kScalarSlots) != hipSuccess ||
+        hipHostMalloc(&ctx->h_scalars, sizeof(double) * kScalarSlots, hipHostMallocDefault) != hipSuccess) {
+        set_error("omx_init: scalar buffer allocation failed");
+        return OMX_ERR_NOMEM;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return OMX_OK;
+}
+
+int omx_finalize(omx_ctx ctx) {
+    if (!ctx) return OMX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_drain(ctx);
+    for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
+    if (ctx->h_scalars) (void)hipHostFree(ctx->h_scalars);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OMX_OK;
+}
+
+int omx_sync(omx_ctx ctx) {
+    OMX_REQUIRE(ctx, "omx_sync: null context");
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+void *omx_stream(omx_ctx ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int omx_zero(omx_ctx ctx, void *dptr, size_t bytes) {
+    OMX_REQUIRE(ctx, "omx_zero: null context");
+    if (bytes == 0) return OMX_OK;
+    OMX_REQUIRE(dptr, "omx_zero: null pointer");
+    size_t blocks = std::min<size_t>((bytes / 16 + 255) / 256 + 1, 16384);
+    hipLaunchKernelGGL(k_zero_bytes, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (uint8_t *)dptr, bytes);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_malloc(omx_ctx ctx, size_t bytes, void **dptr) {
+    OMX_REQUIRE(ctx && dptr, "omx_malloc: null argument");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        set_error("omx_malloc: allocation of %zu bytes failed", bytes);
+        *dptr = nullptr;
+        return OMX_ERR_NOMEM;
+    }
+    return omx_zero(ctx, *dptr, bytes);
+}
+
+int omx_free(omx_ctx ctx, void *dptr) {
+    OMX_REQUIRE(ctx, "omx_free: null context");
+    if (!dptr) return OMX_OK;
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    OMX_CHECK_HIP(hipFree(dptr));
+    return OMX_OK;
+}
+
+int omx_h2d(omx_ctx ctx, void *dst, const void *src, size_t bytes) {
+    OMX_REQUIRE(ctx, "omx_h2d: null context");
+    if (!bytes) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+int omx_d2h(omx_ctx ctx, void *dst, const void *src, size_t bytes) {
+    OMX_REQUIRE(ctx, "omx_d2h: null context");
+    if (!bytes) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+int omx_d2d(omx_ctx ctx, void *dst, const void *src, size_t bytes) {
+    OMX_REQUIRE(ctx, "omx_d2d: null context");
+    if (!bytes) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return OMX_OK;
+}
+
+int omx_h2d_2d(omx_ctx ctx, void *dst, int64_t ldd, const void *src, int64_t lds, int m, int n, int esize) {
+    OMX_REQUIRE(ctx && m >= 0 && n >= 0 && esize > 0 && ldd >= m && lds >= m, "omx_h2d_2d: bad arguments");
+    if (!m || !n) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpy2DAsync(dst, ldd * esize, src, lds * esize, (size_t)m * esize, n, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+int omx_d2h_2d(omx_ctx ctx, void *dst, int64_t ldd, const void *src, int64_t lds, int m, int n, int esize) {
+    OMX_REQUIRE(ctx && m >= 0 && n >= 0 && esize > 0 && ldd >= m && lds >= m, "omx_d2h_2d: bad arguments");
+    if (!m || !n) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpy2DAsync(dst, ldd * esize, src, lds * esize, (size_t)m * esize, n, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return OMX_OK;
+}
+
+int omx_convert(omx_ctx ctx, int m, int n, const void *src, int dt_src, int64_t lds, void *dst, int dt_dst,
+                int64_t ldd) {
+    OMX_REQUIRE(ctx && src && dst && m >= 0 && n >= 0 && lds >= m && ldd >= m, "omx_convert: bad arguments");
+    switch (dt_src) {
+        case OMX_F64: return convert_from(ctx, m, n, (const double *)src, lds, dst, dt_dst, ldd);
+        case OMX_F32: return convert_from(ctx, m, n, (const float *)src, lds, dst, dt_dst, ldd);
+        case OMX_BF16: return convert_from(ctx, m, n, (const __hip_bfloat16 *)src, lds, dst, dt_dst, ldd);
+        default: set_error("omx_convert: unsupported source type %d", dt_src); return OMX_ERR_UNSUPPORTED;
+    }
+}
+
+// ---- sparse operators -----------------------------------------------------------------------
+
+static int bsr_upload(omx_ctx ctx, int nb, const std::vector<int> &colptr, const std::vector<int> &rowidx,
+                      const double *val, int is_complex, omx_bsr *out) {
+    omx_bsr a = new omx_bsr_s();
+    a->ctx = ctx;
+    a->nb = nb;
+    a->nbc = (nb + 3) / 4;
+    a->nnzb = (int64_t)rowidx.size();
+    a->is_complex = is_complex;
+    int mx = 0;
+    for (int j = 0; j < a->nbc; ++j) mx = std::max(mx, colptr[j + 1] - colptr[j]);
+    a->max_blocks_per_col = mx;
+    size_t vbytes = sizeof(double) * 16 * (size_t)a->nnzb * (is_complex ? 2 : 1);
+    if (hipMalloc(&a->d_colptr, sizeof(int) * (a->nbc + 1)) != hipSuccess ||
+        hipMalloc(&a->d_rowidx, sizeof(int) * std::max<int64_t>(a->nnzb, 1)) != hipSuccess ||
+        hipMalloc(&a->d_val, std::max<size_t>(vbytes, 16)) != hipSuccess) {
+        omx_bsr_destroy(a);
+        set_error("omx_bsr: device allocation failed");
+        return OMX_ERR_NOMEM;
+    }
+    (void)hipMemcpyAsync(a->d_colptr, colptr.data(), sizeof(int) * (a->nbc + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (a->nnzb) {
+        (void)hipMemcpyAsync(a->d_rowidx, rowidx.data(), sizeof(int) * a->nnzb, hipMemcpyHostToDevice, ctx->stream);
+        (void)hipMemcpyAsync(a->d_val, val, vbytes, hipMemcpyHostToDevice, ctx->stream);
+    }
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    *out = a;
+    return OMX_OK;
+}
+
+int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval, omx_bsr *out) {
+    OMX_REQUIRE(ctx && out && bcolptr && nb > 0, "omx_bsr_create: bad arguments");
+    int nbc = (nb + 3) / 4;
+    OMX_REQUIRE(bcolptr[0] == 0, "omx_bsr_create: bcolptr must be 0-based");
+    for (int j = 0; j < nbc; ++j)
+        OMX_REQUIRE(bcolptr[j + 1] >= bcolptr[j], "omx_bsr_create: bcolptr not monotone at %d", j);
+    int64_t nnzb = bcolptr[nbc];
+    OMX_REQUIRE(nnzb == 0 || (browidx && bval), "omx_bsr_create: null block arrays");
+    for (int j = 0; j < nbc; ++j)
+        for (int l = bcolptr[j]; l < bcolptr[j + 1]; ++l) {
+            OMX_REQUIRE(browidx[l] >= 0 && browidx[l] < nbc, "omx_bsr_create: block row %d out of range", browidx[l]);
+            OMX_REQUIRE(l == bcolptr[j] || browidx[l] > browidx[l - 1],
+                        "omx_bsr_create: block rows of column %d not strictly ascending", j);
+        }
+    std::vector<int> cp(bcolptr, bcolptr + nbc + 1), ri(browidx, browidx + nnzb);
+    return bsr_upload(ctx, nb, cp, ri, bval, 0, out);
+}
+
+int omx_bsr_from_csc(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
+                     int rowbase, omx_bsr *out) {
+    OMX_REQUIRE(ctx && out && colptr && nb > 0, "omx_bsr_from_csc: bad arguments");
+    OMX_REQUIRE((ptrbase == 0 || ptrbase == 1) && (rowbase == 0 || rowbase == 1), "omx_bsr_from_csc: bad base");
+    int nbc = (nb + 3) / 4;
+    int64_t nnz = (int64_t)colptr[nb] - ptrbase;
+    OMX_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val)), "omx_bsr_from_csc: bad nnz");
+    // Per block column: ordered map block row -> block index.
+    std::vector<int> cp(nbc + 1, 0), ri;
+    std::vector<double> bv;
+    for (int J = 0; J < nbc; ++J) {
+        std::map<int, std::vector<std::pair<int, double>>> blocks;  // I -> (r + 4c, value)
+        for (int c = 0; c < 4; ++c) {
+            int j = 4 * J + c;
+            if (j >= nb) break;
+            for (int64_t l = colptr[j] - ptrbase; l < colptr[j + 1] - ptrbase; ++l) {
+                int i = rowidx[l] - rowbase;
+                OMX_REQUIRE(i >= 0 && i < nb, "omx_bsr_from_csc: row index %d out of range", i + rowbase);
+                blocks[i / 4].push_back({(i % 4) + 4 * c, val[l]});
+            }
+        }
+        for (auto &kv : blocks) {
+            ri.push_back(kv.first);
+            size_t base = bv.size();
+            bv.resize(base + 16, 0.0);
+            for (auto &e : kv.second) bv[base + e.first] += e.second;
+        }
+        cp[J + 1] = (int)ri.size();
+    }
+    return bsr_upload(ctx, nb, cp, ri, bv.data(), 0, out);
+}
+
+int omx_bsr_destroy(omx_bsr a) {
+    if (!a) return OMX_OK;
+    if (a->ctx) (void)hipStreamSynchronize(a->ctx->stream);
+    if (a->d_colptr) (void)hipFree(a->d_colptr);
+    if (a->d_rowidx) (void)hipFree(a->d_rowidx);
+    if (a->d_val) (void)hipFree(a->d_val);
+    delete a;
+    return OMX_OK;
+}
+
+int omx_bsr_info(omx_bsr a, int *nb, int64_t *nnzb, int *is_complex) {
+    OMX_REQUIRE(a, "omx_bsr_info: null operator");
+    if (nb) *nb = a->nb;
+    if (nnzb) *nnzb = a->nnzb;
+    if (is_complex) *is_complex = a->is_complex;
+    return OMX_OK;
+}
+
+int omx_bsr_update_values(omx_bsr a, const double *bval) {
+    OMX_REQUIRE(a && (bval || a->nnzb == 0), "omx_bsr_update_values: bad arguments");
+    size_t vbytes = sizeof(double) * 16 * (size_t)a->nnzb * (a->is_complex ? 2 : 1);
+    if (!vbytes) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpyAsync(a->d_val, bval, vbytes, hipMemcpyHostToDevice, a->ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+    return OMX_OK;
+}
+
+// ---- element access --------------------------------------------------------------------------
+
+int omx_set_element(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha, double beta) {
+    OMX_REQUIRE(ctx && C && i >= 1 && j >= 1 && ldc >= i, "omx_set_element: bad arguments");
+    int64_t off = (int64_t)(i - 1) + (int64_t)(j - 1) * ldc;
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_set_element<double>, dim3(1), dim3(1), 0, ctx->stream, (double *)C, off, alpha, beta);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_set_element<float>, dim3(1), dim3(1), 0, ctx->stream, (float *)C, off, alpha, beta);
+    else {
+        set_error("omx_set_element: unsupported type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_get_element(omx_ctx ctx, int dt, const void *C, int64_t ldc, int i, int j, double *out) {
+    OMX_REQUIRE(ctx && C && out && i >= 1 && j >= 1 && ldc >= i, "omx_get_element: bad arguments");
+    int64_t off = (int64_t)(i - 1) + (int64_t)(j - 1) * ldc;
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_get_element<double>, dim3(1), dim3(1), 0, ctx->stream, (const double *)C, off,
+                           ctx->d_scalars);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_get_element<float>, dim3(1), dim3(1), 0, ctx->stream, (const float *)C, off,
+                           ctx->d_scalars);
+    else {
+        set_error("omx_get_element: unsupported type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    int rc = read_scalars(ctx, 1);
+    if (rc) return rc;
+    *out = ctx->h_scalars[0];
+    return OMX_OK;
+}
+
+// ---- event timing ----------------------------------------------------------------------------
+
+int omx_prof_enable(omx_ctx ctx, int enable) {
+    OMX_REQUIRE(ctx, "omx_prof_enable: null context");
+    ctx->prof_on = enable != 0;
+    return OMX_OK;
+}
+
+int omx_prof_read(omx_ctx ctx, int kclass, double *ms, int64_t *launches) {
+    OMX_REQUIRE(ctx && kclass >= 0 && kclass < OMX_K_COUNT, "omx_prof_read: bad arguments");
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    prof_drain(ctx);
+    if (ms) *ms = ctx->prof_ms[kclass];
+    if (launches) *launches = ctx->prof_count[kclass];
+    return OMX_OK;
+}
+
+int omx_prof_reset(omx_ctx ctx) {
+    OMX_REQUIRE(ctx, "omx_prof_reset: null context");
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    prof_drain(ctx);
+    for (int k = 0; k < OMX_K_COUNT; ++k) {
+        ctx->prof_ms[k] = 0;
+        ctx->prof_count[k] = 0;
+    }
+    return OMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,520 @@
+// Memory-bound kernels of the OMM loop: MatrixSwitch m_add / m_scale / m_set, traces, the fused
+// line-search coefficient reduction, and the fused CG vector updates.
+//
+// Reductions are two-stage and deterministic: every workgroup writes FP64 partial sums for its
+// tile range into the context workspace, then one workgroup adds the partials in workgroup
+// order and writes the result to the context's scalar slots (read back through pinned memory).
+#include <algorithm>
+
+#include "omx_internal.h"
+
+namespace omx {
+
+constexpr int EW_NT = 256;
+constexpr int TT = 32;  // transpose / trace tile edge
+
+template <typename T>
+__device__ inline double ld(const T *p) { return (double)*p; }
+
+// ---- block reduction helpers ----------------------------------------------------------------
+
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], double *__restrict__ out) {
+    __shared__ double red[NV][EW_NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) red[q][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < EW_NT / 64; ++w) s += red[threadIdx.x][w];
+        out[threadIdx.x] = s;
+    }
+}
+
+// Final stage: out[q] = sum over blocks b (ascending) of partial[b*NV + q].
+template <int NV>
+__global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const double *__restrict__ partial,
+                                                           double *__restrict__ out) {
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += EW_NT)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] += partial[(int64_t)b * NV + q];
+    block_reduce_store<NV>(v, out);
+}
+
+// ---- m_add / m_scale / m_set -----------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_add_n(int m, int n, double alpha, const T *__restrict__ A, int64_t lda,
+                                                 double beta, T *__restrict__ C, int64_t ldc) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m;
+        double a = alpha * (double)A[i + j * lda];
+        C[i + j * ldc] = (T)(beta == 0.0 ? a : beta * (double)C[i + j * ldc] + a);
+    }
+}
+
+// C(i,j) := beta*C(i,j) + alpha*A(j,i): 32x32 tiles through LDS.
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_add_t(int m, int n, double alpha, const T *__restrict__ A, int64_t lda,
+                                                 double beta, T *__restrict__ C, int64_t ldc) {
+    __shared__ T tile[TT][TT + 1];
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;  // 32 x 8
+    const int ti = blockIdx.x * TT, tj = blockIdx.y * TT;     // C tile origin
+    // A is n x m; read A(tj + r, ti + c) for the C tile (ti.., tj..).
+    for (int c = ty; c < TT; c += EW_NT / TT) {
+        int ar = tj + tx, ac = ti + c;
+        tile[c][tx] = (ar < n && ac < m) ? A[(int64_t)ar + (int64_t)ac * lda] : (T)0;
+    }
+    __syncthreads();
+    for (int c = ty; c < TT; c += EW_NT / TT) {
+        int i = ti + tx, j = tj + c;
+        if (i < m && j < n) {
+            double a = alpha * (double)tile[tx][c];
+            T *p = C + (int64_t)i + (int64_t)j * ldc;
+            *p = (T)(beta == 0.0 ? a : beta * (double)*p + a);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_scale(int m, int n, double beta, T *__restrict__ C, int64_t ldc) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m;
+        C[i + j * ldc] = (T)(beta * (double)C[i + j * ldc]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_set(int lu, int m, int n, double alpha, double beta, T *__restrict__ C,
+                                               int64_t ldc) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m;
+        if (lu == 1 && j < i) continue;
+        if (lu == 2 && j > i) continue;
+        C[i + j * ldc] = (T)(i != j ? alpha : beta);
+    }
+}
+
+// ---- traces ----------------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_trace_partial(int n, const T *__restrict__ A, int64_t lda,
+                                                         double *__restrict__ partial) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * EW_NT + threadIdx.x; i < n; i += gridDim.x * EW_NT) v[0] += ld(A + (int64_t)i * (lda + 1));
+    block_reduce_store<1>(v, partial + blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_dot_partial(int m, int n, const T *__restrict__ A, int64_t lda,
+                                                       const T *__restrict__ B, int64_t ldb,
+                                                       double *__restrict__ partial) {
+    double v[1] = {0.0};
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m;
+        v[0] += (double)A[i + j * lda] * (double)B[i + j * ldb];
+    }
+    block_reduce_store<1>(v, partial + blockIdx.x);
+}
+
+// <G, G - G_p> and <G_p, G_p> (Polak-Ribiere numerator and denominator).
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_pr_partial(int64_t total_m, int m, const T *__restrict__ G,
+                                                      const T *__restrict__ Gp, int64_t ld,
+                                                      double *__restrict__ partial) {
+    double v[2] = {0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total_m; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m;
+        double g = (double)G[i + j * ld], p = (double)Gp[i + j * ld];
+        double w = (double)(T)(g - p);
+        v[0] += g * w;
+        v[1] += p * p;
+    }
+    block_reduce_store<2>(v, partial + 2 * (int64_t)blockIdx.x);
+}
+
+// ---- fused line-search coefficients (calc_coeff, libOMM/src/omm_ops.F90:156-219) --------------
+// Thirteen sums over n x n matrices; tile (I,J) also reads the SWd tile (J,I) for the
+// transposed term TrHdSd = sum_ij SWd(j,i)*HWd(i,j).
+// Partial layout per block: [TrH, TrHS, TrHd, TrHdS, TrHSd, TrHdd, TrHddS, TrHSdd, TrHdSd,
+//                            TrHdSdH, TrHddSd, TrHdSdd, TrHddSdd]
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, const T *__restrict__ HW,
+                                                         const T *__restrict__ SW, const T *__restrict__ HWd,
+                                                         const T *__restrict__ SWd, const T *__restrict__ HWdd,
+                                                         const T *__restrict__ SWdd, double *__restrict__ partial) {
+    __shared__ double swdT[TT][TT + 1];
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    const int tiles = (n + TT - 1) / TT;
+    const int I = blockIdx.x % tiles, J = blockIdx.x / tiles;
+    const int i0 = I * TT, j0 = J * TT;
+    // SWd tile (J,I): element (j0 + a, i0 + b) -> swdT[a][b]
+    for (int b = ty; b < TT; b += EW_NT / TT) {
+        int r = j0 + tx, c = i0 + b;
+        swdT[tx][b] = (r < n && c < n) ? (double)SWd[(int64_t)r + (int64_t)c * ldm] : 0.0;
+    }
+    __syncthreads();
+    double v[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) v[q] = 0.0;
+    for (int c = ty; c < TT; c += EW_NT / TT) {
+        const int i = i0 + tx, j = j0 + c;
+        if (i >= n || j >= n) continue;
+        const int64_t o = (int64_t)i + (int64_t)j * ldm;
+        const double hw = HW[o], sw = SW[o], hwd = HWd[o], swd = SWd[o], hwdd = HWdd[o], swdd = SWdd[o];
+        if (i == j) {
+            v[0] += hw;
+            v[2] += hwd;
+            v[5] += hwdd;
+        }
+        v[1] += hw * sw;
+        v[3] += sw * hwd;
+        v[4] += hw * swd;
+        v[6] += hwdd * sw;
+        v[7] += hw * swdd;
+        v[8] += swdT[c][tx] * hwd;  // SWd(j,i) * HWd(i,j)
+        v[9] += swd * hwd;
+        v[10] += hwdd * swd;
+        v[11] += swdd * hwd;
+        v[12] += hwdd * swdd;
+    }
+    block_reduce_store<13>(v, partial + 13 * (int64_t)blockIdx.x);
+}
+
+__global__ void k_coeff_final(int nblocks, const double *__restrict__ partial, double *__restrict__ out) {
+    // One workgroup: deterministic per-term sums, then the closed-form coefficients.
+    double v[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += EW_NT)
+#pragma unroll
+        for (int q = 0; q < 13; ++q) v[q] += partial[(int64_t)b * 13 + q];
+    __shared__ double tr[13];
+    block_reduce_store<13>(v, tr);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double TrH = tr[0], TrHS = tr[1], TrHd = tr[2], TrHdS = tr[3], TrHSd = tr[4], TrHdd = tr[5],
+                     TrHddS = tr[6], TrHSdd = tr[7], TrHdSd = tr[8], TrHdSdH = tr[9], TrHddSd = tr[10],
+                     TrHdSdd = tr[11], TrHddSdd = tr[12];
+        out[0] = 2.0 * TrH - TrHS;
+        out[1] = 2.0 * (2.0 * TrHd - TrHdS - TrHSd);
+        out[2] = 2.0 * (TrHdd - TrHdSd - TrHdSdH) - TrHddS - TrHSdd;
+        out[3] = -2.0 * (TrHddSd + TrHdSdd);
+        out[4] = -TrHddSdd;
+        for (int q = 0; q < 13; ++q) out[5 + q] = tr[q];
+    }
+}
+
+// ---- fused CG updates ------------------------------------------------------------------------
+
+// D := G + lambda*D; G_p := G  (omm.F90:440-445)
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_cg_direction(int m, int n, double lambda, const T *__restrict__ G,
+                                                        T *__restrict__ D, T *__restrict__ Gp, int64_t ld) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
+        int64_t i = q % m, j = q / m, o = i + j * ld;
+        double g = (double)G[o];
+        double d = (lambda == 0.0) ? g : lambda * (double)D[o] + g;
+        D[o] = (T)d;
+        Gp[o] = (T)g;
+    }
+}
+
+// W := W + x*Wd^T + x*Wd + x^2*Wdd, accumulated in that order (omm.F90:499-501, 514-516).
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_update_w(int n, int64_t ldm, double x, const T *__restrict__ Wd,
+                                                    const T *__restrict__ Wdd, T *__restrict__ W) {
+    __shared__ double tileT[TT][TT + 1];
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    const int i0 = blockIdx.x * TT, j0 = blockIdx.y * TT;
+    for (int b = ty; b < TT; b += EW_NT / TT) {
+        int r = j0 + tx, c = i0 + b;  // Wd(j0+tx, i0+b)
+        tileT[tx][b] = (r < n && c < n) ? (double)Wd[(int64_t)r + (int64_t)c * ldm] : 0.0;
+    }
+    __syncthreads();
+    const double x2 = x * x;
+    for (int c = ty; c < TT; c += EW_NT / TT) {
+        int i = i0 + tx, j = j0 + c;
+        if (i >= n || j >= n) continue;
+        int64_t o = (int64_t)i + (int64_t)j * ldm;
+        double w = (double)W[o];
+        w = w + x * tileT[c][tx];
+        w = w + x * (double)Wd[o];
+        w = w + x2 * (double)Wdd[o];
+        W[o] = (T)w;
+    }
+}
+
+inline int grid_for(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + EW_NT - 1) / EW_NT, 65536)); }
+
+// Two-stage sum: partial kernel already launched with `blocks` workgroups and NV values each.
+template <int NV>
+static int finish_reduce(omx_ctx ctx, int blocks, double *partial) {
+    hipLaunchKernelGGL((k_reduce_partials<NV>), dim3(1), dim3(EW_NT), 0, ctx->stream, blocks, partial, ctx->d_scalars);
+    OMX_CHECK_HIP(hipGetLastError());
+    return read_scalars(ctx, NV);
+}
+
+}  // namespace omx
+
+using namespace omx;
+
+extern "C" {
+
+int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const void *A, int64_t lda, double beta,
+            void *C, int64_t ldc) {
+    OMX_REQUIRE(ctx && valid_op(opA) && m >= 0 && n >= 0, "omx_add: bad arguments");
+    bool tr = is_trans(opA);
+    OMX_REQUIRE(ldc >= std::max(1, m) && lda >= std::max(1, tr ? n : m), "omx_add: bad leading dimension");
+    if (!m || !n) return OMX_OK;
+    OMX_REQUIRE(A && C, "omx_add: null operand");
+    OMX_REQUIRE(!(tr && A == C), "omx_add: in-place transpose is not supported");
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_ELEM, &ev);
+    if (!tr) {
+        int grid = grid_for((int64_t)m * n);
+        if (dt == OMX_F64)
+            hipLaunchKernelGGL(k_add_n<double>, dim3(grid), dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const double *)A,
+                               lda, beta, (double *)C, ldc);
+        else if (dt == OMX_F32)
+            hipLaunchKernelGGL(k_add_n<float>, dim3(grid), dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const float *)A,
+                               lda, beta, (float *)C, ldc);
+        else {
+            set_error("omx_add: unsupported element type %d", dt);
+            return OMX_ERR_UNSUPPORTED;
+        }
+    } else {
+        dim3 grid((m + TT - 1) / TT, (n + TT - 1) / TT);
+        if (dt == OMX_F64)
+            hipLaunchKernelGGL(k_add_t<double>, grid, dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const double *)A, lda,
+                               beta, (double *)C, ldc);
+        else if (dt == OMX_F32)
+            hipLaunchKernelGGL(k_add_t<float>, grid, dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const float *)A, lda,
+                               beta, (float *)C, ldc);
+        else {
+            set_error("omx_add: unsupported element type %d", dt);
+            return OMX_ERR_UNSUPPORTED;
+        }
+    }
+    prof_end(ctx, OMX_K_ELEM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_scale(omx_ctx ctx, int dt, int m, int n, double beta, void *C, int64_t ldc) {
+    OMX_REQUIRE(ctx && m >= 0 && n >= 0 && ldc >= std::max(1, m), "omx_scale: bad arguments");
+    if (!m || !n) return OMX_OK;
+    OMX_REQUIRE(C, "omx_scale: null operand");
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, beta,
+                           (double *)C, ldc);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, beta,
+                           (float *)C, ldc);
+    else {
+        set_error("omx_scale: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_set(omx_ctx ctx, int dt, char seC, int m, int n, double alpha, double beta, void *C, int64_t ldc) {
+    OMX_REQUIRE(ctx && m >= 0 && n >= 0 && ldc >= std::max(1, m), "omx_set: bad arguments");
+    if (!m || !n) return OMX_OK;
+    OMX_REQUIRE(C, "omx_set: null operand");
+    int lu = (seC == 'l' || seC == 'L') ? 2 : ((seC == 'u' || seC == 'U') ? 1 : 0);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_set<double>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, lu, m, n, alpha,
+                           beta, (double *)C, ldc);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_set<float>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, lu, m, n, alpha,
+                           beta, (float *)C, ldc);
+    else {
+        set_error("omx_set: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_trace(omx_ctx ctx, int dt, int n, const void *A, int64_t lda, double *out) {
+    OMX_REQUIRE(ctx && out && n >= 0 && lda >= std::max(1, n), "omx_trace: bad arguments");
+    if (!n) {
+        *out = 0.0;
+        return OMX_OK;
+    }
+    OMX_REQUIRE(A, "omx_trace: null operand");
+    int blocks = std::min(256, (n + EW_NT - 1) / EW_NT);
+    int rc = ensure_ws(ctx, sizeof(double) * blocks);
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws;
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_trace_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, n, (const double *)A,
+                           lda, partial);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_trace_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, n, (const float *)A, lda,
+                           partial);
+    else {
+        set_error("omx_trace: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    rc = finish_reduce<1>(ctx, blocks, partial);
+    if (rc) return rc;
+    *out = ctx->h_scalars[0];
+    return OMX_OK;
+}
+
+int omx_mm_trace(omx_ctx ctx, int dt, int m, int n, const void *A, int64_t lda, const void *B, int64_t ldb,
+                 double *out) {
+    OMX_REQUIRE(ctx && out && m >= 0 && n >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, m),
+                "omx_mm_trace: bad arguments");
+    if (!m || !n) {
+        *out = 0.0;
+        return OMX_OK;
+    }
+    OMX_REQUIRE(A && B, "omx_mm_trace: null operand");
+    int blocks = std::min(2048, grid_for((int64_t)m * n));
+    int rc = ensure_ws(ctx, sizeof(double) * blocks);
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_DOT, &ev);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_dot_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, m, n, (const double *)A,
+                           lda, (const double *)B, ldb, partial);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_dot_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, m, n, (const float *)A, lda,
+                           (const float *)B, ldb, partial);
+    else {
+        set_error("omx_mm_trace: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    prof_end(ctx, OMX_K_DOT, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    rc = finish_reduce<1>(ctx, blocks, partial);
+    if (rc) return rc;
+    *out = ctx->h_scalars[0];
+    return OMX_OK;
+}
+
+int omx_omm_coeff(omx_ctx ctx, int dt, int n, int64_t ldm, const void *HW, const void *SW, const void *HWd,
+                  const void *SWd, const void *HWdd, const void *SWdd, double *coeff) {
+    OMX_REQUIRE(ctx && coeff && n > 0 && ldm >= n, "omx_omm_coeff: bad arguments");
+    OMX_REQUIRE(HW && SW && HWd && SWd && HWdd && SWdd, "omx_omm_coeff: null operand");
+    int tiles = (n + TT - 1) / TT;
+    int64_t blocks = (int64_t)tiles * tiles;
+    OMX_REQUIRE(blocks < (1LL << 31), "omx_omm_coeff: n too large");
+    int rc = ensure_ws(ctx, sizeof(double) * 13 * blocks);
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_COEFF, &ev);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_coeff_partial<double>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
+                           (const double *)HW, (const double *)SW, (const double *)HWd, (const double *)SWd,
+                           (const double *)HWdd, (const double *)SWdd, partial);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_coeff_partial<float>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
+                           (const float *)HW, (const float *)SW, (const float *)HWd, (const float *)SWd,
+                           (const float *)HWdd, (const float *)SWdd, partial);
+    else {
+        set_error("omx_omm_coeff: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_coeff_final, dim3(1), dim3(EW_NT), 0, ctx->stream, (int)blocks, partial, ctx->d_scalars);
+    prof_end(ctx, OMX_K_COEFF, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    rc = read_scalars(ctx, 18);
+    if (rc) return rc;
+    for (int q = 0; q < 5; ++q) coeff[q] = ctx->h_scalars[q];
+    return OMX_OK;
+}
+
+int omx_omm_pr_dots(omx_ctx ctx, int dt, int n, int m, const void *G, const void *G_p, int64_t ld, double *out) {
+    OMX_REQUIRE(ctx && out && n > 0 && m > 0 && ld >= n && G && G_p, "omx_omm_pr_dots: bad arguments");
+    int64_t total = (int64_t)n * m;
+    int blocks = std::min(2048, grid_for(total));
+    int rc = ensure_ws(ctx, sizeof(double) * 2 * blocks);
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_DOT, &ev);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_pr_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n,
+                           (const double *)G, (const double *)G_p, ld, partial);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_pr_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n, (const float *)G,
+                           (const float *)G_p, ld, partial);
+    else {
+        set_error("omx_omm_pr_dots: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    prof_end(ctx, OMX_K_DOT, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    rc = finish_reduce<2>(ctx, blocks, partial);
+    if (rc) return rc;
+    out[0] = ctx->h_scalars[0];
+    out[1] = ctx->h_scalars[1];
+    return OMX_OK;
+}
+
+int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const void *G, void *D, void *G_p,
+                         int64_t ld) {
+    OMX_REQUIRE(ctx && n > 0 && m > 0 && ld >= n && G && D && G_p, "omx_omm_cg_direction: bad arguments");
+    int grid = grid_for((int64_t)n * m);
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_ELEM, &ev);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_cg_direction<double>, dim3(grid), dim3(EW_NT), 0, ctx->stream, n, m, lambda,
+                           (const double *)G, (double *)D, (double *)G_p, ld);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_cg_direction<float>, dim3(grid), dim3(EW_NT), 0, ctx->stream, n, m, lambda,
+                           (const float *)G, (float *)D, (float *)G_p, ld);
+    else {
+        set_error("omx_omm_cg_direction: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    prof_end(ctx, OMX_K_ELEM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const void *Wd, const void *Wdd, void *W) {
+    OMX_REQUIRE(ctx && n > 0 && ld >= n && Wd && Wdd && W, "omx_omm_update_w: bad arguments");
+    OMX_REQUIRE(W != Wd && W != Wdd, "omx_omm_update_w: W must not alias Wd or Wdd");
+    dim3 grid((n + TT - 1) / TT, (n + TT - 1) / TT);
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_ELEM, &ev);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_update_w<double>, grid, dim3(EW_NT), 0, ctx->stream, n, ld, x, (const double *)Wd,
+                           (const double *)Wdd, (double *)W);
+    else if (dt == OMX_F32)
+        hipLaunchKernelGGL(k_update_w<float>, grid, dim3(EW_NT), 0, ctx->stream, n, ld, x, (const float *)Wd,
+                           (const float *)Wdd, (float *)W);
+    else {
+        set_error("omx_omm_update_w: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    prof_end(ctx, OMX_K_ELEM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+}  // extern "C"

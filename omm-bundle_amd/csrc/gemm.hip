@@ -1,0 +1,459 @@
+// Dense products on the CDNA4 matrix cores: C := alpha*op(A)*op(B) + beta*C.
+//
+// One kernel template covers the OMM products
+//   Gram      (No x Nb)(Nb x No)  -> op = ('n','t'): HWd = HC*D^T, SWd = SC*D^T, HWdd = HG*D^T, ...
+//   gradient  (No x No)(No x Nb)  -> op = ('n','n'): SW*HC, HW*SC (fused pair: calc_G)
+//   density   (Nb x No)(No x Nb)  -> op = ('t','n'): D_min = C^T*(QW*C)
+// and the three precisions:
+//   FP64: v_mfma_f64_16x16x4f64, FP64 accumulate (the oracle-grade path);
+//   FP32: operands rounded to FP32, v_mfma_f32_16x16x4f32, FP32 accumulate;
+//   BF16: operands rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulate.
+//
+// Work decomposition: 256-thread workgroups (4 wave64s, 2x2) own a BM x BN tile of C; each wave
+// owns a (BM/2) x (BN/2) sub-tile held as 16x16 MFMA accumulator blocks.  K is consumed in BK
+// slices staged through LDS (double-buffered: the next slice is fetched into registers while
+// the current one feeds the MFMAs).  Large K with few output tiles is split over grid.z into
+// FP64/FP32 partial tiles that a second kernel reduces in a fixed order (deterministic).
+// The linear workgroup id is remapped so that each XCD (ids are dealt round-robin over the 8
+// XCDs) walks a contiguous band of tile rows, keeping its A panels in its own L2.
+#include <algorithm>
+
+#include "omx_internal.h"
+
+namespace omx {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+template <typename T>
+__device__ inline double to_d(T v) { return (double)v; }
+
+// ---------------------------------------------------------------------------------------------
+// Tile geometry
+// ---------------------------------------------------------------------------------------------
+constexpr int BM = 128, BN = 128;
+constexpr int WM = 64, WN = 64;          // per-wave sub-tile
+constexpr int NT = 256;                  // threads per workgroup
+constexpr int FB = WM / 16;              // 16x16 blocks per wave along M (and N)
+
+// Per-precision traits: K slice, LDS element type, and the MFMA step.
+template <int PREC>
+struct Prec;
+
+template <>
+struct Prec<OMX_PREC_F64> {
+    static constexpr int BK = 16;
+    typedef double lds_t;
+    typedef double acc_t;
+    typedef d4v accv;
+    static constexpr int KSTEP = 4;
+};
+template <>
+struct Prec<OMX_PREC_F32> {
+    static constexpr int BK = 32;
+    typedef float lds_t;
+    typedef float acc_t;
+    typedef f4v accv;
+    static constexpr int KSTEP = 4;
+};
+template <>
+struct Prec<OMX_PREC_BF16> {
+    static constexpr int BK = 64;
+    typedef __bf16 lds_t;
+    typedef float acc_t;
+    typedef f4v accv;
+    static constexpr int KSTEP = 32;
+};
+
+template <typename TS, int PREC>
+__device__ inline typename Prec<PREC>::lds_t to_lds(TS v) {
+    return (typename Prec<PREC>::lds_t)v;
+}
+template <>
+__device__ inline __bf16 to_lds<double, OMX_PREC_BF16>(double v) { return (__bf16)(float)v; }
+template <>
+__device__ inline __bf16 to_lds<float, OMX_PREC_BF16>(float v) { return (__bf16)v; }
+
+// LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
+// lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
+// consecutive k values per MFMA step as one 16-byte read.
+template <int PREC>
+struct Lds {
+    static constexpr int BK = Prec<PREC>::BK;
+    static constexpr bool KMAJOR = (PREC != OMX_PREC_BF16);
+    static constexpr int PAD = KMAJOR ? 4 : 8;
+    // k-major: [BK][BM+PAD]; i-major: [BM][BK+PAD]
+    static constexpr int A_ELEMS = KMAJOR ? BK * (BM + PAD) : BM * (BK + PAD);
+    static constexpr int B_ELEMS = KMAJOR ? BK * (BN + PAD) : BN * (BK + PAD);
+    __device__ static inline int a_off(int i, int k) { return KMAJOR ? k * (BM + PAD) + i : i * (BK + PAD) + k; }
+    __device__ static inline int b_off(int j, int k) { return KMAJOR ? k * (BN + PAD) + j : j * (BK + PAD) + k; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Global -> register staging of one BK slice of op(A) (BM x BK) or op(B)^T (BN x BK).
+// "Row" index r (i for A, j for B) and k; CONTIG_R: the row index is contiguous in memory
+// (A 'n', B 't'); otherwise k is contiguous (A 't', B 'n').  Each thread loads ELEMS values.
+// ---------------------------------------------------------------------------------------------
+template <typename TS, int PREC, bool CONTIG_R, int ROWS>
+struct Stage {
+    static constexpr int BK = Prec<PREC>::BK;
+    static constexpr int ELEMS = ROWS * BK / NT;
+    typename Prec<PREC>::lds_t v[ELEMS];
+
+    // Element e of this thread -> (r, k) within the slice.  For CONTIG_R the fastest index
+    // across threads is r (coalesced along r); otherwise k.  For BF16 with CONTIG_R, each thread
+    // holds KSEG consecutive k of one r so that the LDS write ([r][k] layout) is one vector store.
+    __device__ static inline void coord(int t, int e, int &r, int &k) {
+        if (PREC == OMX_PREC_BF16 && CONTIG_R) {
+            // ELEMS = ROWS*BK/NT; thread t owns row r = t % ROWS (x ROWS/NT groups) and a run of k.
+            constexpr int RUNS = NT / ROWS;  // threads per row
+            constexpr int RUN = BK / RUNS;   // consecutive k per thread (== ELEMS)
+            r = t % ROWS;
+            k = (t / ROWS) * RUN + e;
+        } else if (CONTIG_R) {
+            int idx = t + NT * e;
+            r = idx % ROWS;
+            k = idx / ROWS;
+        } else {
+            int idx = t + NT * e;
+            k = idx % BK;
+            r = idx / BK;
+        }
+    }
+
+    __device__ inline void load(const TS *__restrict__ P, int64_t ld, int r0, int k0, int rlim, int klim, int t) {
+#pragma unroll
+        for (int e = 0; e < ELEMS; ++e) {
+            int r, k;
+            coord(t, e, r, k);
+            int gr = r0 + r, gk = k0 + k;
+            TS x = (TS)0;
+            if (gr < rlim && gk < klim) x = CONTIG_R ? P[(int64_t)gr + (int64_t)gk * ld] : P[(int64_t)gk + (int64_t)gr * ld];
+            v[e] = to_lds<TS, PREC>(x);
+        }
+    }
+
+    template <bool IS_A>
+    __device__ inline void store(typename Prec<PREC>::lds_t *lds, int t) const {
+#pragma unroll
+        for (int e = 0; e < ELEMS; ++e) {
+            int r, k;
+            coord(t, e, r, k);
+            lds[IS_A ? Lds<PREC>::a_off(r, k) : Lds<PREC>::b_off(r, k)] = v[e];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// MFMA step over one BK slice held in LDS.
+// ---------------------------------------------------------------------------------------------
+template <int PREC>
+__device__ inline void mma_slice(const typename Prec<PREC>::lds_t *As, const typename Prec<PREC>::lds_t *Bs,
+                                 typename Prec<PREC>::accv (&acc)[FB][FB], int wm, int wn, int lane) {
+    constexpr int BK = Prec<PREC>::BK;
+    const int li = lane & 15, lk = lane >> 4;
+    if constexpr (PREC == OMX_PREC_F64) {
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double a[FB], b[FB];
+#pragma unroll
+            for (int f = 0; f < FB; ++f) {
+                a[f] = As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + lk)];
+                b[f] = Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + lk)];
+            }
+#pragma unroll
+            for (int fm = 0; fm < FB; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FB; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+        }
+    } else if constexpr (PREC == OMX_PREC_F32) {
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            float a[FB], b[FB];
+#pragma unroll
+            for (int f = 0; f < FB; ++f) {
+                a[f] = As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + lk)];
+                b[f] = Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + lk)];
+            }
+#pragma unroll
+            for (int fm = 0; fm < FB; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FB; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
+            bf8v a[FB], b[FB];
+#pragma unroll
+            for (int f = 0; f < FB; ++f) {
+                a[f] = *reinterpret_cast<const bf8v *>(&As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + 8 * lk)]);
+                b[f] = *reinterpret_cast<const bf8v *>(&Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + 8 * lk)]);
+            }
+#pragma unroll
+            for (int fm = 0; fm < FB; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FB; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+        }
+    }
+}
+
+// Tile index for workgroup id w: XCD-contiguous bands of tile rows, grouped raster inside.
+__device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn) {
+    int T = tiles_m * tiles_n;
+    int t = w;
+    if ((T & 7) == 0) t = (w & 7) * (T >> 3) + (w >> 3);
+    constexpr int GM = 8;
+    int group = t / (GM * tiles_n);
+    int first_m = group * GM;
+    int gm = min(tiles_m - first_m, GM);
+    int in = t % (GM * tiles_n);
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+
+// Pair mode (NPAIR = 2): C := alpha*(A1*B1 + A2*B2) + beta0*C0 + beta*C over a K range made of
+// the two contractions back to back (calc_G).  NPAIR = 1: the plain product.
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+__global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, const TS *__restrict__ A1,
+                                             const TS *__restrict__ A2, int64_t lda, const TS *__restrict__ B1,
+                                             const TS *__restrict__ B2, int64_t ldb, double beta0,
+                                             const TS *__restrict__ C0, double beta, TS *__restrict__ C, int64_t ldc,
+                                             int k_per_split, typename Prec<PREC>::acc_t *__restrict__ partial) {
+    typedef typename Prec<PREC>::lds_t LT;
+    typedef typename Prec<PREC>::acc_t AT;
+    typedef typename Prec<PREC>::accv AV;
+    constexpr int BK = Prec<PREC>::BK;
+    __shared__ LT smem[2 * (Lds<PREC>::A_ELEMS + Lds<PREC>::B_ELEMS)];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int i0 = tm * BM, j0 = tn * BN;
+
+    // K range of this split over the virtual contraction [0, NPAIR*k).
+    const int ktot = NPAIR * k;
+    const int kb = blockIdx.z * k_per_split;
+    const int ke = min(ktot, kb + k_per_split);
+
+    AV acc[FB][FB];
+#pragma unroll
+    for (int a = 0; a < FB; ++a)
+#pragma unroll
+        for (int b = 0; b < FB; ++b) acc[a][b] = AV{0, 0, 0, 0};
+
+    Stage<TS, PREC, !TRA, BM> sa;
+    Stage<TS, PREC, TRB, BN> sb;
+
+    auto fetch = [&](int kk) {
+        // kk is a virtual K index; pick the operand pair it falls in.
+        const TS *Ap = A1, *Bp = B1;
+        int kl = kk;
+        if (NPAIR == 2 && kk >= k) {
+            Ap = A2;
+            Bp = B2;
+            kl = kk - k;
+        }
+        int klim = min(k, kl + (ke - kk));  // stop at the split end and at the pair end
+        sa.load(Ap, lda, i0, kl, m, klim, t);
+        sb.load(Bp, ldb, j0, kl, n, klim, t);
+    };
+
+    LT *As0 = smem, *Bs0 = smem + Lds<PREC>::A_ELEMS;
+    LT *As1 = smem + Lds<PREC>::A_ELEMS + Lds<PREC>::B_ELEMS, *Bs1 = As1 + Lds<PREC>::A_ELEMS;
+
+    // In pair mode the host guarantees k % BK == 0, so no slice straddles the pair boundary.
+    int nslices = (ke - kb + BK - 1) / BK;
+    if (nslices > 0) {
+        fetch(kb);
+        sa.template store<true>(As0, t);
+        sb.template store<false>(Bs0, t);
+        __syncthreads();
+        for (int s = 0; s < nslices; ++s) {
+            LT *Ac = (s & 1) ? As1 : As0, *Bc = (s & 1) ? Bs1 : Bs0;
+            LT *An = (s & 1) ? As0 : As1, *Bn = (s & 1) ? Bs0 : Bs1;
+            bool more = (s + 1 < nslices);
+            if (more) fetch(kb + (s + 1) * BK);
+            mma_slice<PREC>(Ac, Bc, acc, wm, wn, lane);
+            if (more) {
+                sa.template store<true>(An, t);
+                sb.template store<false>(Bn, t);
+            }
+            __syncthreads();
+        }
+    }
+
+    // Epilogue.  Accumulator v of block (fm, fn): row 4*(lane/16)+v, column lane%16.
+    const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int fm = 0; fm < FB; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FB; ++fn) {
+            int j = j0 + wn * WN + fn * 16 + li;
+            if (j >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int i = i0 + wm * WM + fm * 16 + 4 * lq + v;
+                if (i >= m) continue;
+                AT val = acc[fm][fn][v];
+                if (partial) {
+                    partial[(int64_t)blockIdx.z * m * n + (int64_t)i + (int64_t)j * m] = val;
+                } else {
+                    double r = alpha * (double)val;
+                    if (C0) r += beta0 * to_d(C0[(int64_t)i + (int64_t)j * ldc]);
+                    if (beta != 0.0) r += beta * to_d(C[(int64_t)i + (int64_t)j * ldc]);
+                    C[(int64_t)i + (int64_t)j * ldc] = (TS)r;
+                }
+            }
+        }
+}
+
+// Reduction of split-K partials in split order, then the alpha/beta epilogue.
+template <typename TS, typename AT>
+__global__ void k_splitk_reduce(int m, int n, int splits, const AT *__restrict__ partial, double alpha, double beta0,
+                                const TS *__restrict__ C0, double beta, TS *__restrict__ C, int64_t ldc) {
+    int64_t total = (int64_t)m * n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = q % m, j = q / m;
+        AT s = partial[q];
+        for (int z = 1; z < splits; ++z) s += partial[(int64_t)z * total + q];
+        double r = alpha * (double)s;
+        if (C0) r += beta0 * (double)C0[i + j * ldc];
+        if (beta != 0.0) r += beta * (double)C[i + j * ldc];
+        C[i + j * ldc] = (TS)r;
+    }
+}
+
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
+                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
+                    int64_t ldc) {
+    typedef typename Prec<PREC>::acc_t AT;
+    constexpr int BK = Prec<PREC>::BK;
+    const int tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    const int ktot = NPAIR * k;
+    // Split K when the tile count cannot fill the chip (aim for >= 2 workgroups per CU).
+    int splits = 1;
+    const int target = 2 * ctx->num_cus;
+    if (tiles < target) {
+        int max_splits = std::max(1, ktot / (4 * BK));
+        splits = std::min(max_splits, (target + tiles - 1) / tiles);
+        splits = std::max(1, std::min(splits, 64));
+    }
+    int kps = (ktot + splits - 1) / splits;
+    kps = (kps + BK - 1) / BK * BK;  // slices never straddle a split
+    splits = (ktot + kps - 1) / kps;
+    AT *partial = nullptr;
+    if (splits > 1) {
+        size_t bytes = sizeof(AT) * (size_t)splits * m * n;
+        int rc = ensure_ws(ctx, bytes);
+        if (rc) return rc;
+        partial = (AT *)ctx->ws;
+    }
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_GEMM, &ev);
+    hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR>), dim3(tiles, 1, splits), dim3(NT), 0, ctx->stream, m, n, k,
+                       alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, partial);
+    if (splits > 1) {
+        int64_t total = (int64_t)m * n;
+        int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_splitk_reduce<TS, AT>), dim3(grid), dim3(256), 0, ctx->stream, m, n, splits, partial,
+                           alpha, beta0, C0, beta, C, ldc);
+    }
+    prof_end(ctx, OMX_K_GEMM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename TS, int PREC>
+static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
+                        int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
+    const TS *a = (const TS *)A, *b = (const TS *)B;
+    TS *c = (TS *)C;
+    if (!trA && !trB) return run_gemm<TS, PREC, false, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    if (!trA && trB) return run_gemm<TS, PREC, false, true, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    if (trA && !trB) return run_gemm<TS, PREC, true, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    return run_gemm<TS, PREC, true, true, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+}
+
+int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
+                int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
+    if (dt == OMX_F64) {
+        if (prec == OMX_PREC_F64) return dispatch_ops<double, OMX_PREC_F64>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_F32) return dispatch_ops<double, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_BF16) return dispatch_ops<double, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    } else if (dt == OMX_F32) {
+        if (prec == OMX_PREC_F32) return dispatch_ops<float, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_BF16) return dispatch_ops<float, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    }
+    set_error("omx_gemm: unsupported (dt=%d, prec=%d)", dt, prec);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+// C := alpha*(A1*B1 + A2*B2) + beta0*C0 + beta*C with A* m x k ('n'), B* k x n ('n').
+int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
+                     const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
+                     const void *C0, double beta, void *C, int64_t ldc) {
+    int bk = (prec == OMX_PREC_F64) ? Prec<OMX_PREC_F64>::BK
+             : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK
+                                      : Prec<OMX_PREC_BF16>::BK;
+    // The pair boundary must fall on a K-slice boundary; otherwise the caller runs separate passes.
+    if (k % bk != 0) return OMX_ERR_UNSUPPORTED;
+    if (dt == OMX_F64) {
+        const double *a1 = (const double *)A1, *a2 = (const double *)A2, *b1 = (const double *)B1,
+                     *b2 = (const double *)B2, *c0 = (const double *)C0;
+        if (prec == OMX_PREC_F64)
+            return run_gemm<double, OMX_PREC_F64, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+        if (prec == OMX_PREC_F32)
+            return run_gemm<double, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+        if (prec == OMX_PREC_BF16)
+            return run_gemm<double, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+    } else if (dt == OMX_F32) {
+        const float *a1 = (const float *)A1, *a2 = (const float *)A2, *b1 = (const float *)B1,
+                    *b2 = (const float *)B2, *c0 = (const float *)C0;
+        if (prec == OMX_PREC_F32)
+            return run_gemm<float, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (float *)C, ldc);
+        if (prec == OMX_PREC_BF16)
+            return run_gemm<float, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (float *)C, ldc);
+    }
+    set_error("calc_G: unsupported (dt=%d, prec=%d)", dt, prec);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+}  // namespace omx
+
+using namespace omx;
+
+extern "C" int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha,
+                        const void *A, int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
+    OMX_REQUIRE(ctx, "omx_gemm: null context");
+    OMX_REQUIRE(valid_op(opA) && valid_op(opB), "omx_gemm: invalid op ('%c','%c')", opA, opB);
+    OMX_REQUIRE(m >= 0 && n >= 0 && k >= 0, "omx_gemm: negative dimension");
+    bool trA = is_trans(opA), trB = is_trans(opB);
+    OMX_REQUIRE(lda >= std::max(1, trA ? k : m) && ldb >= std::max(1, trB ? n : k) && ldc >= std::max(1, m),
+                "omx_gemm: leading dimension too small");
+    if (m == 0 || n == 0) return OMX_OK;
+    if (k == 0) return omx_scale(ctx, dt, m, n, beta, C, ldc);
+    OMX_REQUIRE(A && B && C, "omx_gemm: null operand");
+    return launch_gemm(ctx, prec, dt, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int omx_omm_calc_G(omx_ctx ctx, int prec, int dt, int n, int m, const void *HW, const void *SW,
+                              int64_t ldw, const void *HC, const void *SC, void *G, int64_t ldc) {
+    OMX_REQUIRE(ctx && HW && SW && HC && SC && G, "omx_omm_calc_G: null argument");
+    OMX_REQUIRE(n > 0 && m > 0 && ldw >= n && ldc >= n, "omx_omm_calc_G: bad shape");
+    // G = 4*HC - 2*(SW*HC + HW*SC): pair product with the 4*HC term folded into the epilogue.
+    int rc = launch_gemm_pair(ctx, prec, dt, n, m, n, -2.0, SW, HW, ldw, HC, SC, ldc, 4.0, HC, 0.0, G, ldc);
+    if (rc != OMX_ERR_UNSUPPORTED) return rc;
+    // Unaligned n: three passes (G = 4HC; G -= 2 SW HC; G -= 2 HW SC), as calc_G in omm_ops.F90:35-37.
+    rc = omx_add(ctx, dt, 'n', n, m, 4.0, HC, ldc, 0.0, G, ldc);
+    if (rc) return rc;
+    rc = launch_gemm(ctx, prec, dt, false, false, n, m, n, -2.0, SW, ldw, HC, ldc, 1.0, G, ldc);
+    if (rc) return rc;
+    return launch_gemm(ctx, prec, dt, false, false, n, m, n, -2.0, HW, ldw, SC, ldc, 1.0, G, ldc);
+}

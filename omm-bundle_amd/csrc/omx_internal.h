@@ -1,0 +1,111 @@
+// Internal definitions shared by the libommhip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "omm_hip.h"
+
+namespace omx {
+
+// Per-thread last error message (omx_last_error()).
+void set_error(const char *fmt, ...);
+void clear_error();
+
+#define OMX_CHECK_HIP(expr)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            omx::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,    \
+                           __LINE__);                                                          \
+            return OMX_ERR_HIP;                                                                \
+        }                                                                                      \
+    } while (0)
+
+#define OMX_REQUIRE(cond, ...)                                                                 \
+    do {                                                                                       \
+        if (!(cond)) {                                                                         \
+            omx::set_error(__VA_ARGS__);                                                       \
+            return OMX_ERR_ARG;                                                                \
+        }                                                                                      \
+    } while (0)
+
+// Event pair recorded around one launch of a timed kernel class.
+struct ProfEvent {
+    hipEvent_t start, stop;
+    int kclass;
+};
+
+}  // namespace omx
+
+struct omx_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // Device workspace for split-K partial products and reduction partials (grown on demand).
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    // Small device buffer for reduction results and pinned host mirror for scalar readback.
+    double *d_scalars = nullptr;
+    double *h_scalars = nullptr;
+    // Event timing.
+    bool prof_on = false;
+    std::vector<omx::ProfEvent> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[OMX_K_COUNT] = {0};
+    int64_t prof_count[OMX_K_COUNT] = {0};
+    int num_cus = 256;
+};
+
+struct omx_bsr_s {
+    omx_ctx ctx = nullptr;
+    int nb = 0;        // basis size
+    int nbc = 0;       // block columns = ceil(nb/4)
+    int64_t nnzb = 0;  // stored 4x4 blocks
+    int is_complex = 0;
+    int max_blocks_per_col = 0;
+    int *d_colptr = nullptr;  // nbc+1
+    int *d_rowidx = nullptr;  // nnzb
+    double *d_val = nullptr;  // 16*nnzb (x2 if complex)
+};
+
+namespace omx {
+
+constexpr int kScalarSlots = 256;
+
+// Ensure ctx->ws holds at least `bytes`.  Returns OMX_OK or an error.
+int ensure_ws(omx_ctx ctx, size_t bytes);
+
+// Begin / end a timed launch of kernel class `kclass` (no-ops unless profiling is on).
+void prof_begin(omx_ctx ctx, int kclass, hipEvent_t *ev_start);
+void prof_end(omx_ctx ctx, int kclass, hipEvent_t ev_start);
+
+inline bool valid_real_dt(int dt) { return dt == OMX_F64 || dt == OMX_F32; }
+inline int dt_size(int dt) {
+    switch (dt) {
+        case OMX_F64: return 8;
+        case OMX_F32: return 4;
+        case OMX_BF16: return 2;
+        case OMX_C128: return 16;
+        case OMX_C64: return 8;
+        default: return 0;
+    }
+}
+inline bool is_trans(char op) { return op == 't' || op == 'T' || op == 'c' || op == 'C'; }
+inline bool valid_op(char op) { return op == 'n' || op == 'N' || is_trans(op); }
+
+// Scalar readback: copy `n` doubles from ctx->d_scalars to ctx->h_scalars and wait.
+int read_scalars(omx_ctx ctx, int n);
+
+// Launch helpers implemented in the .hip files.
+int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha,
+                const void *A, int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
+                     const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
+                     const void *C0, double beta, void *C, int64_t ldc);
+
+}  // namespace omx

@@ -1,0 +1,385 @@
+"""ctypes binding of libommhip.so (the C-ABI declared in include/omm_hip.h).
+
+Used by the tests and the benchmark to call the HIP path directly; the Fortran host reaches the
+same entry points through ISO_C_BINDING.  Loading fails loudly when the library has not been
+built: there is no CPU fallback behind these functions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libommhip.so")
+
+OMX_F64, OMX_F32, OMX_BF16, OMX_C128, OMX_C64 = 0, 1, 2, 3, 4
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+K_SPMM, K_GEMM, K_COEFF, K_ELEM, K_DOT = 0, 1, 2, 3, 4
+
+_NP = {OMX_F64: np.float64, OMX_F32: np.float32, OMX_C128: np.complex128, OMX_C64: np.complex64}
+_ESIZE = {OMX_F64: 8, OMX_F32: 4, OMX_BF16: 2, OMX_C128: 16, OMX_C64: 8}
+
+_vp, _i, _i64, _d, _c = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_char
+_pd, _pi = C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "omx_device_count": (_i, [_pi]),
+    "omx_init": (_i, [_i, C.POINTER(_vp)]),
+    "omx_finalize": (_i, [_vp]),
+    "omx_sync": (_i, [_vp]),
+    "omx_last_error": (C.c_char_p, []),
+    "omx_stream": (_vp, [_vp]),
+    "omx_malloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "omx_free": (_i, [_vp, _vp]),
+    "omx_zero": (_i, [_vp, _vp, C.c_size_t]),
+    "omx_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "omx_d2h": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "omx_d2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "omx_h2d_2d": (_i, [_vp, _vp, _i64, _vp, _i64, _i, _i, _i]),
+    "omx_d2h_2d": (_i, [_vp, _vp, _i64, _vp, _i64, _i, _i, _i]),
+    "omx_convert": (_i, [_vp, _i, _i, _vp, _i, _i64, _vp, _i, _i64]),
+    "omx_bsr_create": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "omx_bsr_from_csc": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, C.POINTER(_vp)]),
+    "omx_bsr_destroy": (_i, [_vp]),
+    "omx_bsr_info": (_i, [_vp, _pi, C.POINTER(_i64), _pi]),
+    "omx_bsr_update_values": (_i, [_vp, _vp]),
+    "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
+    "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
+    "omx_add": (_i, [_vp, _i, _c, _i, _i, _d, _vp, _i64, _d, _vp, _i64]),
+    "omx_scale": (_i, [_vp, _i, _i, _i, _d, _vp, _i64]),
+    "omx_set": (_i, [_vp, _i, _c, _i, _i, _d, _d, _vp, _i64]),
+    "omx_trace": (_i, [_vp, _i, _i, _vp, _i64, _pd]),
+    "omx_mm_trace": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _i64, _pd]),
+    "omx_set_element": (_i, [_vp, _i, _vp, _i64, _i, _i, _d, _d]),
+    "omx_get_element": (_i, [_vp, _i, _vp, _i64, _i, _i, _pd]),
+    "omx_omm_coeff": (_i, [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _pd]),
+    "omx_omm_calc_G": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _vp, _i64]),
+    "omx_omm_pr_dots": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _pd]),
+    "omx_omm_cg_direction": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp, _i64]),
+    "omx_omm_update_w": (_i, [_vp, _i, _i, _i64, _d, _vp, _vp, _vp]),
+    "omx_prof_enable": (_i, [_vp, _i]),
+    "omx_prof_read": (_i, [_vp, _i, _pd, C.POINTER(_i64)]),
+    "omx_prof_reset": (_i, [_vp]),
+    "omx_lattice_bsr": (_i, [_i, _i, _i, C.c_uint64, _i, C.POINTER(_i64), _vp, _vp, _vp]),
+    "omx_lattice_bsr_theta": (_i, [_i, _i, _i, _vp, C.POINTER(_i64), _vp, _vp, _vp]),
+    "omx_lattice_theta0": (_i, [C.c_uint64, _vp]),
+    "omx_random_c0": (_i, [_i, _i, C.c_uint64, _vp, _i64]),
+    "omx_u01": (_d, [C.c_uint64, C.c_uint64, C.c_uint64]),
+}
+
+_lib = None
+
+
+class OmxError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libommhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OmxError(f"{LIB_PATH} not found: build it with omm-bundle_amd/build.sh")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().omx_last_error().decode(errors="replace")
+        raise OmxError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    check(lib().omx_device_count(C.byref(c)), "omx_device_count")
+    return c.value
+
+
+# ---------------------------------------------------------------------------------------------
+# synthetic lattice inputs (host)
+# ---------------------------------------------------------------------------------------------
+
+DEFAULT_SEED = 0x0EEDC0DE
+
+
+def lattice_bsr(L, which: int, seed: int = DEFAULT_SEED):
+    """BSR-4 arrays (bcolptr, browidx, bval[nnzb,16]) of H (which=0) or S (which=1)."""
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    nnzb = C.c_int64(0)
+    check(lib().omx_lattice_bsr(Lx, Ly, Lz, seed, which, C.byref(nnzb), None, None, None), "omx_lattice_bsr")
+    ns = Lx * Ly * Lz
+    cp = np.zeros(ns + 1, np.int32)
+    ri = np.zeros(nnzb.value, np.int32)
+    bv = np.zeros((nnzb.value, 16), np.float64)
+    check(lib().omx_lattice_bsr(Lx, Ly, Lz, seed, which, C.byref(nnzb), _ptr(cp), _ptr(ri), _ptr(bv)),
+          "omx_lattice_bsr")
+    return cp, ri, bv
+
+
+def lattice_bsr_theta(L, theta):
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    theta = np.ascontiguousarray(theta, np.float64)
+    nnzb = C.c_int64(0)
+    check(lib().omx_lattice_bsr_theta(Lx, Ly, Lz, _ptr(theta), C.byref(nnzb), None, None, None), "lattice theta")
+    ns = Lx * Ly * Lz
+    cp = np.zeros(ns + 1, np.int32)
+    ri = np.zeros(nnzb.value, np.int32)
+    bv = np.zeros((nnzb.value, 16), np.float64)
+    check(lib().omx_lattice_bsr_theta(Lx, Ly, Lz, _ptr(theta), C.byref(nnzb), _ptr(cp), _ptr(ri), _ptr(bv)),
+          "lattice theta")
+    return cp, ri, bv
+
+
+def lattice_theta0(seed: int = DEFAULT_SEED):
+    th = np.zeros(20, np.float64)
+    check(lib().omx_lattice_theta0(seed, _ptr(th)), "theta0")
+    return th
+
+
+def random_c0(no: int, nb: int, seed: int = DEFAULT_SEED):
+    c = np.zeros((no, nb), np.float64, order="F")
+    check(lib().omx_random_c0(no, nb, seed, _ptr(c), no), "omx_random_c0")
+    return c
+
+
+def u01(seed, stream, idx) -> float:
+    return lib().omx_u01(seed, stream, idx)
+
+
+def bsr_to_dense(nb: int, cp, ri, bv) -> np.ndarray:
+    """Dense nb x nb matrix of a BSR-4 operator (host, for checks)."""
+    nbp = 4 * (len(cp) - 1)
+    A = np.zeros((nbp, nbp))
+    for J in range(len(cp) - 1):
+        for l in range(cp[J], cp[J + 1]):
+            I = ri[l]
+            A[4 * I:4 * I + 4, 4 * J:4 * J + 4] = bv[l].reshape(4, 4, order="F")
+    return A[:nb, :nb]
+
+
+def bsr_to_csc(nb: int, cp, ri, bv):
+    """0-based CSC (colptr, rowidx, val) with all stored block entries, rows ascending."""
+    colptr = [0]
+    rows, vals = [], []
+    for j in range(nb):
+        J, c = divmod(j, 4)
+        for l in range(cp[J], cp[J + 1]):
+            I = ri[l]
+            for r in range(4):
+                i = 4 * I + r
+                if i < nb:
+                    rows.append(i)
+                    vals.append(bv[l][r + 4 * c])
+        colptr.append(len(rows))
+    return np.array(colptr, np.int32), np.array(rows, np.int32), np.array(vals, np.float64)
+
+
+# ---------------------------------------------------------------------------------------------
+# device context, matrices, operators
+# ---------------------------------------------------------------------------------------------
+
+
+class Ctx:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().omx_init(device, C.byref(h)), "omx_init")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().omx_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(lib().omx_sync(self.h), "omx_sync")
+
+    # -- profiling --
+    def prof(self, enable=True):
+        check(lib().omx_prof_enable(self.h, int(enable)), "prof_enable")
+
+    def prof_reset(self):
+        check(lib().omx_prof_reset(self.h), "prof_reset")
+
+    def prof_read(self, kclass):
+        ms, n = C.c_double(0), C.c_int64(0)
+        check(lib().omx_prof_read(self.h, kclass, C.byref(ms), C.byref(n)), "prof_read")
+        return ms.value, n.value
+
+
+class DMat:
+    """A device matrix (column-major, ld = rows), zero-initialised."""
+
+    def __init__(self, ctx: Ctx, rows: int, cols: int, dt: int = OMX_F64, ld: int | None = None):
+        self.ctx, self.rows, self.cols, self.dt = ctx, rows, cols, dt
+        self.ld = ld if ld is not None else max(rows, 1)
+        p = C.c_void_p()
+        check(lib().omx_malloc(ctx.h, self.ld * max(cols, 1) * _ESIZE[dt], C.byref(p)), "omx_malloc")
+        self.p = p
+
+    @classmethod
+    def from_host(cls, ctx, a: np.ndarray, dt: int = OMX_F64, ld: int | None = None):
+        m = cls(ctx, a.shape[0], a.shape[1], dt, ld)
+        m.upload(a)
+        return m
+
+    def upload(self, a: np.ndarray):
+        a = np.asfortranarray(a, dtype=_NP[self.dt])
+        assert a.shape == (self.rows, self.cols)
+        check(lib().omx_h2d_2d(self.ctx.h, self.p, self.ld, _ptr(a), max(self.rows, 1), self.rows, self.cols,
+                               _ESIZE[self.dt]), "h2d")
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((self.rows, self.cols), _NP[self.dt], order="F")
+        check(lib().omx_d2h_2d(self.ctx.h, _ptr(out), max(self.rows, 1), self.p, self.ld, self.rows, self.cols,
+                               _ESIZE[self.dt]), "d2h")
+        return out
+
+    def free(self):
+        if self.p:
+            lib().omx_free(self.ctx.h, self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            if self.p and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Bsr:
+    """A BSR-4 operator in HBM."""
+
+    def __init__(self, ctx: Ctx, nb: int, cp, ri, bv):
+        self.ctx, self.nb = ctx, nb
+        cp = np.ascontiguousarray(cp, np.int32)
+        ri = np.ascontiguousarray(ri, np.int32)
+        bv = np.ascontiguousarray(bv, np.float64)
+        h = C.c_void_p()
+        check(lib().omx_bsr_create(ctx.h, nb, _ptr(cp), _ptr(ri), _ptr(bv), C.byref(h)), "omx_bsr_create")
+        self.h = h
+        self.nnzb = len(ri)
+
+    @classmethod
+    def from_csc(cls, ctx, nb, colptr, rowidx, val, ptrbase=0, rowbase=0):
+        self = cls.__new__(cls)
+        self.ctx, self.nb = ctx, nb
+        colptr = np.ascontiguousarray(colptr, np.int32)
+        rowidx = np.ascontiguousarray(rowidx, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        h = C.c_void_p()
+        check(lib().omx_bsr_from_csc(ctx.h, nb, _ptr(colptr), _ptr(rowidx), _ptr(val), ptrbase, rowbase,
+                                     C.byref(h)), "omx_bsr_from_csc")
+        self.h = h
+        nnzb = C.c_int64(0)
+        check(lib().omx_bsr_info(h, None, C.byref(nnzb), None), "bsr_info")
+        self.nnzb = nnzb.value
+        return self
+
+    def free(self):
+        if self.h:
+            lib().omx_bsr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------------
+# thin op wrappers (raise on error)
+# ---------------------------------------------------------------------------------------------
+
+
+def spmm(ctx, X: DMat, A: Bsr, Y: DMat, alpha=1.0, beta=0.0):
+    check(lib().omx_spmm(ctx.h, X.dt, X.rows, X.cols, alpha, X.p, X.ld, A.h, beta, Y.p, Y.ld), "omx_spmm")
+
+
+def gemm(ctx, opA, opB, A: DMat, B: DMat, Cm: DMat, alpha=1.0, beta=0.0, prec=PREC_F64):
+    m = A.cols if opA in "tTcC" else A.rows
+    k = A.rows if opA in "tTcC" else A.cols
+    n = B.rows if opB in "tTcC" else B.cols
+    check(lib().omx_gemm(ctx.h, prec, A.dt, opA.encode(), opB.encode(), m, n, k, alpha, A.p, A.ld, B.p, B.ld, beta,
+                         Cm.p, Cm.ld), "omx_gemm")
+
+
+def add(ctx, opA, A: DMat, Cm: DMat, alpha, beta):
+    check(lib().omx_add(ctx.h, Cm.dt, opA.encode(), Cm.rows, Cm.cols, alpha, A.p, A.ld, beta, Cm.p, Cm.ld),
+          "omx_add")
+
+
+def scale(ctx, Cm: DMat, beta):
+    check(lib().omx_scale(ctx.h, Cm.dt, Cm.rows, Cm.cols, beta, Cm.p, Cm.ld), "omx_scale")
+
+
+def mset(ctx, seC, Cm: DMat, alpha, beta):
+    check(lib().omx_set(ctx.h, Cm.dt, seC.encode(), Cm.rows, Cm.cols, alpha, beta, Cm.p, Cm.ld), "omx_set")
+
+
+def trace(ctx, A: DMat) -> float:
+    out = C.c_double(0)
+    check(lib().omx_trace(ctx.h, A.dt, A.rows, A.p, A.ld, C.byref(out)), "omx_trace")
+    return out.value
+
+
+def mm_trace(ctx, A: DMat, B: DMat) -> float:
+    out = C.c_double(0)
+    check(lib().omx_mm_trace(ctx.h, A.dt, A.rows, A.cols, A.p, A.ld, B.p, B.ld, C.byref(out)), "omx_mm_trace")
+    return out.value
+
+
+def omm_coeff(ctx, HW, SW, HWd, SWd, HWdd, SWdd) -> np.ndarray:
+    out = np.zeros(5)
+    check(lib().omx_omm_coeff(ctx.h, HW.dt, HW.rows, HW.ld, HW.p, SW.p, HWd.p, SWd.p, HWdd.p, SWdd.p, _ptr(out)),
+          "omx_omm_coeff")
+    return out
+
+
+def omm_calc_G(ctx, HW, SW, HC, SC, G, prec=PREC_F64):
+    check(lib().omx_omm_calc_G(ctx.h, prec, HC.dt, HC.rows, HC.cols, HW.p, SW.p, HW.ld, HC.p, SC.p, G.p, G.ld),
+          "omx_omm_calc_G")
+
+
+def omm_pr_dots(ctx, G, Gp):
+    out = np.zeros(2)
+    check(lib().omx_omm_pr_dots(ctx.h, G.dt, G.rows, G.cols, G.p, Gp.p, G.ld, _ptr(out)), "omx_omm_pr_dots")
+    return out
+
+
+def omm_cg_direction(ctx, lam, G, D, Gp):
+    check(lib().omx_omm_cg_direction(ctx.h, G.dt, G.rows, G.cols, lam, G.p, D.p, Gp.p, G.ld), "cg_direction")
+
+
+def omm_update_w(ctx, x, Wd, Wdd, W):
+    check(lib().omx_omm_update_w(ctx.h, W.dt, W.rows, W.ld, x, Wd.p, Wdd.p, W.p), "update_w")
+
+
+def set_element(ctx, M: DMat, i, j, alpha, beta):
+    check(lib().omx_set_element(ctx.h, M.dt, M.p, M.ld, i, j, alpha, beta), "set_element")
+
+
+def get_element(ctx, M: DMat, i, j) -> float:
+    out = C.c_double(0)
+    check(lib().omx_get_element(ctx.h, M.dt, M.p, M.ld, i, j, C.byref(out)), "get_element")
+    return out.value

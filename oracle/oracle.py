@@ -1,0 +1,266 @@
+"""ctypes wrapper of the CPU oracle (liboommoracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle restates libOMM's omm() flavour 0 and MatrixSwitch's 'ref' operations in C
+(omm_oracle.c).  Only tests, the smoke check and the benchmark's CPU-baseline leg use it, as the
+checker; the HIP product path never calls it.  Also drives the reference serial build in
+oracle/_ref (ref_driver) when that has been built from the reference sources.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboommoracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+_vp, _i, _d, _c = C.c_void_p, C.c_int, C.c_double, C.c_char
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.or_gemm.argtypes = [_c, _c, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
+        L.or_spmm_csc.argtypes = [_c, _i, _i, _d, _vp, _i, _vp, _vp, _vp, _d, _vp, _i]
+        L.or_sddmm_csc.argtypes = [_c, _c, _i, _d, _vp, _i, _vp, _i, _i, _vp, _vp, _d, _vp]
+        L.or_add.argtypes = [_c, _i, _i, _d, _vp, _i, _d, _vp, _i]
+        L.or_scale.argtypes = [_i, _i, _d, _vp, _i]
+        L.or_set.argtypes = [_c, _i, _i, _d, _d, _vp, _i]
+        L.or_trace.argtypes = [_i, _vp, _i]
+        L.or_trace.restype = _d
+        L.or_mm_trace.argtypes = [_i, _i, _vp, _i, _vp, _i]
+        L.or_mm_trace.restype = _d
+        L.or_bsd_lcg.argtypes = [C.POINTER(C.c_int32)]
+        L.or_bsd_lcg.restype = _d
+        L.or_solve_quartic.argtypes = [_vp, C.POINTER(_d), C.POINTER(_i)]
+        L.or_calc_coeff.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.or_omm_create.argtypes = [_i]
+        L.or_omm_create.restype = _vp
+        L.or_omm_destroy.argtypes = [_vp]
+        L.or_omm_flat.argtypes = [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_d),
+                                  _vp, _i, _d, _vp, _i, _i, _i, _d, _i, _i, _vp, _i, _vp, _vp]
+        L.or_num_threads.restype = _i
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+# ---- MatrixSwitch 'ref' ops on numpy (Fortran-ordered) arrays --------------------------------
+
+def gemm(opA, opB, A, B, Cm, alpha=1.0, beta=0.0):
+    assert Cm.flags.f_contiguous and Cm.dtype == np.float64
+    A, B = _f(A), _f(B)
+    m, n = Cm.shape
+    k = A.shape[0] if opA in "tTcC" else A.shape[1]
+    lib().or_gemm(opA.encode(), opB.encode(), m, n, k, alpha, _p(A), A.shape[0], _p(B), B.shape[0], beta, _p(Cm),
+                  Cm.shape[0])
+    return Cm
+
+
+def spmm_csc(A, colptr, rowidx, val, Cm, alpha=1.0, beta=0.0, opA="n"):
+    A = _f(A)
+    colptr = np.ascontiguousarray(colptr, np.int32)
+    rowidx = np.ascontiguousarray(rowidx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    m, nb = Cm.shape
+    lib().or_spmm_csc(opA.encode(), m, nb, alpha, _p(A), A.shape[0], _p(colptr), _p(rowidx), _p(val), beta, _p(Cm),
+                      m)
+    return Cm
+
+
+def add(opA, A, Cm, alpha, beta):
+    A = _f(A)
+    m, n = Cm.shape
+    lib().or_add(opA.encode(), m, n, alpha, _p(A), A.shape[0], beta, _p(Cm), m)
+    return Cm
+
+
+def trace(A):
+    A = _f(A)
+    return lib().or_trace(A.shape[0], _p(A), A.shape[0])
+
+
+def mm_trace(A, B):
+    A, B = _f(A), _f(B)
+    return lib().or_mm_trace(A.shape[0], A.shape[1], _p(A), A.shape[0], _p(B), B.shape[0])
+
+
+def mset(seC, Cm, alpha, beta):
+    lib().or_set(seC.encode(), Cm.shape[0], Cm.shape[1], alpha, beta, _p(Cm), Cm.shape[0])
+    return Cm
+
+
+def bsd_lcg_seq(seed: int, count: int):
+    x = C.c_int32(seed)
+    out = []
+    for _ in range(count):
+        r = lib().or_bsd_lcg(C.byref(x))
+        out.append((x.value, r))
+    return out
+
+
+def solve_quartic(c):
+    c = np.ascontiguousarray(c, np.float64)
+    x, fail = C.c_double(0), C.c_int(0)
+    lib().or_solve_quartic(_p(c), C.byref(x), C.byref(fail))
+    return x.value, bool(fail.value)
+
+
+def calc_coeff(HW, SW, HWd, SWd, HWdd, SWdd):
+    n = HW.shape[0]
+    mats = [_f(M) for M in (HW, SW, HWd, SWd, HWdd, SWdd)]
+    work = np.zeros((n, n), order="F")
+    out = np.zeros(5)
+    lib().or_calc_coeff(n, *[_p(M) for M in mats], _p(work), _p(out))
+    return out
+
+
+def num_threads():
+    return lib().or_num_threads()
+
+
+# ---- omm() -------------------------------------------------------------------------------------
+
+class Omm:
+    """omm() flavour 0 with per-ip state, mirroring the reference call sequence."""
+
+    def __init__(self, np_: int = 1):
+        self.h = lib().or_omm_create(np_)
+
+    def close(self):
+        if self.h:
+            lib().or_omm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __call__(self, m, n, H, S, C_min, *, new_S=True, calc_ED=False, eta=0.0, init_C=True, ip=1, cg_tol=-1.0,
+                 dealloc=False, max_iter=0, want_D=True, log_cap=100000, flavour=0):
+        """H, S: dense (m x m array) or a CSC tuple (colptr, rowidx, val), 0-based; S may be None.
+        C_min: Fortran-ordered n x m array, updated in place.  Returns a dict."""
+        assert C_min.flags.f_contiguous and C_min.shape == (n, m)
+        keep = []
+
+        def op(A):
+            if A is None:
+                return 0, None, None, None, None
+            if isinstance(A, tuple):
+                cp, ri, v = (np.ascontiguousarray(A[0], np.int32), np.ascontiguousarray(A[1], np.int32),
+                             np.ascontiguousarray(A[2], np.float64))
+                keep.extend([cp, ri, v])
+                return 2, None, _p(cp), _p(ri), _p(v)
+            Ad = _f(A)
+            keep.append(Ad)
+            return 1, _p(Ad), None, None, None
+
+        hk, hd, hc, hr, hv = op(H)
+        sk, sd, sc, sr, sv = op(S)
+        D = np.zeros((m, m), order="F") if want_D else None
+        e = C.c_double(0)
+        elog = np.zeros(log_cap)
+        info = np.zeros(4, np.int32)
+        out = np.zeros(2)
+        rc = lib().or_omm_flat(self.h, m, n, hk, hd, hc, hr, hv, sk, sd, sc, sr, sv, int(new_S), C.byref(e), _p(D),
+                               int(calc_ED), eta, _p(C_min), int(init_C), flavour, ip, cg_tol, int(dealloc), max_iter,
+                               _p(elog), log_cap, _p(info), _p(out))
+        if rc != 0:
+            raise RuntimeError(f"or_omm failed with {rc}")
+        return {"e_min": e.value, "D": D, "icg": int(info[0]), "converged": bool(info[1]), "nfail": int(info[2]),
+                "e_log": elog[:int(info[3])].copy(), "TrQS": out[0]}
+
+
+# ---- the reference serial build (oracle/_ref) --------------------------------------------------
+
+def have_ref() -> bool:
+    return os.path.exists(REF_DRIVER)
+
+
+def run_ref(m, n, calls, s_present=True, sparse=False, workdir=None):
+    """Replay omm() calls through the reference build.  Each call is a dict with keys H (m x m),
+    S (m x m, if s_present), optional C (n x m, given to the solver), new_S, init_C, calc_ED,
+    dealloc, eta, cg_tol.  Returns a list of dicts (e_min, C, D) and the libOMM.log text."""
+    own = workdir is None
+    wd = tempfile.mkdtemp(prefix="omm_ref_") if own else workdir
+    fin, fout = os.path.join(wd, "in.bin"), os.path.join(wd, "out.bin")
+    with open(fin, "wb") as f:
+        f.write(struct.pack("=5i", m, n, len(calls), int(s_present), int(sparse)))
+        for c in calls:
+            give_C = int(c.get("C") is not None)
+            f.write(struct.pack("=6i", int(c.get("new_S", True)), int(c.get("init_C", False)),
+                                int(c.get("calc_ED", False)), int(c.get("dealloc", False)), give_C,
+                                int(c.get("flavour", 0))))
+            f.write(struct.pack("=2d", float(c.get("eta", 0.0)), float(c.get("cg_tol", -1.0))))
+            f.write(np.asfortranarray(c["H"], np.float64).tobytes(order="F"))
+            if s_present:
+                f.write(np.asfortranarray(c["S"], np.float64).tobytes(order="F"))
+            if give_C:
+                f.write(np.asfortranarray(c["C"], np.float64).tobytes(order="F"))
+    subprocess.run([REF_DRIVER, fin, fout], cwd=wd, check=True, capture_output=True)
+    raw = np.fromfile(fout, np.float64)
+    res, off = [], 0
+    for _ in calls:
+        e = raw[off]
+        off += 1
+        Cm = raw[off:off + n * m].reshape((n, m), order="F")
+        off += n * m
+        D = raw[off:off + m * m].reshape((m, m), order="F")
+        off += m * m
+        res.append({"e_min": float(e), "C": Cm.copy(), "D": D.copy()})
+    log = open(os.path.join(wd, "libOMM.log")).read() if os.path.exists(os.path.join(wd, "libOMM.log")) else ""
+    return res, log
+
+
+def icg_from_log(log: str):
+    """The 'minim: icg' counts printed by the reference (omm.F90:588), in call order."""
+    out = []
+    for line in log.splitlines():
+        if "minim: icg" in line:
+            out.append(int(line.split("=")[1].split("|")[0]))
+    return out
+
+
+def e_log_from_log(log: str):
+    """Per-step energies printed with long_out (omm.F90:504), grouped per call."""
+    calls, cur = [], None
+    for line in log.splitlines():
+        s = line.strip()
+        if s.startswith("| libOMM"):
+            cur = []
+            calls.append(cur)
+        elif cur is not None and s.startswith("|") and len(s.split()) == 6:
+            parts = s.strip("|").split()
+            try:
+                cur.append(float(parts[2]))
+            except ValueError:
+                pass
+    return calls
+
+
+def read_example_gamma(path):
+    """example_gamma.dat: header 'm n num', then 'i j H_ij S_ij' (1-based)."""
+    with open(path) as f:
+        m, n, num = (int(x) for x in f.readline().split())
+        H = np.zeros((m, m))
+        S = np.zeros((m, m))
+        for _ in range(num):
+            i, j, h, s = f.readline().split()
+            H[int(i) - 1, int(j) - 1] = float(h)
+            S[int(i) - 1, int(j) - 1] = float(s)
+    return m, n, H, S

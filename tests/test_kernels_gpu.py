@@ -1,0 +1,205 @@
+"""HIP kernels of libommhip.so against numpy FP64 (and rounded-operand) references.
+
+Each test prints the case it runs before the GPU call so that a failing run names it.
+Tolerances: FP64 paths 1e-12 relative to the operand scale (FMA and summation order differ from
+numpy); FP32 paths 1e-5; BF16 paths 2e-2 (8-bit mantissa operands, FP32 accumulation).
+"""
+import numpy as np
+import pytest
+
+import ommhip as oh
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(7)
+
+
+def _note(*a):
+    print("CASE", *a, flush=True)
+
+
+def _bf16(x):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    f = np.asarray(x, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+OPS = [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")]
+
+
+@pytest.mark.parametrize("opA,opB", OPS)
+@pytest.mark.parametrize("shape", [(64, 48, 40), (200, 130, 300), (256, 256, 1024), (1, 5, 3)])
+def test_gemm_f64(ctx, opA, opB, shape):
+    m, n, k = shape
+    _note("gemm_f64", opA, opB, shape)
+    A = rng.standard_normal((k, m) if opA == "t" else (m, k))
+    B = rng.standard_normal((n, k) if opB == "t" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    dA, dB, dC = oh.DMat.from_host(ctx, A), oh.DMat.from_host(ctx, B), oh.DMat.from_host(ctx, C0)
+    oh.gemm(ctx, opA, opB, dA, dB, dC, alpha=-2.0, beta=0.5)
+    ref = -2.0 * ((A.T if opA == "t" else A) @ (B.T if opB == "t" else B)) + 0.5 * C0
+    got = dC.download()
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.sqrt(k) * 4
+
+
+@pytest.mark.parametrize("prec", [oh.PREC_F32, oh.PREC_BF16])
+@pytest.mark.parametrize("opA,opB", [("n", "t"), ("n", "n"), ("t", "n")])
+def test_gemm_low_precision(ctx, prec, opA, opB):
+    m, n, k = 192, 160, 512
+    _note("gemm_lowp", prec, opA, opB)
+    A = rng.standard_normal((k, m) if opA == "t" else (m, k))
+    B = rng.standard_normal((n, k) if opB == "t" else (k, n))
+    dA, dB, dC = oh.DMat.from_host(ctx, A), oh.DMat.from_host(ctx, B), oh.DMat(ctx, m, n)
+    oh.gemm(ctx, opA, opB, dA, dB, dC, prec=prec)
+    rnd = (lambda x: x.astype(np.float32).astype(np.float64)) if prec == oh.PREC_F32 else _bf16
+    ref = (rnd(A).T if opA == "t" else rnd(A)) @ (rnd(B).T if opB == "t" else rnd(B))
+    got = dC.download()
+    tol = 1e-5 if prec == oh.PREC_F32 else 1e-4
+    assert np.max(np.abs(got - ref)) <= tol * np.sqrt(k)
+
+
+def test_gemm_f32_storage(ctx):
+    m, n, k = 130, 70, 256
+    _note("gemm_f32_storage")
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k)).astype(np.float32)
+    dA = oh.DMat.from_host(ctx, A, oh.OMX_F32)
+    dB = oh.DMat.from_host(ctx, B, oh.OMX_F32)
+    dC = oh.DMat(ctx, m, n, oh.OMX_F32)
+    oh.gemm(ctx, "n", "t", dA, dB, dC, prec=oh.PREC_F32)
+    ref = A.astype(np.float64) @ B.astype(np.float64).T
+    assert np.max(np.abs(dC.download() - ref)) <= 1e-4 * np.sqrt(k)
+
+
+@pytest.mark.parametrize("L,no", [(3, 54), (4, 128), (4, 127), (5, 250)])
+def test_spmm_lattice(ctx, L, no):
+    _note("spmm", L, no)
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    nb = 4 * L ** 3
+    H = oh.bsr_to_dense(nb, cp, ri, bv)
+    assert np.allclose(H, H.T)
+    X = rng.standard_normal((no, nb))
+    Y0 = rng.standard_normal((no, nb))
+    A = oh.Bsr(ctx, nb, cp, ri, bv)
+    dX, dY = oh.DMat.from_host(ctx, X), oh.DMat.from_host(ctx, Y0)
+    oh.spmm(ctx, dX, A, dY, alpha=1.5, beta=-0.5)
+    ref = 1.5 * X @ H - 0.5 * Y0
+    assert np.max(np.abs(dY.download() - ref)) < 1e-12 * 30
+
+
+def test_spmm_from_csc_general(ctx):
+    # non-multiple-of-4 basis and an irregular pattern, 1-based CSC as MatrixSwitch 'sdcsc'
+    nb, no = 37, 20
+    _note("spmm_csc_general", nb, no)
+    A = rng.standard_normal((nb, nb)) * (rng.random((nb, nb)) < 0.2)
+    cols = [np.nonzero(A[:, j])[0] for j in range(nb)]
+    colptr = np.cumsum([0] + [len(c) for c in cols]).astype(np.int32)
+    rows = np.concatenate(cols).astype(np.int32) + 1
+    vals = np.concatenate([A[c, j] for j, c in enumerate(cols)])
+    B = oh.Bsr.from_csc(ctx, nb, colptr, rows, vals, ptrbase=0, rowbase=1)
+    X = rng.standard_normal((no, nb))
+    dX, dY = oh.DMat.from_host(ctx, X), oh.DMat(ctx, no, nb)
+    oh.spmm(ctx, dX, B, dY)
+    assert np.max(np.abs(dY.download() - X @ A)) < 1e-12 * 30
+
+
+def test_spmm_f32(ctx):
+    L, no = 4, 256
+    _note("spmm_f32")
+    cp, ri, bv = oh.lattice_bsr(L, 1)
+    nb = 4 * L ** 3
+    S = oh.bsr_to_dense(nb, cp, ri, bv)
+    X = rng.standard_normal((no, nb)).astype(np.float32)
+    A = oh.Bsr(ctx, nb, cp, ri, bv)
+    dX = oh.DMat.from_host(ctx, X, oh.OMX_F32)
+    dY = oh.DMat(ctx, no, nb, oh.OMX_F32)
+    oh.spmm(ctx, dX, A, dY)
+    assert np.max(np.abs(dY.download() - X.astype(np.float64) @ S)) < 1e-5
+
+
+@pytest.mark.parametrize("op", ["n", "t"])
+def test_add_scale_set(ctx, op):
+    m, n = 70, 45
+    _note("add", op)
+    A = rng.standard_normal((n, m) if op == "t" else (m, n))
+    C0 = rng.standard_normal((m, n))
+    dA, dC = oh.DMat.from_host(ctx, A), oh.DMat.from_host(ctx, C0)
+    oh.add(ctx, op, dA, dC, 0.7, -1.25)
+    ref = -1.25 * C0 + 0.7 * (A.T if op == "t" else A)
+    assert np.max(np.abs(dC.download() - ref)) < 1e-14 * 10
+    oh.scale(ctx, dC, 0.25)
+    assert np.max(np.abs(dC.download() - 0.25 * ref)) < 1e-14 * 10
+    oh.mset(ctx, "a", dC, 0.0, 2.0)
+    E = np.zeros((m, n))
+    np.fill_diagonal(E, 2.0)
+    assert np.array_equal(dC.download(), E)
+    oh.mset(ctx, "u", dC, 3.0, 1.0)
+    E2 = E.copy()
+    for i in range(m):
+        for j in range(i, n):
+            E2[i, j] = 1.0 if i == j else 3.0
+    assert np.array_equal(dC.download(), E2)
+
+
+def test_traces_and_elements(ctx):
+    n = 300
+    _note("traces")
+    A, B = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    dA, dB = oh.DMat.from_host(ctx, A), oh.DMat.from_host(ctx, B)
+    assert abs(oh.trace(ctx, dA) - np.trace(A)) < 1e-12
+    assert abs(oh.mm_trace(ctx, dA, dB) - np.sum(A * B)) < 1e-10
+    oh.set_element(ctx, dA, 3, 5, 2.0, 0.5)
+    assert oh.get_element(ctx, dA, 3, 5) == 2.0 + 0.5 * A[2, 4]
+
+
+def _coeff_np(HW, SW, HWd, SWd, HWdd, SWdd):
+    TrH, TrHS = np.trace(HW), np.sum(HW * SW)
+    TrHd, TrHdS, TrHSd = np.trace(HWd), np.sum(SW * HWd), np.sum(HW * SWd)
+    TrHdd, TrHddS, TrHSdd = np.trace(HWdd), np.sum(HWdd * SW), np.sum(HW * SWdd)
+    TrHdSd, TrHdSdH = np.sum(SWd.T * HWd), np.sum(SWd * HWd)
+    TrHddSd, TrHdSdd, TrHddSdd = np.sum(HWdd * SWd), np.sum(SWdd * HWd), np.sum(HWdd * SWdd)
+    return np.array([2 * TrH - TrHS, 2 * (2 * TrHd - TrHdS - TrHSd),
+                     2 * (TrHdd - TrHdSd - TrHdSdH) - TrHddS - TrHSdd, -2 * (TrHddSd + TrHdSdd), -TrHddSdd])
+
+
+@pytest.mark.parametrize("n", [31, 64, 257])
+def test_omm_coeff(ctx, n):
+    _note("coeff", n)
+    mats = [rng.standard_normal((n, n)) for _ in range(6)]
+    d = [oh.DMat.from_host(ctx, M) for M in mats]
+    got = oh.omm_coeff(ctx, *d)
+    ref = _coeff_np(*mats)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-10 * n)
+
+
+@pytest.mark.parametrize("n,m", [(64, 128), (96, 200), (33, 70)])
+def test_calc_G(ctx, n, m):
+    _note("calc_G", n, m)
+    HW, SW = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    HC, SC = rng.standard_normal((n, m)), rng.standard_normal((n, m))
+    d = [oh.DMat.from_host(ctx, M) for M in (HW, SW, HC, SC)]
+    G = oh.DMat(ctx, n, m)
+    oh.omm_calc_G(ctx, d[0], d[1], d[2], d[3], G)
+    ref = 4 * HC - 2 * SW @ HC - 2 * HW @ SC
+    assert np.max(np.abs(G.download() - ref)) < 1e-12 * np.sqrt(n) * 10
+
+
+def test_cg_updates(ctx):
+    n, m = 50, 90
+    _note("cg_updates")
+    G, D = rng.standard_normal((n, m)), rng.standard_normal((n, m))
+    dG, dD, dGp = oh.DMat.from_host(ctx, G), oh.DMat.from_host(ctx, D), oh.DMat(ctx, n, m)
+    oh.omm_cg_direction(ctx, 0.3, dG, dD, dGp)
+    assert np.allclose(dD.download(), G + 0.3 * D, rtol=0, atol=1e-15)
+    assert np.array_equal(dGp.download(), G)
+    G2 = rng.standard_normal((n, m))
+    dG.upload(G2)
+    num, den = oh.omm_pr_dots(ctx, dG, dGp)
+    assert abs(num - np.sum(G2 * (G2 - G))) < 1e-10 and abs(den - np.sum(G * G)) < 1e-10
+    W, Wd, Wdd = (rng.standard_normal((n, n)) for _ in range(3))
+    dW, dWd, dWdd = (oh.DMat.from_host(ctx, M) for M in (W, Wd, Wdd))
+    oh.omm_update_w(ctx, -0.37, dWd, dWdd, dW)
+    ref = W - 0.37 * (Wd.T + Wd) + 0.37 ** 2 * Wdd
+    assert np.max(np.abs(dW.download() - ref)) < 1e-14 * 20
