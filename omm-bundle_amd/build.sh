@@ -1,15 +1,17 @@
 #!/bin/bash
-# Builds libommhip.so (HIP kernels + C-ABI) for gfx950 in-tree.
+# Builds, in-tree:
+#   libommhip.so  HIP kernels + C-ABI (hipcc, gfx950)
+#   libommf.so    Fortran MatrixSwitch / omm() drop-in over the C-ABI (amdflang)
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 ROOT="$(cd "$HERE/.." && pwd)"
-OUT="$HERE/libommhip.so"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
-SRC=("$HERE"/csrc/api.hip "$HERE"/csrc/gemm.hip "$HERE"/csrc/spmm.hip "$HERE"/csrc/elementwise.hip "$HERE"/csrc/lattice.cpp)
+FC="${FC_OMM:-/opt/rocm/bin/amdflang}"
 OBJDIR="$HERE/_build"
-mkdir -p "$OBJDIR"
+mkdir -p "$OBJDIR/f"
+SRC=("$HERE"/csrc/api.hip "$HERE"/csrc/gemm.hip "$HERE"/csrc/spmm.hip "$HERE"/csrc/elementwise.hip "$HERE"/csrc/lattice.cpp)
 FLAGS=(--offload-arch=gfx950 -O3 -fPIC -std=c++17 -I"$ROOT/include" -I"$HERE/csrc"
-       -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result)
+       -Xclang -target-feature -Xclang -packed-fp32-ops)
 pids=()
 objs=()
 for s in "${SRC[@]}"; do
@@ -23,6 +25,16 @@ for s in "${SRC[@]}"; do
   pids+=($!)
 done
 for p in "${pids[@]}"; do wait "$p"; done
-"$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$OUT.tmp" "${objs[@]}"
-mv "$OUT.tmp" "$OUT"
-echo "built $OUT"
+"$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$HERE/libommhip.so.tmp" "${objs[@]}"
+mv "$HERE/libommhip.so.tmp" "$HERE/libommhip.so"
+echo "built $HERE/libommhip.so"
+
+# Fortran drop-in (module order follows the USE graph)
+FOBJS=()
+for f in omx_capi MatrixSwitch omm_hip_ops omm omm_c; do
+  "$FC" -cpp -O2 -fPIC -J"$OBJDIR/f" -c "$HERE/fortran/$f.F90" -o "$OBJDIR/f/$f.o"
+  FOBJS+=("$OBJDIR/f/$f.o")
+done
+"$FC" -shared -fPIC -o "$HERE/libommf.so.tmp" "${FOBJS[@]}" -L"$HERE" -lommhip -Wl,-rpath,'$ORIGIN'
+mv "$HERE/libommf.so.tmp" "$HERE/libommf.so"
+echo "built $HERE/libommf.so"

@@ -48,6 +48,8 @@ struct Prec<OMX_PREC_F64> {
     typedef double acc_t;
     typedef d4v accv;
     static constexpr int KSTEP = 4;
+    // v_mfma_f64_16x16x4f64 result: lane l, register pair v -> row (l/16) + 4v, column l%16
+    __device__ static inline int out_row(int lq, int v) { return lq + 4 * v; }
 };
 template <>
 struct Prec<OMX_PREC_F32> {
@@ -56,6 +58,8 @@ struct Prec<OMX_PREC_F32> {
     typedef float acc_t;
     typedef f4v accv;
     static constexpr int KSTEP = 4;
+    // v_mfma_f32_16x16x4f32 result: lane l, register v -> row 4(l/16) + v, column l%16
+    __device__ static inline int out_row(int lq, int v) { return 4 * lq + v; }
 };
 template <>
 struct Prec<OMX_PREC_BF16> {
@@ -64,6 +68,8 @@ struct Prec<OMX_PREC_BF16> {
     typedef float acc_t;
     typedef f4v accv;
     static constexpr int KSTEP = 32;
+    // v_mfma_f32_16x16x32_bf16 result: lane l, register v -> row 4(l/16) + v, column l%16
+    __device__ static inline int out_row(int lq, int v) { return 4 * lq + v; }
 };
 
 template <typename TS, int PREC>
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, 
         }
     }
 
-    // Epilogue.  Accumulator v of block (fm, fn): row 4*(lane/16)+v, column lane%16.
+    // Epilogue.  Accumulator v of block (fm, fn) holds column lane%16 and row out_row(lane/16, v).
     const int li = lane & 15, lq = lane >> 4;
 #pragma unroll
     for (int fm = 0; fm < FB; ++fm)
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, 
             if (j >= n) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int i = i0 + wm * WM + fm * 16 + 4 * lq + v;
+                int i = i0 + wm * WM + fm * 16 + Prec<PREC>::out_row(lq, v);
                 if (i >= m) continue;
                 AT val = acc[fm][fn][v];
                 if (partial) {

@@ -1,0 +1,418 @@
+!==================================================================================================!
+! omm(): orbital-minimization solver, drop-in for libOMM's external subroutine                    !
+! (libOMM/src/omm.F90:5-672, identical argument list and meaning).                                 !
+!                                                                                                  !
+! Minimises the Kim functional E(C) = tr[(2I - SW) HW], HW = C H C^T, SW = C S C^T, by             !
+! Polak-Ribiere conjugate gradients with the exact quartic line search, keeping the reference's    !
+! per-(spin,k)-point state between calls (HW, HWd, HWdd, SW, SC, SG, work2, QW, CD, x_min) and     !
+! its restart / rescaling rules.  Storage is chosen by m_storage: with a 'g' label ('gdden',      !
+! 'gsden') every work matrix lives in HBM and each step runs as MFMA products, the BSR-4 SpMM and  !
+! fused FP64 reductions; 'sdden' keeps the reference host semantics.                              !
+!                                                                                                  !
+! m_operation: 'hip' FP64, 'h32' FP32 products, 'hbf' BF16 products (FP32 accumulate), and 'hmx': !
+! mixed precision -- CG with low-precision products until the relative energy change falls below  !
+! the 'lowp_tol' option, then FP64 CG from the resulting coefficients (all quantities recomputed   !
+! exactly) to cg_tol.  Traces, the line search and the energy are FP64 in every mode.              !
+!                                                                                                  !
+! Flavour 0 (basic) is implemented; flavours 1-4 (Cholesky / preconditioned) stop with die().      !
+!==================================================================================================!
+subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, scale_T, flavour, np, ip, cg_tol, &
+               long_out, dealloc, m_storage, m_operation)
+  use iso_c_binding
+  use omx_capi
+  use MatrixSwitch
+  use omm_hip_ops
+  implicit none
+
+  character(5), intent(in) :: m_storage
+  character(3), intent(in) :: m_operation
+  logical, intent(in) :: new_S, calc_ED, init_C, long_out, dealloc
+  integer, intent(in) :: m, n, flavour, np, ip
+  real(dp), intent(in) :: eta, cg_tol, scale_T
+  real(dp), intent(out) :: e_min
+  type(matrix), intent(inout) :: H, S, D_min, C_min, T
+
+  logical :: no_S, conv, ls_fail, dev, mixed
+  logical, save :: log_start = .false.
+  logical, allocatable, save :: first_call(:)
+  integer :: i, j, k, seed, icg, nfail, icg_lowp, log_unit
+  integer :: n_step_max_inner
+  integer, parameter :: n_step_max_outer = 100
+  real(dp) :: rn(2), el, e_diff, TrQS, cg_tol_internal, lambda, lambda_d, lambda_n, e_min_old, coeff(0:4), dots(2)
+  real(dp), allocatable, save :: x_min(:)
+  real(dp), allocatable :: cinit(:,:)
+  character(3) :: prod
+  type(matrix) :: SWd, SWdd, G, G_p, D, HC, HG, work1
+  type(matrix), allocatable, save :: HW(:), HWd(:), HWdd(:), SW(:), SC(:), SG(:), work2(:), QW(:), CD(:)
+  integer(c_int) :: rc
+
+  if (log_start) then
+    open(newunit=log_unit, file='libOMM.log', position='append')
+  else
+    open(newunit=log_unit, file='libOMM.log', status='replace')
+    log_start = .true.
+  end if
+
+  no_S = .not. S%is_initialized
+  if (.not. no_S) then
+    if (flavour /= 0) then
+      if (flavour >= 1 .and. flavour <= 4) then
+        call omm_die('omm: flavours 1-4 (Cholesky / preconditioning) are not available in this build')
+      else
+        call omm_die('omm: illegal value of flavour')
+      end if
+    end if
+  end if
+  mixed = (m_operation == 'hmx')
+  if (mixed) then
+    prod = 'hip'
+  else
+    prod = m_operation
+  end if
+  if (cg_tol < 0.0_dp) then
+    cg_tol_internal = 1.0e-9_dp
+  else
+    cg_tol_internal = cg_tol
+  end if
+
+  if (calc_ED) then
+    if ((.not. allocated(first_call)) .or. first_call(ip)) call omm_die('omm: cannot calculate ED matrix before D matrix')
+    ! ED = C^T (HW + eta SW) (2I - SW) C, with the HW update the convergence exit skipped (omm.F90:163-251)
+    call update_W(HW(ip), HWd(ip), HWdd(ip), x_min(ip))
+    if (eta /= 0.0_dp) call m_add(SW(ip), 'n', HW(ip), eta, 1.0_dp, prod)
+    call calc_A2(HW(ip), C_min, CD(ip), D_min, work2(ip), prod)
+    if ((.not. dealloc) .and. (eta /= 0.0_dp)) call m_add(SW(ip), 'n', HW(ip), -eta, 1.0_dp, prod)
+    if (dealloc) call free_all()
+    close(log_unit)
+    return
+  end if
+
+  if (.not. allocated(first_call)) then
+    allocate(first_call(np))
+    first_call = .true.
+  end if
+  if (.not. allocated(x_min)) allocate(x_min(np))
+  if (.not. allocated(HW)) allocate(HW(np), HWd(np), HWdd(np), SW(np), SC(np), SG(np), work2(np), QW(np), CD(np))
+  if (.not. HW(ip)%is_initialized) call m_allocate(HW(ip), n, n, m_storage)
+  if (.not. HWd(ip)%is_initialized) call m_allocate(HWd(ip), n, n, m_storage)
+  if (.not. HWdd(ip)%is_initialized) call m_allocate(HWdd(ip), n, n, m_storage)
+  if (.not. SW(ip)%is_initialized) call m_allocate(SW(ip), n, n, m_storage)
+  if (.not. SC(ip)%is_initialized) call m_allocate(SC(ip), n, m, m_storage)
+  if (.not. SG(ip)%is_initialized) call m_allocate(SG(ip), n, m, m_storage)
+  if (.not. work2(ip)%is_initialized) call m_allocate(work2(ip), n, m, m_storage)
+  call m_allocate(SWd, n, n, m_storage)
+  call m_allocate(SWdd, n, n, m_storage)
+  call m_allocate(G, n, m, m_storage)
+  call m_allocate(G_p, n, m, m_storage)
+  call m_allocate(D, n, m, m_storage)
+  call m_allocate(HC, n, m, m_storage)
+  call m_allocate(HG, n, m, m_storage)
+  call m_allocate(work1, n, n, m_storage)
+  dev = opt_fused .and. C_min%is_device .and. G%is_device
+
+  ! random initial guess, normalised (omm.F90:329-343)
+  if (first_call(ip) .and. (.not. init_C)) then
+    seed = 123456
+    allocate(cinit(n, m))
+    do i = 1, m
+      do j = 1, n
+        do k = 1, 2
+          call omm_bsd_lcg(seed, rn(k))
+        end do
+        cinit(j, i) = sign(0.5_dp*rn(1), rn(2) - 0.5_dp)
+      end do
+    end do
+    if (C_min%is_device) then
+      call ms_upload(C_min, cinit)
+    else
+      C_min%dval = cinit
+    end if
+    deallocate(cinit)
+    call m_scale(C_min, 1.0e-2_dp/sqrt(real(m, dp)), scale_label())
+  end if
+
+  if (mpi_rank0()) then
+    write(log_unit, '(a)') '+---------------------------------------------+'
+    write(log_unit, '(a)') '| libOMM                                      |'
+    write(log_unit, '(a)') '+---------------------------------------------+'
+    if (long_out) write(log_unit, '(a)') '|             e_min            e_diff         |'
+  end if
+
+  icg = 0
+  nfail = 0
+  icg_lowp = 0
+  stat_nlog = 0
+  if (m < 10000) then
+    n_step_max_inner = m*n - 1
+  else
+    n_step_max_inner = 10000000
+  end if
+
+  if (mixed) then
+    ! low-precision phase, then FP64 from the resulting coefficients
+    call initial_energy(opt_lowp_label, first_call(ip) .or. init_C .or. new_S)
+    call cg(opt_lowp_label, opt_lowp_tol, opt_lowp_max_iter)
+    icg_lowp = icg
+    call initial_energy('hip', .true.)
+    call cg('hip', cg_tol_internal, opt_max_iter)
+  else
+    call initial_energy(prod, first_call(ip) .or. init_C .or. new_S)
+    call cg(prod, cg_tol_internal, opt_max_iter)
+  end if
+  if ((mpi_rank0()) .and. long_out) write(log_unit, '(a)') '+---------------------------------------------+'
+
+  call m_deallocate(work1)
+  call m_deallocate(HG)
+  call m_deallocate(HC)
+  call m_deallocate(D)
+  call m_deallocate(G_p)
+  call m_deallocate(G)
+  call m_deallocate(SWdd)
+  call m_deallocate(SWd)
+
+  if (.not. QW(ip)%is_initialized) call m_allocate(QW(ip), n, n, m_storage)
+  if (.not. CD(ip)%is_initialized) call m_allocate(CD(ip), n, m, m_storage)
+
+  ! density matrix D = C^T (2I - SW) C (omm.F90:575-593); skipped when D_min is not allocated
+  call m_set(QW(ip), 'a', 0.0_dp, 2.0_dp, scale_label())
+  call m_add(SW(ip), 'n', QW(ip), -1.0_dp, 1.0_dp, scale_label())
+  if (D_min%is_initialized) then
+    call calc_A(QW(ip), C_min, D_min, CD(ip), prod)
+  else
+    call mm_multiply(QW(ip), 'n', C_min, 'n', CD(ip), 1.0_dp, 0.0_dp, prod)
+  end if
+  call mm_trace(QW(ip), SW(ip), TrQS, scale_label())
+  if (mpi_rank0()) then
+    write(log_unit, '(a,i5,a)') '| minim: icg           = ', icg, '                |'
+    write(log_unit, '(a,f13.7,a)') '| minim: Tr[(2*I-S)*S] = ', TrQS, '        |'
+    write(log_unit, '(a)') '+---------------------------------------------+'
+  end if
+  e_min = e_min + TrQS*eta
+
+  stat_icg = icg
+  stat_icg_lowp = icg_lowp
+  stat_nfail = nfail
+  stat_TrQS = TrQS
+  if (conv) then
+    stat_conv = 1
+  else
+    stat_conv = 0
+  end if
+
+  if (dealloc) then
+    call free_all()
+  else
+    first_call(ip) = .false.
+  end if
+  close(log_unit)
+
+contains
+
+  logical function mpi_rank0()
+    mpi_rank0 = .true.
+  end function mpi_rank0
+
+  !> Label for adds, scales and reductions (never a low-precision product label).
+  function scale_label() result(l)
+    character(3) :: l
+    if (C_min%is_device) then
+      l = 'hip'
+    else
+      l = 'ref'
+    end if
+  end function scale_label
+
+  !> HW/SW += x (Wd^T + Wd) + x^2 Wdd (omm.F90:496-502, 511-517)
+  subroutine update_W(W, Wd, Wdd, x)
+    type(matrix), intent(inout) :: W
+    type(matrix), intent(in) :: Wd, Wdd
+    real(dp), intent(in) :: x
+    if (opt_fused .and. W%is_device .and. Wd%is_device .and. Wdd%is_device) then
+      rc = omx_omm_update_w(ms_hip_ctx(), W%dtype, int(W%dim1, c_int), W%ld, x, Wd%dptr, Wdd%dptr, W%dptr)
+      if (rc /= 0) call omm_die('omm: '//omx_error_string())
+    else
+      call m_add(Wd, 'c', W, x, 1.0_dp, scale_label_of(W))
+      call m_add(Wd, 'n', W, x, 1.0_dp, scale_label_of(W))
+      call m_add(Wdd, 'n', W, x**2, 1.0_dp, scale_label_of(W))
+    end if
+  end subroutine update_W
+
+  function scale_label_of(A) result(l)
+    type(matrix), intent(in) :: A
+    character(3) :: l
+    if (A%is_device) then
+      l = 'hip'
+    else
+      l = 'ref'
+    end if
+  end function scale_label_of
+
+  !> Energy, gradient and line-search data at the current C (omm.F90:350-413).
+  subroutine initial_energy(lab, recompute_S)
+    character(3), intent(in) :: lab
+    logical, intent(in) :: recompute_S
+    call calc_AW(H, C_min, HW(ip), HC, lab)
+    if (no_S) then
+      if (recompute_S) call mm_multiply(C_min, 'n', C_min, 'c', SW(ip), 1.0_dp, 0.0_dp, lab)
+    else
+      if (recompute_S) then
+        call calc_AW(S, C_min, SW(ip), SC(ip), lab)
+      else
+        call m_add(SG(ip), 'n', SC(ip), x_min(ip), 1.0_dp, scale_label())
+      end if
+    end if
+    if (no_S) then
+      call calc_G(HW(ip), SW(ip), G, HC, C_min, lab)
+      call mm_multiply(C_min, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+    else
+      call calc_G(HW(ip), SW(ip), G, HC, SC(ip), lab)
+      call mm_multiply(SC(ip), 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+    end if
+    call mm_multiply(HC, 'n', G, 'c', HWd(ip), 1.0_dp, 0.0_dp, lab)
+    call calc_AW(H, G, HWdd(ip), HG, lab)
+    if (no_S) then
+      call mm_multiply(G, 'n', G, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
+    else
+      call calc_AW(S, G, SWdd, SG(ip), lab)
+    end if
+    call calc_coeff(HW(ip), SW(ip), HWd(ip), SWd, HWdd(ip), SWdd, work1, coeff, lab)
+    e_min = coeff(0)
+  end subroutine initial_energy
+
+  !> Polak-Ribiere CG with exact quartic line search (omm.F90:429-555).
+  !> Stops when e_diff <= tol, or after max_steps steps of this phase (0 = reference caps only).
+  subroutine cg(lab, tol, max_steps)
+    character(3), intent(in) :: lab
+    real(dp), intent(in) :: tol
+    integer, intent(in) :: max_steps
+    integer :: outer, inner, steps
+    logical :: ls_conv, capped
+    conv = .false.
+    capped = .false.
+    steps = 0
+    do outer = 1, n_step_max_outer
+      lambda = 0.0_dp
+      do inner = 1, n_step_max_inner
+        if (dev) then
+          rc = omx_omm_cg_direction(ms_hip_ctx(), G%dtype, int(n, c_int), int(m, c_int), lambda, G%dptr, D%dptr, &
+                                    G_p%dptr, G%ld)
+          if (rc /= 0) call omm_die('omm: '//omx_error_string())
+        else
+          call m_add(G, 'n', D, 1.0_dp, lambda, scale_label())
+          call m_add(G, 'n', G_p, 1.0_dp, 0.0_dp, scale_label())
+        end if
+        e_min_old = e_min
+        ! after the first step of a phase, line-search data along the new direction (omm.F90:448-464)
+        if (steps > 0) then
+          call mm_multiply(HC, 'n', D, 'c', HWd(ip), 1.0_dp, 0.0_dp, lab)
+          if (no_S) then
+            call mm_multiply(C_min, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+          else
+            call mm_multiply(SC(ip), 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+          end if
+          call calc_AW(H, D, HWdd(ip), HG, lab)
+          if (no_S) then
+            call mm_multiply(D, 'n', D, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
+          else
+            call calc_AW(S, D, SWdd, SG(ip), lab)
+          end if
+          call calc_coeff(HW(ip), SW(ip), HWd(ip), SWd, HWdd(ip), SWdd, work1, coeff, lab)
+        end if
+        call omm_solve_quartic(coeff(0:4), x_min(ip), ls_fail)
+        if (ls_fail) then
+          if (mpi_rank0()) write(log_unit, '(a)') '| WARNING: Rescaling coefficients!            |'
+          e_min = 3.0_dp*e_min
+          call m_scale(C_min, 0.5_dp, scale_label())
+          ls_conv = .false.
+          nfail = nfail + 1
+        else
+          e_min = coeff(4)*x_min(ip)**4 + coeff(3)*x_min(ip)**3 + coeff(2)*x_min(ip)**2 + coeff(1)*x_min(ip) + coeff(0)
+          call m_add(D, 'n', C_min, x_min(ip), 1.0_dp, scale_label())
+          ls_conv = .true.
+        end if
+        if (ls_fail) then
+          call m_scale(SW(ip), 0.25_dp, scale_label())
+        else
+          call update_W(SW(ip), SWd, SWdd, x_min(ip))
+        end if
+        e_diff = 2.0_dp*abs((e_min - e_min_old)/(e_min + e_min_old))
+        if (mpi_rank0() .and. long_out) write(log_unit, '(a,2(1x,i5),2(1x,es15.7e3),1x,a)') '|', outer, inner, &
+          e_min, e_diff, '|'
+        if (stat_nlog < stat_log_cap) then
+          stat_nlog = stat_nlog + 1
+          stat_elog(stat_nlog) = e_min
+        end if
+        icg = icg + 1
+        steps = steps + 1
+        if (e_diff <= tol) then
+          conv = .true.
+          exit
+        end if
+        if (max_steps > 0 .and. steps >= max_steps) then
+          capped = .true.
+          exit
+        end if
+        if (ls_fail) then
+          call m_scale(HW(ip), 0.25_dp, scale_label())
+        else
+          call update_W(HW(ip), HWd(ip), HWdd(ip), x_min(ip))
+        end if
+        if (ls_fail) then
+          call m_scale(HC, 0.5_dp, scale_label())
+          if (.not. no_S) call m_scale(SC(ip), 0.5_dp, scale_label())
+          call m_add(G_p, 'n', G, 1.0_dp, 0.0_dp, scale_label())
+          call m_add(HC, 'n', G, 1.5_dp, 1.0_dp, scale_label())
+        else
+          call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
+          if (no_S) then
+            call calc_G(HW(ip), SW(ip), G, HC, C_min, lab)
+          else
+            call m_add(SG(ip), 'n', SC(ip), x_min(ip), 1.0_dp, scale_label())
+            call calc_G(HW(ip), SW(ip), G, HC, SC(ip), lab)
+          end if
+        end if
+        if (ls_conv) then
+          if (dev) then
+            rc = omx_omm_pr_dots(ms_hip_ctx(), G%dtype, int(n, c_int), int(m, c_int), G%dptr, G_p%dptr, G%ld, dots)
+            if (rc /= 0) call omm_die('omm: '//omx_error_string())
+            lambda_n = dots(1)
+            lambda_d = dots(2)
+          else
+            call m_add(G, 'n', work2(ip), 1.0_dp, 0.0_dp, scale_label())
+            call m_add(G_p, 'n', work2(ip), -1.0_dp, 1.0_dp, scale_label())
+            call mm_trace(G, work2(ip), lambda_n, scale_label())
+            call mm_trace(G_p, G_p, lambda_d, scale_label())
+          end if
+          lambda = lambda_n/lambda_d
+        else
+          exit
+        end if
+      end do
+      if (conv .or. capped) exit
+    end do
+    if (outer > n_step_max_outer) then
+      if (mpi_rank0()) write(log_unit, '(a)') '| WARNING: OMM failed to converge!            |'
+    end if
+  end subroutine cg
+
+  subroutine free_all()
+    integer :: q
+    if (allocated(HW)) then
+      do q = 1, size(HW)
+        if (CD(q)%is_initialized) call m_deallocate(CD(q))
+        if (QW(q)%is_initialized) call m_deallocate(QW(q))
+        if (work2(q)%is_initialized) call m_deallocate(work2(q))
+        if (SG(q)%is_initialized) call m_deallocate(SG(q))
+        if (SC(q)%is_initialized) call m_deallocate(SC(q))
+        if (SW(q)%is_initialized) call m_deallocate(SW(q))
+        if (HWdd(q)%is_initialized) call m_deallocate(HWdd(q))
+        if (HWd(q)%is_initialized) call m_deallocate(HWd(q))
+        if (HW(q)%is_initialized) call m_deallocate(HW(q))
+      end do
+      deallocate(HW, HWd, HWdd, SW, SC, SG, work2, QW, CD)
+    end if
+    if (allocated(x_min)) deallocate(x_min)
+    if (allocated(first_call)) deallocate(first_call)
+  end subroutine free_all
+
+end subroutine omm

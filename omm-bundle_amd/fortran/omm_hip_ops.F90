@@ -1,0 +1,233 @@
+!==================================================================================================!
+! libOMM operator kernels over MatrixSwitch matrices, with fused device paths.                    !
+!                                                                                                  !
+! calc_G, calc_AW, calc_A, calc_A2 and calc_coeff keep the meaning of libOMM/src/omm_ops.F90:17-219 !
+! (same arguments).  When every operand is device-resident ('g' storage) the line-search          !
+! coefficients come from one fused FP64 reduction and the gradient from one MFMA product with a   !
+! 2n-long contraction; otherwise the reference sequence of MatrixSwitch calls runs.               !
+! Also: the line-search quartic solver (omm_quartic.F90:60-153), the BSD LCG of the random       !
+! initial guess (omm_rand.F90:19-55), solver options beyond the omm() signature and the last-call !
+! statistics.                                                                                      !
+!==================================================================================================!
+module omm_hip_ops
+  use iso_c_binding
+  use omx_capi
+  use MatrixSwitch
+  implicit none
+
+  real(dp), parameter :: Pi = 3.141592653589793238462643383279502884197_dp
+
+  ! ---- options (set with omm_set_option; defaults reproduce the reference) ----
+  integer, save :: opt_max_iter = 0          ! > 0: cap on CG steps per call (icg)
+  real(dp), save :: opt_lowp_tol = 1.0e-6_dp ! mixed precision: e_diff that ends the low-precision phase
+  integer, save :: opt_lowp_max_iter = 0     ! mixed precision: cap on low-precision steps (0 = none)
+  character(3), save :: opt_lowp_label = 'hbf'
+  logical, save :: opt_fused = .true.        ! use fused device kernels when all operands are on device
+
+  ! ---- statistics of the last omm() call ----
+  integer, save :: stat_icg = 0, stat_conv = 0, stat_nfail = 0, stat_icg_lowp = 0
+  real(dp), save :: stat_TrQS = 0.0_dp
+  integer, parameter :: stat_log_cap = 100000
+  real(dp), save :: stat_elog(stat_log_cap)
+  integer, save :: stat_nlog = 0
+
+contains
+
+  logical function on_device(A)
+    type(matrix), intent(in) :: A
+    on_device = A%is_device
+  end function on_device
+
+  !> G := 4HC - 2 SW HC - 2 HW SC  (omm_ops.F90:17-39)
+  subroutine calc_G(HW, SW, G, HC, SC, m_operation)
+    character(3), intent(in) :: m_operation
+    type(matrix), intent(in) :: HW, SW
+    type(matrix), intent(inout) :: G, HC, SC
+    integer(c_int) :: rc
+    if (opt_fused .and. on_device(HW) .and. on_device(SW) .and. on_device(G) .and. on_device(HC) &
+        .and. on_device(SC)) then
+      rc = omx_omm_calc_G(ms_hip_ctx(), ms_label_prec(m_operation), G%dtype, int(G%dim1, c_int), &
+                          int(G%dim2, c_int), HW%dptr, SW%dptr, HW%ld, HC%dptr, SC%dptr, G%dptr, G%ld)
+      if (rc /= 0) call die('calc_G: '//omx_error_string())
+      return
+    end if
+    call m_add(HC, 'n', G, 4.0_dp, 0.0_dp, m_operation)
+    call mm_multiply(SW, 'n', HC, 'n', G, -2.0_dp, 1.0_dp, m_operation)
+    call mm_multiply(HW, 'n', SC, 'n', G, -2.0_dp, 1.0_dp, m_operation)
+  end subroutine calc_G
+
+  !> AC := C*A; AW := AC*C^T  (omm_ops.F90:44-64)
+  subroutine calc_AW(A, C, AW, AC, m_operation)
+    character(3), intent(in) :: m_operation
+    type(matrix), intent(in) :: A, C
+    type(matrix), intent(inout) :: AC, AW
+    call mm_multiply(C, 'n', A, 'n', AC, 1.0_dp, 0.0_dp, m_operation)
+    call mm_multiply(AC, 'n', C, 'c', AW, 1.0_dp, 0.0_dp, m_operation)
+  end subroutine calc_AW
+
+  !> CAW := AW*C1; A := C1^T*CAW  (omm_ops.F90:104-124)
+  subroutine calc_A(AW, C1, A, CAW, m_operation)
+    character(3), intent(in) :: m_operation
+    type(matrix), intent(in) :: AW, C1
+    type(matrix), intent(inout) :: A, CAW
+    call mm_multiply(AW, 'n', C1, 'n', CAW, 1.0_dp, 0.0_dp, m_operation)
+    call mm_multiply(C1, 'c', CAW, 'n', A, 1.0_dp, 0.0_dp, m_operation)
+  end subroutine calc_A
+
+  !> CAW := AW*C2; A := C1^T*CAW  (omm_ops.F90:129-150)
+  subroutine calc_A2(AW, C1, C2, A, CAW, m_operation)
+    character(3), intent(in) :: m_operation
+    type(matrix), intent(in) :: AW, C1, C2
+    type(matrix), intent(inout) :: A, CAW
+    call mm_multiply(AW, 'n', C2, 'n', CAW, 1.0_dp, 0.0_dp, m_operation)
+    call mm_multiply(C1, 'c', CAW, 'n', A, 1.0_dp, 0.0_dp, m_operation)
+  end subroutine calc_A2
+
+  !> Quartic line-search coefficients (omm_ops.F90:156-219).
+  subroutine calc_coeff(HW, SW, HWd, SWd, HWdd, SWdd, SWdH, coeff, m_operation)
+    character(3), intent(in) :: m_operation
+    type(matrix), intent(in) :: HW, SW, HWd, SWd, HWdd, SWdd
+    type(matrix), intent(inout) :: SWdH
+    real(dp), intent(inout) :: coeff(0:4)
+    real(dp) :: TrH, TrHS, TrHd, TrHdS, TrHSd, TrHdd, TrHddS, TrHSdd, TrHdSd, TrHdSdH, TrHddSd, TrHdSdd, TrHddSdd
+    real(dp) :: c5(5)
+    integer(c_int) :: rc
+    if (opt_fused .and. on_device(HW) .and. on_device(SW) .and. on_device(HWd) .and. on_device(SWd) &
+        .and. on_device(HWdd) .and. on_device(SWdd)) then
+      rc = omx_omm_coeff(ms_hip_ctx(), HW%dtype, int(HW%dim1, c_int), HW%ld, HW%dptr, SW%dptr, HWd%dptr, &
+                         SWd%dptr, HWdd%dptr, SWdd%dptr, c5)
+      if (rc /= 0) call die('calc_coeff: '//omx_error_string())
+      coeff(0:4) = c5
+      return
+    end if
+    call m_trace(HW, TrH, trace_label(m_operation))
+    call mm_trace(HW, SW, TrHS, trace_label(m_operation))
+    call m_trace(HWd, TrHd, trace_label(m_operation))
+    call mm_trace(SW, HWd, TrHdS, trace_label(m_operation))
+    call mm_trace(HW, SWd, TrHSd, trace_label(m_operation))
+    call m_trace(HWdd, TrHdd, trace_label(m_operation))
+    call mm_trace(HWdd, SW, TrHddS, trace_label(m_operation))
+    call mm_trace(HW, SWdd, TrHSdd, trace_label(m_operation))
+    call m_add(SWd, 'c', SWdH, 1.0_dp, 0.0_dp, trace_label(m_operation))
+    call mm_trace(SWdH, HWd, TrHdSd, trace_label(m_operation))
+    call mm_trace(SWd, HWd, TrHdSdH, trace_label(m_operation))
+    call mm_trace(HWdd, SWd, TrHddSd, trace_label(m_operation))
+    call mm_trace(SWdd, HWd, TrHdSdd, trace_label(m_operation))
+    call mm_trace(HWdd, SWdd, TrHddSdd, trace_label(m_operation))
+    coeff(0) = 2.0_dp*TrH - TrHS
+    coeff(1) = 2.0_dp*(2.0_dp*TrHd - TrHdS - TrHSd)
+    coeff(2) = 2.0_dp*(TrHdd - TrHdSd - TrHdSdH) - TrHddS - TrHSdd
+    coeff(3) = -2.0_dp*(TrHddSd + TrHdSdd)
+    coeff(4) = -TrHddSdd
+  end subroutine calc_coeff
+
+  !> Reductions always accumulate in FP64; low-precision product labels map to 'hip' for them.
+  function trace_label(lab) result(t)
+    character(3), intent(in) :: lab
+    character(3) :: t
+    t = lab
+    if (lab == 'h32' .or. lab == 'hbf') t = 'hip'
+  end function trace_label
+
+  !> Minimum of the quartic line-search polynomial (omm_quartic.F90:60-153, Numerical Recipes cubic).
+  subroutine omm_solve_quartic(c, x_min, fail)
+    real(dp), intent(in) :: c(0:4)
+    real(dp), intent(out) :: x_min
+    logical, intent(out) :: fail
+    integer :: x_order(3)
+    real(dp) :: t(3), z(3), a, b, d, Q, R, theta, S, U
+    fail = .false.
+    a = 3.0_dp*c(3)/(4.0_dp*c(4))
+    b = 2.0_dp*c(2)/(4.0_dp*c(4))
+    if ((abs(b) >= 1.0e11_dp) .or. (abs(c(4)) <= 1.0e-11_dp)) then
+      x_min = -0.5_dp*c(1)/c(2)
+      return
+    end if
+    d = c(1)/(4.0_dp*c(4))
+    Q = (a**2 - 3.0_dp*b)/9.0_dp
+    R = (2.0_dp*a**3 - 9.0_dp*a*b + 27.0_dp*d)/54.0_dp
+    if (R**2 < Q**3) then
+      theta = acos(R/sqrt(Q**3))
+      t(1) = -2.0_dp*sqrt(Q)*cos(theta/3.0_dp) - a/3.0_dp
+      t(2) = -2.0_dp*sqrt(Q)*cos((theta + 2.0_dp*Pi)/3.0_dp) - a/3.0_dp
+      t(3) = -2.0_dp*sqrt(Q)*cos((theta - 2.0_dp*Pi)/3.0_dp) - a/3.0_dp
+      z(1:3) = c(4)*t(1:3)**4 + c(3)*t(1:3)**3 + c(2)*t(1:3)**2 + c(1)*t(1:3) + c(0)
+      if (c(4) > 0.0_dp) then
+        if (all(z(1) >= z(2:3))) then
+          x_order(1:3) = (/1, 2, 3/)
+        else if (z(2) > z(3)) then
+          x_order(1:3) = (/2, 3, 1/)
+        else
+          x_order(1:3) = (/3, 1, 2/)
+        end if
+        if ((0.0_dp <= t(x_order(1))) .and. (t(x_order(2)) <= t(x_order(1)))) then
+          x_min = t(x_order(2))
+        else
+          x_min = t(x_order(3))
+        end if
+      else
+        if (all(z(1) <= z(2:3))) then
+          x_min = t(1)
+        else if (z(2) < z(3)) then
+          x_min = t(2)
+        else
+          x_min = t(3)
+        end if
+      end if
+    else
+      S = -sign(1.0_dp, R)*(abs(R) + sqrt(R**2 - Q**3))**(1.0_dp/3.0_dp)
+      if (S == 0.0_dp) then
+        U = 0.0_dp
+      else
+        U = Q/S
+      end if
+      x_min = (S + U) - (a/3.0_dp)
+      if (c(4) < 0.0_dp) fail = .true.
+    end if
+  end subroutine omm_solve_quartic
+
+  !> x <- (1103515245 x + 12345) mod 2^31, r = x / 2^31  (omm_rand.F90:37-55)
+  subroutine omm_bsd_lcg(x, r)
+    integer, intent(inout) :: x
+    real(dp), intent(out) :: r
+    integer(c_int64_t), parameter :: a = 1103515245_c_int64_t, m = 2147483648_c_int64_t
+    x = int(mod(a*int(x, c_int64_t) + 12345_c_int64_t, m))
+    r = real(x, dp)/real(m, dp)
+  end subroutine omm_bsd_lcg
+
+  !> Fatal error in libOMM (omm_ops.F90:1336-1358): message to libOMM.err, stop 1.
+  subroutine omm_die(message)
+    character(*), intent(in), optional :: message
+    integer :: err_unit
+    open(newunit=err_unit, file='libOMM.err', status='replace')
+    write(err_unit, '(a)') 'FATAL ERROR in libOMM!'
+    if (present(message)) write(err_unit, '(a)') message
+    close(err_unit)
+    stop 1
+  end subroutine omm_die
+
+  !> Set a solver option by name (extensions beyond the omm() argument list).
+  subroutine omm_set_option(name, value)
+    character(*), intent(in) :: name
+    real(dp), intent(in) :: value
+    select case (name)
+    case ('max_iter')
+      opt_max_iter = nint(value)
+    case ('lowp_tol')
+      opt_lowp_tol = value
+    case ('lowp_max_iter')
+      opt_lowp_max_iter = nint(value)
+    case ('lowp_bf16')
+      if (value /= 0.0_dp) then
+        opt_lowp_label = 'hbf'
+      else
+        opt_lowp_label = 'h32'
+      end if
+    case ('fused')
+      opt_fused = (value /= 0.0_dp)
+    case default
+      call omm_die('omm_set_option: unknown option '//name)
+    end select
+  end subroutine omm_set_option
+
+end module omm_hip_ops

@@ -1,0 +1,124 @@
+"""ctypes access to the Fortran drop-in (libommf.so): MatrixSwitch matrices by handle and omm().
+
+This mirrors how a Fortran SCF code drives libOMM (libOMM/examples/example_gamma.F90:134-245):
+allocate matrices with a storage label, fill them, call omm() once per SCF step with the
+reference argument list, read e_min and the density matrix back.  Storage labels starting with
+'g' keep the matrices in HBM and run the HIP kernels of libommhip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libommf.so")
+
+_i, _d, _vp, _i64 = C.c_int, C.c_double, C.c_void_p, C.c_int64
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not found: build it with omm-bundle_amd/build.sh")
+        L = C.CDLL(LIB_PATH)
+        L.omm_c_setup.argtypes = [_i]
+        L.omm_c_mat_alloc.argtypes = [_i, _i, C.c_char_p]
+        L.omm_c_mat_from_csc.argtypes = [_i, _vp, _vp, _vp, C.c_char_p]
+        L.omm_c_mat_upload.argtypes = [_i, _vp]
+        L.omm_c_mat_download.argtypes = [_i, _vp]
+        L.omm_c_mat_free.argtypes = [_i]
+        L.omm_c_mat_dptr.argtypes = [_i, C.POINTER(_i64)]
+        L.omm_c_mat_dptr.restype = _vp
+        L.omm_c_solve.argtypes = [_i, _i, _i, _i, _i, C.POINTER(_d), _i, _i, _d, _i, _i, _i, _i, _i, _d, _i, _i,
+                                  C.c_char_p, C.c_char_p]
+        L.omm_c_stats.argtypes = [C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
+        L.omm_c_elog.argtypes = [_vp, _i]
+        L.omm_c_set_option.argtypes = [C.c_char_p, _d]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Mat:
+    """A MatrixSwitch matrix held by the Fortran library (handle)."""
+
+    def __init__(self, rows, cols, label="gdden", data=None):
+        self.rows, self.cols, self.label = rows, cols, label
+        self.h = lib().omm_c_mat_alloc(rows, cols, label.encode())
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_alloc failed")
+        if data is not None:
+            self.upload(data)
+
+    @classmethod
+    def from_csc(cls, nb, colptr, rowidx, val, label="gdcsc"):
+        self = cls.__new__(cls)
+        self.rows = self.cols = nb
+        self.label = label
+        cp = np.ascontiguousarray(colptr, np.int32)
+        ri = np.ascontiguousarray(rowidx, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self.h = lib().omm_c_mat_from_csc(nb, _p(cp), _p(ri), _p(v), label.encode())
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_from_csc failed")
+        return self
+
+    def upload(self, a):
+        a = np.asfortranarray(a, np.float64)
+        assert a.shape == (self.rows, self.cols)
+        if lib().omm_c_mat_upload(self.h, _p(a)) != 0:
+            raise RuntimeError("upload failed")
+
+    def download(self):
+        out = np.zeros((self.rows, self.cols), order="F")
+        if lib().omm_c_mat_download(self.h, _p(out)) != 0:
+            raise RuntimeError("download failed")
+        return out
+
+    def dptr(self):
+        ld = C.c_int64(0)
+        p = lib().omm_c_mat_dptr(self.h, C.byref(ld))
+        return p, ld.value
+
+    def free(self):
+        if self.h > 0:
+            lib().omm_c_mat_free(self.h)
+            self.h = 0
+
+
+def setup(device=0):
+    lib().omm_c_setup(device)
+
+
+def sync():
+    lib().omm_c_sync()
+
+
+def set_option(name: str, value: float):
+    lib().omm_c_set_option(name.encode(), float(value))
+
+
+def omm(m, n, H: Mat, S: Mat | None, C_min: Mat, *, new_S=True, D_min: Mat | None = None, calc_ED=False, eta=0.0,
+        init_C=True, flavour=0, np_=1, ip=1, cg_tol=-1.0, long_out=False, dealloc=False, storage="gdden",
+        operation="hip"):
+    """Call the Fortran omm() (same arguments as libOMM/src/omm.F90:5-6).  Returns a dict."""
+    e = C.c_double(0)
+    rc = lib().omm_c_solve(m, n, H.h, S.h if S is not None else 0, int(new_S), C.byref(e),
+                           D_min.h if D_min is not None else 0, int(calc_ED), eta, C_min.h, int(init_C), flavour, np_,
+                           ip, cg_tol, int(long_out), int(dealloc), storage.encode(), operation.encode())
+    if rc != 0:
+        raise RuntimeError("omm_c_solve failed (bad handle)")
+    icg, conv, nfail, icg_lowp = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    trqs = C.c_double(0)
+    lib().omm_c_stats(C.byref(icg), C.byref(conv), C.byref(nfail), C.byref(icg_lowp), C.byref(trqs))
+    buf = np.zeros(100000)
+    k = lib().omm_c_elog(_p(buf), len(buf))
+    return {"e_min": e.value, "icg": icg.value, "converged": bool(conv.value), "nfail": nfail.value,
+            "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy()}

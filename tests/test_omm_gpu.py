@@ -1,0 +1,144 @@
+"""The Fortran omm() drop-in on device storage ('gdden' + 'gdcsc', HIP kernels) against the CPU
+oracle (oracle/omm_oracle.c, itself bit-identical to the reference serial build).
+
+Tolerances (FP64): converged energies agree to 1e-10 relative, per-step
+energies of the trajectory to 1e-9 relative, density matrices to 1e-8 absolute, CG step counts
+within 2 (the e_diff test can flip at the threshold when the last bits differ).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ommhip as oh
+import oracle as O
+import pyomm as F
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _note(*a):
+    print("CASE", *a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    F.setup(0)
+    yield
+
+
+def csc_of(A):
+    nb = A.shape[0]
+    cols = [np.nonzero(A[:, j])[0] for j in range(nb)]
+    return (np.cumsum([0] + [len(c) for c in cols]).astype(np.int32), np.concatenate(cols).astype(np.int32),
+            np.concatenate([A[c, j] for j, c in enumerate(cols)]))
+
+
+def lattice(L, seed=oh.DEFAULT_SEED):
+    nb = 4 * L ** 3
+    Hb, Sb = oh.lattice_bsr(L, 0, seed), oh.lattice_bsr(L, 1, seed)
+    return nb, Hb, Sb
+
+
+def test_example_gamma_device_matches_oracle():
+    _note("example_gamma device")
+    m, n, H, S = O.read_example_gamma(os.path.join(GOLD, "example_gamma.dat"))
+    H2 = H.copy()
+    H2[np.diag_indices(m)] += -0.001
+    om = O.Omm()
+    Cc = np.zeros((n, m), order="F")
+    o1 = om(m, n, H, S, Cc, new_S=True, init_C=False)
+    o2 = om(m, n, H2, S, Cc, new_S=False, init_C=True)
+    o3 = om(m, n, H2, S, Cc, new_S=False, init_C=True, calc_ED=True, dealloc=True)
+
+    Hm, Sm = F.Mat.from_csc(m, *csc_of(H)), F.Mat.from_csc(m, *csc_of(S))
+    Cm, Dm, EDm = F.Mat(n, m), F.Mat(m, m), F.Mat(m, m)
+    r1 = F.omm(m, n, Hm, Sm, Cm, new_S=True, D_min=Dm, init_C=False)
+    D1 = Dm.download()
+    Hm.free()
+    Hm = F.Mat.from_csc(m, *csc_of(H2))
+    r2 = F.omm(m, n, Hm, Sm, Cm, new_S=False, D_min=Dm, init_C=True)
+    D2 = Dm.download()
+    F.omm(m, n, Hm, Sm, Cm, new_S=False, D_min=EDm, calc_ED=True, init_C=True, dealloc=True)
+    for r, o in ((r1, o1), (r2, o2)):
+        assert abs(r["e_min"] - o["e_min"]) <= 1e-10 * abs(o["e_min"])
+        assert abs(r["icg"] - o["icg"]) <= 2
+        k = min(len(r["e_log"]), len(o["e_log"]), 20)
+        assert np.allclose(r["e_log"][:k], o["e_log"][:k], rtol=1e-9, atol=0)
+    assert np.max(np.abs(D1 - o1["D"])) < 1e-8 and np.max(np.abs(D2 - o2["D"])) < 1e-8
+    assert np.max(np.abs(EDm.download() - o3["D"])) < 1e-8
+    for M in (Hm, Sm, Cm, Dm, EDm):
+        M.free()
+
+
+@pytest.mark.parametrize("L,eta", [(3, 0.0), (4, 0.2)])
+def test_lattice_device_matches_oracle(L, eta):
+    _note("lattice device", L, eta)
+    nb, Hb, Sb = lattice(L)
+    no = nb // 2
+    H = oh.bsr_to_dense(nb, *Hb)
+    S = oh.bsr_to_dense(nb, *Sb)
+    Hs = H - eta * S
+    C0 = oh.random_c0(no, nb)
+    Cc = C0.copy(order="F")
+    o = O.Omm()(nb, no, csc_of(Hs), csc_of(S), Cc, init_C=True, eta=eta, cg_tol=1e-9, max_iter=300)
+    Hm, Sm = F.Mat.from_csc(nb, *csc_of(Hs)), F.Mat.from_csc(nb, *csc_of(S))
+    Cm, Dm = F.Mat(no, nb, data=C0), F.Mat(nb, nb)
+    F.set_option("max_iter", 300)
+    try:
+        r = F.omm(nb, no, Hm, Sm, Cm, D_min=Dm, init_C=True, eta=eta, cg_tol=1e-9, dealloc=True)
+    finally:
+        F.set_option("max_iter", 0)
+    assert abs(r["e_min"] - o["e_min"]) <= 1e-10 * abs(o["e_min"])
+    assert abs(r["icg"] - o["icg"]) <= 2
+    k = min(len(r["e_log"]), len(o["e_log"]), 30)
+    assert np.allclose(r["e_log"][:k], o["e_log"][:k], rtol=1e-9, atol=0)
+    assert np.max(np.abs(Dm.download() - o["D"])) < 1e-7
+    assert abs(r["TrQS"] - o["TrQS"]) < 1e-7
+    for M in (Hm, Sm, Cm, Dm):
+        M.free()
+
+
+def test_no_S_device_matches_oracle():
+    _note("no_S device")
+    nb, Hb, _ = lattice(3)
+    no = 40
+    H = oh.bsr_to_dense(nb, *Hb) - 0.5 * np.eye(nb)
+    C0 = oh.random_c0(no, nb)
+    Cc = C0.copy(order="F")
+    o = O.Omm()(nb, no, csc_of(H), None, Cc, init_C=True)
+    Hm, Cm = F.Mat.from_csc(nb, *csc_of(H)), F.Mat(no, nb, data=C0)
+    r = F.omm(nb, no, Hm, None, Cm, init_C=True, dealloc=True)
+    assert abs(r["e_min"] - o["e_min"]) <= 1e-10 * abs(o["e_min"])
+    assert np.max(np.abs(Cm.download() - Cc)) < 1e-7
+    Hm.free()
+    Cm.free()
+
+
+@pytest.mark.parametrize("lowp", ["hbf", "h32"])
+def test_mixed_precision_reaches_fp64_energy(lowp):
+    """'hmx': low-precision products until e_diff < lowp_tol, then FP64 refinement; the energy
+    matches the FP64 device solve to 1e-10 relative."""
+    _note("mixed", lowp)
+    L, eta = 4, 0.2
+    nb, Hb, Sb = lattice(L)
+    no = nb // 2
+    H, S = oh.bsr_to_dense(nb, *Hb), oh.bsr_to_dense(nb, *Sb)
+    Hs = H - eta * S
+    C0 = oh.random_c0(no, nb)
+    Hm, Sm = F.Mat.from_csc(nb, *csc_of(Hs)), F.Mat.from_csc(nb, *csc_of(S))
+    Cm = F.Mat(no, nb, data=C0)
+    ref = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True)
+    Cm.upload(C0)
+    F.set_option("lowp_bf16", 1 if lowp == "hbf" else 0)
+    F.set_option("lowp_tol", 1e-5)
+    try:
+        r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True, operation="hmx")
+    finally:
+        F.set_option("lowp_bf16", 1)
+        F.set_option("lowp_tol", 1e-6)
+    assert r["icg_lowp"] > 0
+    assert abs(r["e_min"] - ref["e_min"]) <= 1e-10 * abs(ref["e_min"]) * 10
+    for M in (Hm, Sm, Cm):
+        M.free()
