@@ -341,6 +341,7 @@ contains
         if (stat_nlog < stat_log_cap) then
           stat_nlog = stat_nlog + 1
           stat_elog(stat_nlog) = e_min
+          stat_tlog(stat_nlog) = wall_seconds()  ! after the coefficient readback: the stream is idle
         end if
         icg = icg + 1
         steps = steps + 1

@@ -234,6 +234,21 @@ contains
     omm_c_elog = k
   end function omm_c_elog
 
+  !> Wall-clock seconds at the end of each CG step of the last call (up to cap); returns the count.
+  integer(c_int) function omm_c_tlog(buf, cap) bind(c, name='omm_c_tlog')
+    real(c_double), intent(out) :: buf(*)
+    integer(c_int), value :: cap
+    integer :: k
+    k = min(int(cap), stat_nlog)
+    buf(1:k) = stat_tlog(1:k)
+    omm_c_tlog = k
+  end function omm_c_tlog
+
+  !> The libommhip context used by the 'g' storage (for event timing from the caller).
+  type(c_ptr) function omm_c_ctx() bind(c, name='omm_c_ctx')
+    omm_c_ctx = ms_hip_ctx()
+  end function omm_c_ctx
+
   integer(c_int) function omm_c_set_option(name, value) bind(c, name='omm_c_set_option')
     character(kind=c_char), intent(in) :: name(*)
     real(c_double), value :: value

@@ -29,9 +29,17 @@ module omm_hip_ops
   real(dp), save :: stat_TrQS = 0.0_dp
   integer, parameter :: stat_log_cap = 100000
   real(dp), save :: stat_elog(stat_log_cap)
+  real(dp), save :: stat_tlog(stat_log_cap)   ! wall-clock seconds at the end of each CG step
   integer, save :: stat_nlog = 0
 
 contains
+
+  !> Monotonic wall clock in seconds.
+  real(dp) function wall_seconds()
+    integer(c_int64_t) :: cnt, rate
+    call system_clock(cnt, rate)
+    wall_seconds = real(cnt, dp)/real(rate, dp)
+  end function wall_seconds
 
   logical function on_device(A)
     type(matrix), intent(in) :: A

@@ -38,6 +38,8 @@ def lib():
         L.omm_c_stats.argtypes = [C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
         L.omm_c_elog.argtypes = [_vp, _i]
         L.omm_c_set_option.argtypes = [C.c_char_p, _d]
+        L.omm_c_tlog.argtypes = [_vp, _i]
+        L.omm_c_ctx.restype = _vp
         _lib = L
     return _lib
 
@@ -120,5 +122,12 @@ def omm(m, n, H: Mat, S: Mat | None, C_min: Mat, *, new_S=True, D_min: Mat | Non
     lib().omm_c_stats(C.byref(icg), C.byref(conv), C.byref(nfail), C.byref(icg_lowp), C.byref(trqs))
     buf = np.zeros(100000)
     k = lib().omm_c_elog(_p(buf), len(buf))
+    tbuf = np.zeros(100000)
+    kt = lib().omm_c_tlog(_p(tbuf), len(tbuf))
     return {"e_min": e.value, "icg": icg.value, "converged": bool(conv.value), "nfail": nfail.value,
-            "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy()}
+            "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy(), "t_log": tbuf[:kt].copy()}
+
+
+def ctx_handle():
+    """The libommhip context (void*) behind the 'g' storage, for omx_prof_* event timing."""
+    return lib().omm_c_ctx()

@@ -19,11 +19,11 @@ def _note(*a):
 
 
 def _bf16(x):
-    """Round-to-nearest-even to bfloat16, returned as float64."""
-    f = np.asarray(x, np.float32)
-    u = f.view(np.uint32).astype(np.uint64)
-    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+    """Round-to-nearest-even of float64 values to bfloat16 in one step (the kernels convert the
+    stored FP64 value directly; rounding through FP32 first differs near ties), as float64."""
+    u = np.ascontiguousarray(x, np.float64).view(np.uint64)
+    u = (u + np.uint64((1 << 44) - 1) + ((u >> np.uint64(45)) & np.uint64(1))) & ~np.uint64((1 << 45) - 1)
+    return u.view(np.float64)
 
 
 OPS = [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")]
