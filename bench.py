@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""OMM conjugate-gradient benchmark on one MI355X.
+
+Workload: the periodic 16x16x16 simple-cubic lattice, 4 orbitals per site (basis n = 16384,
+m = 8192 occupied states), block-sparse H and S from the seeded generator, C0 ~ U(-0.5,0.5)/sqrt(n),
+H shifted by the mid-gap eta (H - eta*S), solved by omm() (flavour 0) through the Fortran
+MatrixSwitch drop-in with device storage ('gdden' / 'gdcsc').
+
+A step is one CG iteration of omm() (2 BSR SpMMs, 4 Gram products, the fused gradient product,
+the fused line-search reduction, the quartic solve and the vector updates).  Inputs are resident
+in HBM before timing.  `value` = FP64 CG iterations/s over exactly --steps steps, timed between
+the stream-idle points that end each step's coefficient readback (t0 = start of the CG loop).
+The same measurement is repeated with BF16 and FP32 products ('hbf', 'h32'); --ttc adds the
+time-to-converge (|dE|/E <= 1e-10) of the FP64 and the mixed ('hmx': BF16 phase + FP64
+refinement) solves with the energy and density-matrix errors of the mixed run.
+
+Roofline: the dominant kernel is the FP64 MFMA GEMM (Gram and gradient products); `achieved` =
+algorithmic flops of all its launches in the timed call / their summed HIP-event durations on the
+library stream.  The CPU baseline times the oracle (oracle/omm_oracle.c, the reference algorithm
+restated in C, 'ref' operation order, OpenMP) on a row sample of one CG iteration's products.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "omm-bundle_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+# Mid-gap shifts eta = (lambda_m + lambda_{m+1})/2 of the generalised problem H x = lambda S x
+# for the default seed, from `python bench.py --compute-eta L` (scipy generalised eigensolver).
+ETA_MID_GAP = {8: 0.04129763547490679, 16: 0.05171565082133889}
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def compute_eta(L):
+    import scipy.linalg as sla
+
+    import ommhip as oh
+
+    nb = 4 * L ** 3
+    no = nb // 2
+    H = oh.bsr_to_dense(nb, *oh.lattice_bsr(L, 0))
+    S = oh.bsr_to_dense(nb, *oh.lattice_bsr(L, 1))
+    w = sla.eigh(H, S, eigvals_only=True, subset_by_index=[no - 1, no], driver="gvx")
+    return 0.5 * (w[0] + w[1]), w[1] - w[0]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def prof_totals(oh, ctxh):
+    """Per kernel class: (summed event ms, launches, algorithmic flops, algorithmic bytes)."""
+    out = {}
+    for k, name in ((oh.K_GEMM, "gemm"), (oh.K_SPMM, "spmm"), (oh.K_COEFF, "coeff"), (oh.K_ELEM, "elem"),
+                    (oh.K_DOT, "dot")):
+        ms, n = ctypes.c_double(0), ctypes.c_int64(0)
+        fl, by = ctypes.c_double(0), ctypes.c_double(0)
+        oh.check(oh.lib().omx_prof_read(ctxh, k, ctypes.byref(ms), ctypes.byref(n)), "prof_read")
+        oh.check(oh.lib().omx_prof_work(ctxh, k, ctypes.byref(fl), ctypes.byref(by)), "prof_work")
+        out[name] = (ms.value, n.value, fl.value, by.value)
+    return out
+
+
+def run_steps(F, oh, ctxh, mats, nb, no, eta, C0, operation, warmup, steps):
+    """Warm-up call, then a timed call of exactly `steps` CG steps.  Returns (iters/s, prof)."""
+    H, S, Cm = mats
+    F.set_option("max_iter", max(1, warmup))
+    Cm.upload(C0)
+    F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=1e-300, operation=operation, dealloc=True)
+    F.set_option("max_iter", steps)
+    Cm.upload(C0)
+    F.sync()
+    oh.check(oh.lib().omx_prof_reset(ctxh), "prof_reset")
+    oh.check(oh.lib().omx_prof_enable(ctxh, 1), "prof_enable")
+    r = F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=1e-300, operation=operation, dealloc=True)
+    oh.check(oh.lib().omx_prof_enable(ctxh, 0), "prof_disable")
+    F.set_option("max_iter", 0)
+    assert r["icg"] == steps, (r["icg"], steps)
+    dt = r["t_log"][steps - 1] - r["t0"]
+    return steps / dt, dt, prof_totals(oh, ctxh), r
+
+
+def cpu_baseline(nb, no, seconds_hint=20.0):
+    """Oracle timing on a row sample of one CG iteration's products (scaled by the sample factor)."""
+    import oracle as O
+
+    import ommhip as oh
+
+    L = round((nb // 4) ** (1 / 3))
+    frac = 16  # rows of the sample = no/16
+    rs = max(1, no // frac)
+    rng = np.random.default_rng(0)
+    A = np.asfortranarray(rng.standard_normal((rs, nb)))
+    B = np.asfortranarray(rng.standard_normal((no, nb)))
+    Cg = np.zeros((rs, no), order="F")
+    t = time.perf_counter()
+    O.gemm("n", "c", A, B, Cg)  # Gram sample: (no/16 x nb)(nb x no)
+    t_gram = (time.perf_counter() - t) * frac
+    W = np.asfortranarray(rng.standard_normal((rs, no)))
+    Gs = np.zeros((rs, nb), order="F")
+    t = time.perf_counter()
+    O.gemm("n", "n", W, B, Gs, -2.0, 1.0)  # gradient sample: (no/16 x no)(no x nb)
+    t_grad = (time.perf_counter() - t) * frac
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    colptr, rowidx, val = bsr_to_csc_fast(nb, cp, ri, bv)
+    Y = np.zeros((rs, nb), order="F")
+    t = time.perf_counter()
+    O.spmm_csc(A, colptr, rowidx, val, Y)  # SpMM sample
+    t_spmm = (time.perf_counter() - t) * frac
+    # per iteration: 4 Gram + 2 gradient GEMMs + 2 SpMMs (element-wise passes and traces not counted)
+    t_iter = 4 * t_gram + 2 * t_grad + 2 * t_spmm
+    return {"value": 1.0 / t_iter, "unit": "CG iters/s", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"oracle (C restatement of omm()/MatrixSwitch 'ref', OpenMP) on rows 1..{rs} of one Gram "
+                       f"(x4), one gradient GEMM (x2) and one BSR SpMM (x2) of a CG iteration at n={nb}, "
+                       f"scaled by {frac}; element-wise passes and traces excluded (favours the CPU)"),
+            "seconds_per_iter": t_iter}
+
+
+def bsr_to_csc_fast(nb, cp, ri, bv):
+    """0-based CSC of a BSR-4 operator (vectorised)."""
+    nbc = len(cp) - 1
+    counts = np.diff(cp)
+    J = np.repeat(np.arange(nbc), counts)
+    # entries (l, r, c): row 4*I + r, column 4*J + c, value bv[l, r + 4c]
+    l_idx = np.arange(len(ri))
+    rows = (4 * ri[:, None, None] + np.arange(4)[None, :, None]).repeat(4, axis=2)  # [l, r, c]
+    cols = (4 * J[:, None, None] + np.arange(4)[None, None, :]).repeat(4, axis=1)
+    vals = bv.reshape(-1, 4, 4, order="F")  # [l, r, c] = bv[l, r + 4c]
+    rows, cols, vals = rows.ravel(), cols.ravel(), vals.ravel()
+    keep = (rows < nb) & (cols < nb)
+    rows, cols, vals = rows[keep], cols[keep], vals[keep]
+    order = np.lexsort((rows, cols))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    colptr = np.zeros(nb + 1, np.int64)
+    np.add.at(colptr, cols + 1, 1)
+    colptr = np.cumsum(colptr)
+    del l_idx
+    return colptr.astype(np.int32), rows.astype(np.int32), vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--L", type=int, default=16, help="lattice edge: n = 4 L^3 basis functions, m = n/2")
+    ap.add_argument("--eta", type=float, default=None)
+    ap.add_argument("--ttc", action="store_true", help="also measure time-to-converge (FP64 and mixed)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lowp", action="store_true", help="skip the BF16 / FP32 step-rate runs")
+    ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
+    args = ap.parse_args()
+
+    if args.compute_eta:
+        eta, gap = compute_eta(args.compute_eta)
+        print(json.dumps({"L": args.compute_eta, "eta": eta, "gap": gap}))
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding of the OMM loop is a design note, not a measured path)", file=sys.stderr)
+        return 2
+
+    import ommhip as oh
+    import pyomm as F
+
+    L = args.L
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    eta = args.eta if args.eta is not None else ETA_MID_GAP.get(L)
+    if eta is None:
+        eta, _ = compute_eta(L)
+    log(f"bench: L={L} n={nb} m={no} eta={eta:.6f}")
+    t = time.time()
+    Hb = oh.lattice_bsr(L, 0)
+    Sb = oh.lattice_bsr(L, 1)
+    # H - eta*S on the union pattern (same pattern: onsite + 6 neighbour blocks)
+    assert np.array_equal(Hb[0], Sb[0]) and np.array_equal(Hb[1], Sb[1])
+    Hs = (Hb[0], Hb[1], Hb[2] - eta * Sb[2])
+    C0 = oh.random_c0(no, nb)
+    log(f"bench: inputs generated in {time.time() - t:.1f}s")
+
+    F.setup(0)
+    ctxh = F.ctx_handle()
+    H = F.Mat.from_bsr(nb, *Hs)
+    S = F.Mat.from_bsr(nb, *Sb)
+    Cm = F.Mat(no, nb)
+    mats = (H, S, Cm)
+    nnzb = len(Hb[1])
+
+    res = {}
+    ips, dt, prof, r64 = run_steps(F, oh, ctxh, mats, nb, no, eta, C0, "hip", args.warmup, args.steps)
+    log(f"bench: FP64 {ips:.3f} CG it/s ({dt:.3f}s for {args.steps} steps) e={r64['e_min']:.10f}")
+    res["fp64"] = (ips, dt, prof)
+    if not args.no_lowp:
+        for lab in ("hbf", "h32"):
+            i2, d2, p2, r2 = run_steps(F, oh, ctxh, mats, nb, no, eta, C0, lab, args.warmup, args.steps)
+            log(f"bench: {lab} {i2:.3f} CG it/s ({d2:.3f}s)")
+            res[lab] = (i2, d2, p2)
+
+    # roofline of the FP64 GEMM (dominant kernel) from the timed FP64 call: algorithmic flops
+    # (2 m n k per product, counted by the library per launch) over the summed event durations
+    steps = args.steps
+    gemm_ms, gemm_n, gemm_fl, _ = prof["gemm"]
+    achieved_tf = gemm_fl / (gemm_ms * 1e-3) / 1e12
+    spmm_ms, spmm_n, _, spmm_by = prof["spmm"]
+    spmm_gbs = spmm_by / (spmm_ms * 1e-3) / 1e9
+    coeff_ms, coeff_n, _, coeff_by = prof["coeff"]
+    coeff_gbs = coeff_by / (coeff_ms * 1e-3) / 1e9
+
+    line = {
+        "metric": "OMM CG iterations/s (FP64, n=16384, m=8192); time-to-converge |dE|/E<1e-10 with --ttc",
+        "value": round(ips, 4),
+        "unit": "CG iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice generator)",
+        "config": {"workload": f"omm flavour 0, {L}x{L}x{L} lattice, n={nb}, m={no}, eta={eta:.6f} (mid-gap)",
+                   "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm FP64 (v_mfma_f64_16x16x4f64)",
+                     "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "launches": gemm_n, "flops_per_launch": gemm_fl / max(gemm_n, 1),
+                     "avg_launch_ms": round(gemm_ms / max(gemm_n, 1), 4)},
+        "spmm": {"achieved_GBs": round(spmm_gbs, 1), "frac_hbm": round(spmm_gbs / PEAK_HBM_GBS, 4),
+                 "avg_launch_ms": round(spmm_ms / max(spmm_n, 1), 4), "bytes_per_launch": spmm_by / max(spmm_n, 1)},
+        "coeff_reduction": {"achieved_GBs": round(coeff_gbs, 1), "avg_launch_ms": round(coeff_ms / max(coeff_n, 1), 4)},
+        "kernel_ms_in_timed_call": {k: round(v[0], 3) for k, v in prof.items()},
+    }
+    for lab, key in (("hbf", "bf16"), ("h32", "fp32")):
+        if lab in res:
+            line[f"{key}_products_iters_per_s"] = round(res[lab][0], 4)
+            g = res[lab][2]["gemm"]
+            line[f"{key}_products_gemm_TFLOPs"] = round(g[2] / (g[0] * 1e-3) / 1e12, 2)
+
+    if args.ttc:
+        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0)
+
+    if not args.no_cpu:
+        t = time.time()
+        line["cpu_baseline"] = cpu_baseline(nb, no)
+        log(f"bench: cpu baseline {line['cpu_baseline']['value']:.5f} it/s (sampled in {time.time() - t:.1f}s)")
+    for M in mats:
+        M.free()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10):
+    H, S, Cm = mats
+    out = {"cg_tol": tol}
+    Dref = F.Mat(nb, nb)
+    Cm.upload(C0)
+    F.sync()
+    t = time.perf_counter()
+    r = F.omm(nb, no, H, S, Cm, D_min=Dref, init_C=True, eta=eta, cg_tol=tol, operation="hip", dealloc=True)
+    F.sync()
+    t64 = time.perf_counter() - t
+    out["fp64"] = {"seconds": t64, "icg": r["icg"], "e_min": r["e_min"], "converged": r["converged"]}
+    log(f"bench: ttc FP64 {t64:.2f}s icg={r['icg']} e={r['e_min']:.12f}")
+    Dmx = F.Mat(nb, nb)
+    Cm.upload(C0)
+    F.set_option("lowp_tol", 1e-6)
+    F.sync()
+    t = time.perf_counter()
+    rm = F.omm(nb, no, H, S, Cm, D_min=Dmx, init_C=True, eta=eta, cg_tol=tol, operation="hmx", dealloc=True)
+    F.sync()
+    tmx = time.perf_counter() - t
+    # ||D_mx - D_64||_F on the device
+    p1, ld = Dref.dptr()
+    p2, _ = Dmx.dptr()
+    oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
+    fro = ctypes.c_double(0)
+    oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
+    out["mixed"] = {"seconds": tmx, "icg": rm["icg"], "icg_bf16": rm["icg_lowp"], "e_min": rm["e_min"],
+                    "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
+                    "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
+                    "speedup_vs_fp64": t64 / tmx}
+    log(f"bench: ttc mixed {tmx:.2f}s icg={rm['icg']} (bf16 {rm['icg_lowp']}) e={rm['e_min']:.12f}")
+    Dref.free()
+    Dmx.free()
+    return out
+
+
+if __name__ == "__main__":
+    sys.exit(main())

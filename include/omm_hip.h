@@ -191,6 +191,11 @@ int omx_prof_enable(omx_ctx ctx, int enable);
 /* Total milliseconds and launch count accumulated for a kernel class (synchronises). */
 int omx_prof_read(omx_ctx ctx, int kclass, double *ms, int64_t *launches);
 int omx_prof_reset(omx_ctx ctx);
+/* Algorithmic flops and minimum HBM bytes of the launches timed for a kernel class:
+ * products 2*m*n*k flops (operands read once, result written once); SpMM 2*nnz*no flops,
+ * X read + Y written + block values and indices; reductions and element-wise passes: each
+ * operand element read or written once. */
+int omx_prof_work(omx_ctx ctx, int kclass, double *flops, double *bytes);
 
 /* ==== synthetic inputs ====================================================================== */
 

@@ -38,3 +38,10 @@ done
 "$FC" -shared -fPIC -o "$HERE/libommf.so.tmp" "${FOBJS[@]}" -L"$HERE" -lommhip -Wl,-rpath,'$ORIGIN'
 mv "$HERE/libommf.so.tmp" "$HERE/libommf.so"
 echo "built $HERE/libommf.so"
+
+# Fortran example programs (link against the drop-in)
+for ex in example_gamma_hip example_sdden_hip; do
+  "$FC" -cpp -O2 -I"$OBJDIR/f" -o "$HERE/examples/$ex" "$HERE/examples/$ex.F90" -L"$HERE" -lommf -lommhip \
+    -Wl,-rpath,'$ORIGIN/..'
+done
+echo "built examples"

@@ -481,7 +481,16 @@ int omx_prof_reset(omx_ctx ctx) {
     for (int k = 0; k < OMX_K_COUNT; ++k) {
         ctx->prof_ms[k] = 0;
         ctx->prof_count[k] = 0;
+        ctx->prof_flops[k] = 0;
+        ctx->prof_bytes[k] = 0;
     }
+    return OMX_OK;
+}
+
+int omx_prof_work(omx_ctx ctx, int kclass, double *flops, double *bytes) {
+    OMX_REQUIRE(ctx && kclass >= 0 && kclass < OMX_K_COUNT, "omx_prof_work: bad arguments");
+    if (flops) *flops = ctx->prof_flops[kclass];
+    if (bytes) *bytes = ctx->prof_bytes[kclass];
     return OMX_OK;
 }
 

@@ -426,6 +426,7 @@ int omx_omm_coeff(omx_ctx ctx, int dt, int n, int64_t ldm, const void *HW, const
     double *partial = (double *)ctx->ws;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_COEFF, &ev);
+    prof_work(ctx, OMX_K_COEFF, 26.0 * n * (double)n, (dt == OMX_F64 ? 8.0 : 4.0) * 6.0 * n * (double)n);
     if (dt == OMX_F64)
         hipLaunchKernelGGL(k_coeff_partial<double>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
                            (const double *)HW, (const double *)SW, (const double *)HWd, (const double *)SWd,

@@ -363,6 +363,8 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
     }
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_GEMM, &ev);
+    prof_work(ctx, OMX_K_GEMM, 2.0 * m * n * (double)ktot,
+              sizeof(TS) * ((double)NPAIR * ((double)m * k + (double)k * n) + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
     hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR>), dim3(tiles, 1, splits), dim3(NT), 0, ctx->stream, m, n, k,
                        alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, partial);
     if (splits > 1) {

@@ -58,6 +58,8 @@ struct omx_ctx_s {
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[OMX_K_COUNT] = {0};
     int64_t prof_count[OMX_K_COUNT] = {0};
+    double prof_flops[OMX_K_COUNT] = {0};  // algorithmic flops of the timed launches
+    double prof_bytes[OMX_K_COUNT] = {0};  // algorithmic (minimum) HBM bytes of the timed launches
     int num_cus = 256;
 };
 
@@ -83,6 +85,12 @@ int ensure_ws(omx_ctx ctx, size_t bytes);
 // Begin / end a timed launch of kernel class `kclass` (no-ops unless profiling is on).
 void prof_begin(omx_ctx ctx, int kclass, hipEvent_t *ev_start);
 void prof_end(omx_ctx ctx, int kclass, hipEvent_t ev_start);
+// Add the algorithmic work of one launch to the class totals (only while profiling is on).
+inline void prof_work(omx_ctx ctx, int kclass, double flops, double bytes) {
+    if (!ctx->prof_on) return;
+    ctx->prof_flops[kclass] += flops;
+    ctx->prof_bytes[kclass] += bytes;
+}
 
 inline bool valid_real_dt(int dt) { return dt == OMX_F64 || dt == OMX_F32; }
 inline int dt_size(int dt) {

@@ -162,6 +162,8 @@ static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64
     OMX_REQUIRE(total < (1LL << 31), "omx_spmm: grid too large");
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_SPMM, &ev);
+    prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * A->nnzb * no,
+              sizeof(T) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (16.0 * 8 + 4) + 4.0 * (A->nbc + 1));
     hipLaunchKernelGGL((k_spmm_bsr4<T, VEC>), dim3((unsigned)total), dim3(SPMM_NT), 0, ctx->stream, no, nb, A->nbc,
                        alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, rchunks, jper);
     prof_end(ctx, OMX_K_SPMM, ev);

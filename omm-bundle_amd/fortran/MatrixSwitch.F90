@@ -58,7 +58,7 @@ module MatrixSwitch
   end interface
 
   public :: m_allocate, m_deallocate, m_copy, mm_multiply, m_add, m_trace, mm_trace, m_scale, m_set
-  public :: m_set_element, m_get_element, m_register_sden
+  public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr
   public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download
   public :: ms_label_prec, die
 
@@ -314,6 +314,30 @@ contains
     m_name%dval_is_allocated = .false.
     m_name%is_initialized = .true.
   end subroutine m_register_sden
+
+  !> Copy a block-sparse operator given as 0-based BSR-4 block columns (block-CSC: bcolptr has
+  !> ceil(nb/4)+1 entries, block rows ascending, 16 column-major values per block) into device
+  !> storage ('gdcsc').  The natural input format for tight-binding Hamiltonians with 4-orbital
+  !> sites; like m_register_pcsc (MatrixSwitch_m_register.F90:394-443) the data are copied.
+  subroutine m_register_gbsr(m_name, nb, bcolptr, browidx, bval)
+    type(matrix), intent(inout) :: m_name
+    integer, intent(in) :: nb
+    integer(c_int), intent(in), target :: bcolptr(:), browidx(:)
+    real(dp), intent(in), target :: bval(:)
+    integer(c_int) :: rc
+    m_name%str_type = 'csc'
+    m_name%is_serial = .true.
+    m_name%is_real = .true.
+    m_name%is_sparse = .true.
+    m_name%is_device = .true.
+    m_name%dtype = OMX_F64
+    m_name%dim1 = nb
+    m_name%dim2 = nb
+    m_name%is_square = .true.
+    rc = omx_bsr_create(ctx(), int(nb, c_int), c_loc(bcolptr), c_loc(browidx), c_loc(bval), m_name%bsr)
+    call chk(rc, 'm_register_gbsr')
+    m_name%is_initialized = .true.
+  end subroutine m_register_gbsr
 
   !> Host <-> device transfers of a dense matrix.
   subroutine ms_upload(m_name, A)

@@ -290,6 +290,7 @@ contains
     conv = .false.
     capped = .false.
     steps = 0
+    stat_t0 = wall_seconds()  ! the coefficient readback that precedes this point left the stream idle
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
       do inner = 1, n_step_max_inner

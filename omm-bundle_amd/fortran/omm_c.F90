@@ -133,6 +133,23 @@ contains
     omm_c_mat_from_csc = h
   end function omm_c_mat_from_csc
 
+  !> Device sparse operator from 0-based BSR-4 block columns (m_register_gbsr); returns a handle.
+  integer(c_int) function omm_c_mat_from_bsr(nb, bcolptr, browidx, bval) bind(c, name='omm_c_mat_from_bsr')
+    integer(c_int), value :: nb
+    integer(c_int), intent(in) :: bcolptr(*), browidx(*)
+    real(c_double), intent(in) :: bval(*)
+    integer :: h, nbc, nnzb
+    h = new_handle()
+    if (h < 0) then
+      omm_c_mat_from_bsr = -1
+      return
+    end if
+    nbc = (nb + 3)/4
+    nnzb = bcolptr(nbc+1)
+    call m_register_gbsr(table(h), int(nb), bcolptr(1:nbc+1), browidx(1:max(nnzb,1)), bval(1:max(16*nnzb,1)))
+    omm_c_mat_from_bsr = h
+  end function omm_c_mat_from_bsr
+
   integer(c_int) function omm_c_mat_upload(h, a) bind(c, name='omm_c_mat_upload')
     integer(c_int), value :: h
     type(c_ptr), value :: a
@@ -234,14 +251,16 @@ contains
     omm_c_elog = k
   end function omm_c_elog
 
-  !> Wall-clock seconds at the end of each CG step of the last call (up to cap); returns the count.
+  !> Wall-clock seconds at the start of the CG loop (t0, returned first) and at the end of each CG
+  !> step of the last call (up to cap-1); returns the number written.
   integer(c_int) function omm_c_tlog(buf, cap) bind(c, name='omm_c_tlog')
     real(c_double), intent(out) :: buf(*)
     integer(c_int), value :: cap
     integer :: k
-    k = min(int(cap), stat_nlog)
-    buf(1:k) = stat_tlog(1:k)
-    omm_c_tlog = k
+    k = min(int(cap) - 1, stat_nlog)
+    buf(1) = stat_t0
+    buf(2:k+1) = stat_tlog(1:k)
+    omm_c_tlog = k + 1
   end function omm_c_tlog
 
   !> The libommhip context used by the 'g' storage (for event timing from the caller).

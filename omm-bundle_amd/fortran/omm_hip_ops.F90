@@ -30,6 +30,7 @@ module omm_hip_ops
   integer, parameter :: stat_log_cap = 100000
   real(dp), save :: stat_elog(stat_log_cap)
   real(dp), save :: stat_tlog(stat_log_cap)   ! wall-clock seconds at the end of each CG step
+  real(dp), save :: stat_t0 = 0.0_dp           ! wall-clock seconds when the (last) CG phase started
   integer, save :: stat_nlog = 0
 
 contains

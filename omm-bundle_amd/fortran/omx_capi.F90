@@ -71,6 +71,12 @@ module omx_capi
       integer(c_int), value :: nb, ptrbase, rowbase
       type(c_ptr), intent(out) :: out
     end function
+    integer(c_int) function omx_bsr_create(ctx, nb, bcolptr, browidx, bval, out) bind(c, name='omx_bsr_create')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, bcolptr, browidx, bval
+      integer(c_int), value :: nb
+      type(c_ptr), intent(out) :: out
+    end function
     integer(c_int) function omx_bsr_destroy(a) bind(c, name='omx_bsr_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: a

@@ -63,6 +63,7 @@ SIGNATURES = {
     "omx_prof_enable": (_i, [_vp, _i]),
     "omx_prof_read": (_i, [_vp, _i, _pd, C.POINTER(_i64)]),
     "omx_prof_reset": (_i, [_vp]),
+    "omx_prof_work": (_i, [_vp, _i, _pd, _pd]),
     "omx_lattice_bsr": (_i, [_i, _i, _i, C.c_uint64, _i, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_lattice_bsr_theta": (_i, [_i, _i, _i, _vp, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_lattice_theta0": (_i, [C.c_uint64, _vp]),
@@ -218,6 +219,11 @@ class Ctx:
 
     def prof_reset(self):
         check(lib().omx_prof_reset(self.h), "prof_reset")
+
+    def prof_work(self, kclass):
+        fl, by = C.c_double(0), C.c_double(0)
+        check(lib().omx_prof_work(self.h, kclass, C.byref(fl), C.byref(by)), "prof_work")
+        return fl.value, by.value
 
     def prof_read(self, kclass):
         ms, n = C.c_double(0), C.c_int64(0)

@@ -28,6 +28,7 @@ def lib():
         L.omm_c_setup.argtypes = [_i]
         L.omm_c_mat_alloc.argtypes = [_i, _i, C.c_char_p]
         L.omm_c_mat_from_csc.argtypes = [_i, _vp, _vp, _vp, C.c_char_p]
+        L.omm_c_mat_from_bsr.argtypes = [_i, _vp, _vp, _vp]
         L.omm_c_mat_upload.argtypes = [_i, _vp]
         L.omm_c_mat_download.argtypes = [_i, _vp]
         L.omm_c_mat_free.argtypes = [_i]
@@ -70,6 +71,20 @@ class Mat:
         self.h = lib().omm_c_mat_from_csc(nb, _p(cp), _p(ri), _p(v), label.encode())
         if self.h <= 0:
             raise RuntimeError("omm_c_mat_from_csc failed")
+        return self
+
+    @classmethod
+    def from_bsr(cls, nb, bcolptr, browidx, bval):
+        """Device operator ('gdcsc') from 0-based BSR-4 block columns."""
+        self = cls.__new__(cls)
+        self.rows = self.cols = nb
+        self.label = "gdcsc"
+        cp = np.ascontiguousarray(bcolptr, np.int32)
+        ri = np.ascontiguousarray(browidx, np.int32)
+        v = np.ascontiguousarray(bval, np.float64)
+        self.h = lib().omm_c_mat_from_bsr(nb, _p(cp), _p(ri), _p(v))
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_from_bsr failed")
         return self
 
     def upload(self, a):
@@ -125,7 +140,8 @@ def omm(m, n, H: Mat, S: Mat | None, C_min: Mat, *, new_S=True, D_min: Mat | Non
     tbuf = np.zeros(100000)
     kt = lib().omm_c_tlog(_p(tbuf), len(tbuf))
     return {"e_min": e.value, "icg": icg.value, "converged": bool(conv.value), "nfail": nfail.value,
-            "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy(), "t_log": tbuf[:kt].copy()}
+            "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy(), "t0": tbuf[0],
+            "t_log": tbuf[1:kt].copy()}
 
 
 def ctx_handle():

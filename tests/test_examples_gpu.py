@@ -1,0 +1,32 @@
+"""Fortran programs that use the drop-in exactly as a reference caller would (use MatrixSwitch,
+call omm) with device storage; they check the reference's own constants and exit non-zero on a
+mismatch (example_gamma: 1e-4, example_sdden: 1e-10)."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EX = os.path.join(ROOT, "omm-bundle_amd", "examples")
+
+
+def run(args, cwd):
+    print("CASE", " ".join(args), flush=True)
+    p = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=120)
+    print(p.stdout)
+    return p
+
+
+@pytest.mark.parametrize("op", ["hip", "hmx"])
+def test_example_gamma_hip(tmp_path, op):
+    p = run([os.path.join(EX, "example_gamma_hip"), os.path.join(ROOT, "tests", "golden", "example_gamma.dat"), op],
+            tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "e_min [2]" in p.stdout and "ED_11" in p.stdout
+
+
+def test_example_sdden_hip(tmp_path):
+    p = run([os.path.join(EX, "example_sdden_hip")], tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "E_11" in p.stdout
