@@ -8,8 +8,9 @@ MatrixSwitch drop-in with device storage ('gdden' / 'gdcsc').
 
 A step is one CG iteration of omm() (2 BSR SpMMs, 4 Gram products, the fused gradient product,
 the fused line-search reduction, the quartic solve and the vector updates).  Inputs are resident
-in HBM before timing.  `value` = FP64 CG iterations/s over exactly --steps steps, timed between
-the stream-idle points that end each step's coefficient readback (t0 = start of the CG loop).
+in HBM before timing.  `value` = FP64 CG iterations/s over exactly --steps complete iterations,
+timed between the stream-idle points that end the coefficient readbacks of CG steps 1 and
+steps+1 of one omm() call.
 The same measurement is repeated with BF16 and FP32 products ('hbf', 'h32'); --ttc adds the
 time-to-converge (|dE|/E <= 1e-10) of the FP64 and the mixed ('hmx': BF16 phase + FP64
 refinement) solves with the energy and density-matrix errors of the mixed run.
@@ -75,7 +76,11 @@ def run_steps(F, oh, ctxh, mats, nb, no, eta, C0, operation, warmup, steps):
     F.set_option("max_iter", max(1, warmup))
     Cm.upload(C0)
     F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=1e-300, operation=operation, dealloc=True)
-    F.set_option("max_iter", steps)
+    # The first CG step reuses the line-search data of the initial energy evaluation and the last
+    # one stops before the gradient update, so the timed window runs from the end of step 1 to
+    # the end of step steps+1: exactly `steps` complete iterations (gradient, direction, 2 SpMM,
+    # 4 Gram products, fused coefficients, line search, updates).
+    F.set_option("max_iter", steps + 1)
     Cm.upload(C0)
     F.sync()
     oh.check(oh.lib().omx_prof_reset(ctxh), "prof_reset")
@@ -83,8 +88,8 @@ def run_steps(F, oh, ctxh, mats, nb, no, eta, C0, operation, warmup, steps):
     r = F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=1e-300, operation=operation, dealloc=True)
     oh.check(oh.lib().omx_prof_enable(ctxh, 0), "prof_disable")
     F.set_option("max_iter", 0)
-    assert r["icg"] == steps, (r["icg"], steps)
-    dt = r["t_log"][steps - 1] - r["t0"]
+    assert r["icg"] == steps + 1, (r["icg"], steps)
+    dt = r["t_log"][steps] - r["t_log"][0]
     return steps / dt, dt, prof_totals(oh, ctxh), r
 
 

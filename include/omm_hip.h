@@ -137,6 +137,11 @@ int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, i
  */
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
              int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+/* Force the GEMM tile configuration of later products on this context (-1 restores the choice
+ * by shape) for the FP64 Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
+ * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256.  Used by the tile autotuner; results do
+ * not depend on it. */
+int omx_gemm_set_tile(omx_ctx ctx, int cfg);
 /* C := alpha*op(A) + beta*C  (m_add, MatrixSwitch.F90:1731-1902; MatrixSwitch_m_add.F90:95-175). */
 int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const void *A, int64_t lda, double beta,
             void *C, int64_t ldc);

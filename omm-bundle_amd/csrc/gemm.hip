@@ -7,15 +7,17 @@
 // and the three precisions:
 //   FP64: v_mfma_f64_16x16x4f64, FP64 accumulate (the oracle-grade path);
 //   FP32: operands rounded to FP32, v_mfma_f32_16x16x4f32, FP32 accumulate;
-//   BF16: operands rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulate.
+//   BF16: operands rounded to BF16 (one rounding from the stored value),
+//         v_mfma_f32_16x16x32_bf16, FP32 accumulate.
 //
-// Work decomposition: 256-thread workgroups (4 wave64s, 2x2) own a BM x BN tile of C; each wave
-// owns a (BM/2) x (BN/2) sub-tile held as 16x16 MFMA accumulator blocks.  K is consumed in BK
-// slices staged through LDS (double-buffered: the next slice is fetched into registers while
-// the current one feeds the MFMAs).  Large K with few output tiles is split over grid.z into
-// FP64/FP32 partial tiles that a second kernel reduces in a fixed order (deterministic).
-// The linear workgroup id is remapped so that each XCD (ids are dealt round-robin over the 8
-// XCDs) walks a contiguous band of tile rows, keeping its A panels in its own L2.
+// Work decomposition: a workgroup of WGM x WGN wave64s owns a BM x BN tile of C; each wave owns
+// a (BM/WGM) x (BN/WGN) sub-tile held as 16x16 MFMA accumulator blocks.  K is consumed in BK
+// slices staged through LDS and double-buffered: the next slice is fetched from HBM/L2 into
+// registers (16-byte vector loads along the contiguous index) while the current slice feeds the
+// MFMAs.  When the tile count cannot fill the chip, K is split over grid.z into partial tiles
+// that a second kernel adds in split order (deterministic).  The linear workgroup id is remapped
+// so that each XCD (ids are dealt round-robin over the 8 XCDs) walks a contiguous band of tiles,
+// grouped GM tile-rows at a time, so operand panels are shared inside one XCD's L2.
 #include <algorithm>
 
 #include "omx_internal.h"
@@ -26,18 +28,9 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
-template <typename T>
-__device__ inline double to_d(T v) { return (double)v; }
-
 // ---------------------------------------------------------------------------------------------
-// Tile geometry
+// Precision traits: K slice, LDS element type, accumulator type and result lane layout.
 // ---------------------------------------------------------------------------------------------
-constexpr int BM = 128, BN = 128;
-constexpr int WM = 64, WN = 64;          // per-wave sub-tile
-constexpr int NT = 256;                  // threads per workgroup
-constexpr int FB = WM / 16;              // 16x16 blocks per wave along M (and N)
-
-// Per-precision traits: K slice, LDS element type, and the MFMA step.
 template <int PREC>
 struct Prec;
 
@@ -47,7 +40,6 @@ struct Prec<OMX_PREC_F64> {
     typedef double lds_t;
     typedef double acc_t;
     typedef d4v accv;
-    static constexpr int KSTEP = 4;
     // v_mfma_f64_16x16x4f64 result: lane l, register pair v -> row (l/16) + 4v, column l%16
     __device__ static inline int out_row(int lq, int v) { return lq + 4 * v; }
 };
@@ -57,7 +49,6 @@ struct Prec<OMX_PREC_F32> {
     typedef float lds_t;
     typedef float acc_t;
     typedef f4v accv;
-    static constexpr int KSTEP = 4;
     // v_mfma_f32_16x16x4f32 result: lane l, register v -> row 4(l/16) + v, column l%16
     __device__ static inline int out_row(int lq, int v) { return 4 * lq + v; }
 };
@@ -67,7 +58,6 @@ struct Prec<OMX_PREC_BF16> {
     typedef __bf16 lds_t;
     typedef float acc_t;
     typedef f4v accv;
-    static constexpr int KSTEP = 32;
     // v_mfma_f32_16x16x32_bf16 result: lane l, register v -> row 4(l/16) + v, column l%16
     __device__ static inline int out_row(int lq, int v) { return 4 * lq + v; }
 };
@@ -76,77 +66,154 @@ template <typename TS, int PREC>
 __device__ inline typename Prec<PREC>::lds_t to_lds(TS v) {
     return (typename Prec<PREC>::lds_t)v;
 }
-template <>
-__device__ inline __bf16 to_lds<double, OMX_PREC_BF16>(double v) { return (__bf16)(float)v; }
-template <>
-__device__ inline __bf16 to_lds<float, OMX_PREC_BF16>(float v) { return (__bf16)v; }
+
+// ---------------------------------------------------------------------------------------------
+// Tile configurations.
+// ---------------------------------------------------------------------------------------------
+template <int BM_, int BN_, int WGM_, int WGN_, int MINW_>
+struct Cfg {
+    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_;
+    static constexpr int NT = 64 * WGM * WGN;
+    static constexpr int WM = BM / WGM, WN = BN / WGN;
+    static constexpr int FBM = WM / 16, FBN = WN / 16;
+    static constexpr int MINW = MINW_;  // waves per SIMD the register budget must allow
+};
+// Ids used by the selector and by omx_gemm_set_tile().
+typedef Cfg<128, 128, 2, 2, 2> Cfg0;  // 4 waves, 64x64 per wave, two workgroups per CU
+typedef Cfg<128, 128, 2, 2, 1> Cfg1;  // 4 waves, 64x64 per wave, one workgroup per CU
+typedef Cfg<256, 128, 2, 2, 1> Cfg2;  // 4 waves, 128x64 per wave
+typedef Cfg<128, 256, 2, 2, 1> Cfg3;  // 4 waves, 64x128 per wave
+constexpr int kNumCfg = 4;
 
 // LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
 // lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
 // consecutive k values per MFMA step as one 16-byte read.
-template <int PREC>
-struct Lds {
+template <int PREC, int ROWS>
+struct LdsT {
     static constexpr int BK = Prec<PREC>::BK;
     static constexpr bool KMAJOR = (PREC != OMX_PREC_BF16);
     static constexpr int PAD = KMAJOR ? 4 : 8;
-    // k-major: [BK][BM+PAD]; i-major: [BM][BK+PAD]
-    static constexpr int A_ELEMS = KMAJOR ? BK * (BM + PAD) : BM * (BK + PAD);
-    static constexpr int B_ELEMS = KMAJOR ? BK * (BN + PAD) : BN * (BK + PAD);
-    __device__ static inline int a_off(int i, int k) { return KMAJOR ? k * (BM + PAD) + i : i * (BK + PAD) + k; }
-    __device__ static inline int b_off(int j, int k) { return KMAJOR ? k * (BN + PAD) + j : j * (BK + PAD) + k; }
+    static constexpr int ELEMS = KMAJOR ? BK * (ROWS + PAD) : ROWS * (BK + PAD);
+    __device__ static inline int off(int r, int k) { return KMAJOR ? k * (ROWS + PAD) + r : r * (BK + PAD) + k; }
 };
 
 // ---------------------------------------------------------------------------------------------
 // Global -> register staging of one BK slice of op(A) (BM x BK) or op(B)^T (BN x BK).
-// "Row" index r (i for A, j for B) and k; CONTIG_R: the row index is contiguous in memory
-// (A 'n', B 't'); otherwise k is contiguous (A 't', B 'n').  Each thread loads ELEMS values.
+// "Row" index r (i for A, j for B) and k.  CONTIG_R: r is contiguous in memory (A 'n', B 't');
+// otherwise k is (A 't', B 'n').  Loads are VEC-wide vectors along the contiguous index when
+// the vector lies inside the matrix and the leading dimension keeps it 16-byte aligned.
 // ---------------------------------------------------------------------------------------------
-template <typename TS, int PREC, bool CONTIG_R, int ROWS>
+template <typename TS, int PREC, bool CONTIG_R, int ROWS, int NT>
 struct Stage {
     static constexpr int BK = Prec<PREC>::BK;
-    static constexpr int ELEMS = ROWS * BK / NT;
-    typename Prec<PREC>::lds_t v[ELEMS];
+    static constexpr int VEC = 16 / sizeof(TS);
+    // BF16 with contiguous rows: a thread owns a micro-tile of TR rows x TK consecutive k, loaded
+    // as TK vectors along the rows and written to the i-major LDS tile as TR runs of TK k values
+    // (a register transpose).  Every other case: vectors along the contiguous index.
+    static constexpr bool MICRO = (PREC == OMX_PREC_BF16 && CONTIG_R);
+    static constexpr int TR = 4, TK = 8;
+    static constexpr int NE = ROWS * BK / NT;  // elements per thread
+    static constexpr int NV = NE / VEC;
+    static_assert(NE % VEC == 0, "slice not divisible into vectors");
+    static_assert(!MICRO || (NE == TR * TK && TR % VEC == 0), "micro-tile shape");
+    typename Prec<PREC>::lds_t v[NE];
 
-    // Element e of this thread -> (r, k) within the slice.  For CONTIG_R the fastest index
-    // across threads is r (coalesced along r); otherwise k.  For BF16 with CONTIG_R, each thread
-    // holds KSEG consecutive k of one r so that the LDS write ([r][k] layout) is one vector store.
-    __device__ static inline void coord(int t, int e, int &r, int &k) {
-        if (PREC == OMX_PREC_BF16 && CONTIG_R) {
-            // ELEMS = ROWS*BK/NT; thread t owns row r = t % ROWS (x ROWS/NT groups) and a run of k.
-            constexpr int RUNS = NT / ROWS;  // threads per row
-            constexpr int RUN = BK / RUNS;   // consecutive k per thread (== ELEMS)
-            r = t % ROWS;
-            k = (t / ROWS) * RUN + e;
-        } else if (CONTIG_R) {
-            int idx = t + NT * e;
-            r = idx % ROWS;
-            k = idx / ROWS;
+    // Vector q of this thread -> (r, k) of its first element (non-MICRO modes).
+    __device__ static inline void coord(int t, int q, int &r, int &k) {
+        if constexpr (CONTIG_R) {
+            int idx = t + NT * q;
+            r = (idx % (ROWS / VEC)) * VEC;
+            k = idx / (ROWS / VEC);
         } else {
-            int idx = t + NT * e;
-            k = idx % BK;
-            r = idx / BK;
+            int idx = t + NT * q;
+            k = (idx % (BK / VEC)) * VEC;
+            r = idx / (BK / VEC);
         }
     }
+    // MICRO mode: micro-tile origin of thread t.
+    __device__ static inline void micro_origin(int t, int &r, int &k) {
+        r = (t % (ROWS / TR)) * TR;
+        k = (t / (ROWS / TR)) * TK;
+    }
 
-    __device__ inline void load(const TS *__restrict__ P, int64_t ld, int r0, int k0, int rlim, int klim, int t) {
-#pragma unroll
-        for (int e = 0; e < ELEMS; ++e) {
+    __device__ inline void load(const TS *__restrict__ P, int64_t ld, int r0, int k0, int rlim, int klim, bool vec_ok,
+                                int t) {
+        if constexpr (MICRO) {
             int r, k;
-            coord(t, e, r, k);
-            int gr = r0 + r, gk = k0 + k;
-            TS x = (TS)0;
-            if (gr < rlim && gk < klim) x = CONTIG_R ? P[(int64_t)gr + (int64_t)gk * ld] : P[(int64_t)gk + (int64_t)gr * ld];
-            v[e] = to_lds<TS, PREC>(x);
+            micro_origin(t, r, k);
+            const int gr = r0 + r;
+#pragma unroll
+            for (int kk = 0; kk < TK; ++kk) {
+                const int gk = k0 + k + kk;
+                const TS *src = P + (int64_t)gr + (int64_t)gk * ld;
+                TS x[TR];
+                if (vec_ok && gr + TR <= rlim && gk < klim && TR % VEC == 0) {
+                    typedef TS V __attribute__((ext_vector_type(VEC)));
+#pragma unroll
+                    for (int h = 0; h < TR / VEC; ++h) {
+                        V w = *reinterpret_cast<const V *>(src + h * VEC);
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) x[h * VEC + e] = w[e];
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < TR; ++e) x[e] = (gr + e < rlim && gk < klim) ? src[e] : (TS)0;
+                }
+#pragma unroll
+                for (int e = 0; e < TR; ++e) v[e * TK + kk] = to_lds<TS, PREC>(x[e]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                int r, k;
+                coord(t, q, r, k);
+                const int gr = r0 + r, gk = k0 + k;
+                const bool full = CONTIG_R ? (gr + VEC <= rlim && gk < klim) : (gr < rlim && gk + VEC <= klim);
+                const TS *src = CONTIG_R ? P + (int64_t)gr + (int64_t)gk * ld : P + (int64_t)gk + (int64_t)gr * ld;
+                TS x[VEC];
+                if (vec_ok && full) {
+                    typedef TS V __attribute__((ext_vector_type(VEC)));
+                    V w = *reinterpret_cast<const V *>(src);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) x[e] = w[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const bool in = CONTIG_R ? (gr + e < rlim && gk < klim) : (gr < rlim && gk + e < klim);
+                        x[e] = in ? src[e] : (TS)0;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) v[q * VEC + e] = to_lds<TS, PREC>(x[e]);
+            }
         }
     }
 
-    template <bool IS_A>
     __device__ inline void store(typename Prec<PREC>::lds_t *lds, int t) const {
-#pragma unroll
-        for (int e = 0; e < ELEMS; ++e) {
+        if constexpr (MICRO) {
+            // one 16-byte write of TK = 8 consecutive k per row of the micro-tile
             int r, k;
-            coord(t, e, r, k);
-            lds[IS_A ? Lds<PREC>::a_off(r, k) : Lds<PREC>::b_off(r, k)] = v[e];
+            micro_origin(t, r, k);
+#pragma unroll
+            for (int e = 0; e < TR; ++e) {
+                bf8v w;
+#pragma unroll
+                for (int kk = 0; kk < TK; ++kk) w[kk] = v[e * TK + kk];
+                *reinterpret_cast<bf8v *>(&lds[LdsT<PREC, ROWS>::off(r + e, k)]) = w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                int r, k;
+                coord(t, q, r, k);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    if (CONTIG_R)
+                        lds[LdsT<PREC, ROWS>::off(r + e, k)] = v[q * VEC + e];
+                    else
+                        lds[LdsT<PREC, ROWS>::off(r, k + e)] = v[q * VEC + e];
+                }
+            }
         }
     }
 };
@@ -154,60 +221,52 @@ struct Stage {
 // ---------------------------------------------------------------------------------------------
 // MFMA step over one BK slice held in LDS.
 // ---------------------------------------------------------------------------------------------
-template <int PREC>
+template <int PREC, class CF>
 __device__ inline void mma_slice(const typename Prec<PREC>::lds_t *As, const typename Prec<PREC>::lds_t *Bs,
-                                 typename Prec<PREC>::accv (&acc)[FB][FB], int wm, int wn, int lane) {
+                                 typename Prec<PREC>::accv (&acc)[CF::FBM][CF::FBN], int wm, int wn, int lane) {
     constexpr int BK = Prec<PREC>::BK;
+    typedef LdsT<PREC, CF::BM> LA;
+    typedef LdsT<PREC, CF::BN> LB;
     const int li = lane & 15, lk = lane >> 4;
-    if constexpr (PREC == OMX_PREC_F64) {
+    if constexpr (PREC == OMX_PREC_F64 || PREC == OMX_PREC_F32) {
+        typedef typename Prec<PREC>::lds_t T;
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
-            double a[FB], b[FB];
+            T a[CF::FBM], b[CF::FBN];
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                a[f] = As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + lk)];
-                b[f] = Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + lk)];
-            }
+            for (int f = 0; f < CF::FBM; ++f) a[f] = As[LA::off(wm * CF::WM + f * 16 + li, ks + lk)];
 #pragma unroll
-            for (int fm = 0; fm < FB; ++fm)
+            for (int f = 0; f < CF::FBN; ++f) b[f] = Bs[LB::off(wn * CF::WN + f * 16 + li, ks + lk)];
 #pragma unroll
-                for (int fn = 0; fn < FB; ++fn)
-                    acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
-        }
-    } else if constexpr (PREC == OMX_PREC_F32) {
+            for (int fm = 0; fm < CF::FBM; ++fm)
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 4) {
-            float a[FB], b[FB];
-#pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                a[f] = As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + lk)];
-                b[f] = Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + lk)];
-            }
-#pragma unroll
-            for (int fm = 0; fm < FB; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < FB; ++fn)
-                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+                for (int fn = 0; fn < CF::FBN; ++fn) {
+                    if constexpr (PREC == OMX_PREC_F64)
+                        acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+                    else
+                        acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+                }
         }
     } else {
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
-            bf8v a[FB], b[FB];
+            bf8v a[CF::FBM], b[CF::FBN];
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                a[f] = *reinterpret_cast<const bf8v *>(&As[Lds<PREC>::a_off(wm * WM + f * 16 + li, ks + 8 * lk)]);
-                b[f] = *reinterpret_cast<const bf8v *>(&Bs[Lds<PREC>::b_off(wn * WN + f * 16 + li, ks + 8 * lk)]);
-            }
+            for (int f = 0; f < CF::FBM; ++f)
+                a[f] = *reinterpret_cast<const bf8v *>(&As[LA::off(wm * CF::WM + f * 16 + li, ks + 8 * lk)]);
 #pragma unroll
-            for (int fm = 0; fm < FB; ++fm)
+            for (int f = 0; f < CF::FBN; ++f)
+                b[f] = *reinterpret_cast<const bf8v *>(&Bs[LB::off(wn * CF::WN + f * 16 + li, ks + 8 * lk)]);
 #pragma unroll
-                for (int fn = 0; fn < FB; ++fn)
+            for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < CF::FBN; ++fn)
                     acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
         }
     }
 }
 
-// Tile index for workgroup id w: XCD-contiguous bands of tile rows, grouped raster inside.
+// Tile index for workgroup id w: XCD-contiguous bands of tiles, grouped raster inside.
 __device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn) {
     int T = tiles_m * tiles_n;
     int t = w;
@@ -223,38 +282,41 @@ __device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn
 
 // Pair mode (NPAIR = 2): C := alpha*(A1*B1 + A2*B2) + beta0*C0 + beta*C over a K range made of
 // the two contractions back to back (calc_G).  NPAIR = 1: the plain product.
-template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
-__global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, const TS *__restrict__ A1,
-                                             const TS *__restrict__ A2, int64_t lda, const TS *__restrict__ B1,
-                                             const TS *__restrict__ B2, int64_t ldb, double beta0,
-                                             const TS *__restrict__ C0, double beta, TS *__restrict__ C, int64_t ldc,
-                                             int k_per_split, typename Prec<PREC>::acc_t *__restrict__ partial) {
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR, class CF>
+__global__ __launch_bounds__(CF::NT, CF::MINW) void k_gemm(int m, int n, int k, double alpha, const TS *__restrict__ A1,
+                                                 const TS *__restrict__ A2, int64_t lda, const TS *__restrict__ B1,
+                                                 const TS *__restrict__ B2, int64_t ldb, double beta0,
+                                                 const TS *__restrict__ C0, double beta, TS *__restrict__ C,
+                                                 int64_t ldc, int k_per_split, int vec_a, int vec_b,
+                                                 typename Prec<PREC>::acc_t *__restrict__ partial) {
     typedef typename Prec<PREC>::lds_t LT;
     typedef typename Prec<PREC>::acc_t AT;
     typedef typename Prec<PREC>::accv AV;
+    typedef LdsT<PREC, CF::BM> LA;
+    typedef LdsT<PREC, CF::BN> LB;
     constexpr int BK = Prec<PREC>::BK;
-    __shared__ LT smem[2 * (Lds<PREC>::A_ELEMS + Lds<PREC>::B_ELEMS)];
+    __shared__ LT smem[2 * (LA::ELEMS + LB::ELEMS)];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+    const int wm = wave / CF::WGN, wn = wave % CF::WGN;
+    const int tiles_m = (m + CF::BM - 1) / CF::BM, tiles_n = (n + CF::BN - 1) / CF::BN;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int i0 = tm * BM, j0 = tn * BN;
+    const int i0 = tm * CF::BM, j0 = tn * CF::BN;
 
     // K range of this split over the virtual contraction [0, NPAIR*k).
     const int ktot = NPAIR * k;
     const int kb = blockIdx.z * k_per_split;
     const int ke = min(ktot, kb + k_per_split);
 
-    AV acc[FB][FB];
+    AV acc[CF::FBM][CF::FBN];
 #pragma unroll
-    for (int a = 0; a < FB; ++a)
+    for (int a = 0; a < CF::FBM; ++a)
 #pragma unroll
-        for (int b = 0; b < FB; ++b) acc[a][b] = AV{0, 0, 0, 0};
+        for (int b = 0; b < CF::FBN; ++b) acc[a][b] = AV{0, 0, 0, 0};
 
-    Stage<TS, PREC, !TRA, BM> sa;
-    Stage<TS, PREC, TRB, BN> sb;
+    Stage<TS, PREC, !TRA, CF::BM, CF::NT> sa;
+    Stage<TS, PREC, TRB, CF::BN, CF::NT> sb;
 
     auto fetch = [&](int kk) {
         // kk is a virtual K index; pick the operand pair it falls in.
@@ -266,29 +328,29 @@ __global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, 
             kl = kk - k;
         }
         int klim = min(k, kl + (ke - kk));  // stop at the split end and at the pair end
-        sa.load(Ap, lda, i0, kl, m, klim, t);
-        sb.load(Bp, ldb, j0, kl, n, klim, t);
+        sa.load(Ap, lda, i0, kl, m, klim, vec_a != 0, t);
+        sb.load(Bp, ldb, j0, kl, n, klim, vec_b != 0, t);
     };
 
-    LT *As0 = smem, *Bs0 = smem + Lds<PREC>::A_ELEMS;
-    LT *As1 = smem + Lds<PREC>::A_ELEMS + Lds<PREC>::B_ELEMS, *Bs1 = As1 + Lds<PREC>::A_ELEMS;
+    LT *As0 = smem, *Bs0 = smem + LA::ELEMS;
+    LT *As1 = smem + LA::ELEMS + LB::ELEMS, *Bs1 = As1 + LA::ELEMS;
 
     // In pair mode the host guarantees k % BK == 0, so no slice straddles the pair boundary.
-    int nslices = (ke - kb + BK - 1) / BK;
+    const int nslices = (ke - kb + BK - 1) / BK;
     if (nslices > 0) {
         fetch(kb);
-        sa.template store<true>(As0, t);
-        sb.template store<false>(Bs0, t);
+        sa.store(As0, t);
+        sb.store(Bs0, t);
         __syncthreads();
         for (int s = 0; s < nslices; ++s) {
             LT *Ac = (s & 1) ? As1 : As0, *Bc = (s & 1) ? Bs1 : Bs0;
             LT *An = (s & 1) ? As0 : As1, *Bn = (s & 1) ? Bs0 : Bs1;
-            bool more = (s + 1 < nslices);
+            const bool more = (s + 1 < nslices);
             if (more) fetch(kb + (s + 1) * BK);
-            mma_slice<PREC>(Ac, Bc, acc, wm, wn, lane);
+            mma_slice<PREC, CF>(Ac, Bc, acc, wm, wn, lane);
             if (more) {
-                sa.template store<true>(An, t);
-                sb.template store<false>(Bn, t);
+                sa.store(An, t);
+                sb.store(Bn, t);
             }
             __syncthreads();
         }
@@ -297,22 +359,22 @@ __global__ __launch_bounds__(NT) void k_gemm(int m, int n, int k, double alpha, 
     // Epilogue.  Accumulator v of block (fm, fn) holds column lane%16 and row out_row(lane/16, v).
     const int li = lane & 15, lq = lane >> 4;
 #pragma unroll
-    for (int fm = 0; fm < FB; ++fm)
+    for (int fm = 0; fm < CF::FBM; ++fm)
 #pragma unroll
-        for (int fn = 0; fn < FB; ++fn) {
-            int j = j0 + wn * WN + fn * 16 + li;
+        for (int fn = 0; fn < CF::FBN; ++fn) {
+            const int j = j0 + wn * CF::WN + fn * 16 + li;
             if (j >= n) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int i = i0 + wm * WM + fm * 16 + Prec<PREC>::out_row(lq, v);
+                const int i = i0 + wm * CF::WM + fm * 16 + Prec<PREC>::out_row(lq, v);
                 if (i >= m) continue;
-                AT val = acc[fm][fn][v];
+                const AT val = acc[fm][fn][v];
                 if (partial) {
                     partial[(int64_t)blockIdx.z * m * n + (int64_t)i + (int64_t)j * m] = val;
                 } else {
                     double r = alpha * (double)val;
-                    if (C0) r += beta0 * to_d(C0[(int64_t)i + (int64_t)j * ldc]);
-                    if (beta != 0.0) r += beta * to_d(C[(int64_t)i + (int64_t)j * ldc]);
+                    if (C0) r += beta0 * (double)C0[(int64_t)i + (int64_t)j * ldc];
+                    if (beta != 0.0) r += beta * (double)C[(int64_t)i + (int64_t)j * ldc];
                     C[(int64_t)i + (int64_t)j * ldc] = (TS)r;
                 }
             }
@@ -335,13 +397,16 @@ __global__ void k_splitk_reduce(int m, int n, int splits, const AT *__restrict__
     }
 }
 
-template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
-static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
-                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
-                    int64_t ldc) {
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR, class CF>
+static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
+                   const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
+                   int64_t ldc) {
     typedef typename Prec<PREC>::acc_t AT;
     constexpr int BK = Prec<PREC>::BK;
-    const int tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    constexpr int VEC = 16 / sizeof(TS);
+    const int tiles = ((m + CF::BM - 1) / CF::BM) * ((n + CF::BN - 1) / CF::BN);
     const int ktot = NPAIR * k;
     // Split K when the tile count cannot fill the chip (aim for >= 2 workgroups per CU).
     int splits = 1;
@@ -361,12 +426,15 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
         if (rc) return rc;
         partial = (AT *)ctx->ws;
     }
+    const int vec_a = (lda % VEC == 0) && aligned16(A1) && aligned16(A2);
+    const int vec_b = (ldb % VEC == 0) && aligned16(B1) && aligned16(B2);
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_GEMM, &ev);
     prof_work(ctx, OMX_K_GEMM, 2.0 * m * n * (double)ktot,
-              sizeof(TS) * ((double)NPAIR * ((double)m * k + (double)k * n) + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
-    hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR>), dim3(tiles, 1, splits), dim3(NT), 0, ctx->stream, m, n, k,
-                       alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, partial);
+              sizeof(TS) * ((double)NPAIR * ((double)m * k + (double)k * n) +
+                            (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
+    hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR, CF>), dim3(tiles, 1, splits), dim3(CF::NT), 0, ctx->stream,
+                       m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, vec_a, vec_b, partial);
     if (splits > 1) {
         int64_t total = (int64_t)m * n;
         int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -378,26 +446,69 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
     return OMX_OK;
 }
 
+// Tile configuration: a forced one (omx_gemm_set_tile), else the tuned default.  Only the FP64
+// Gram and gradient products (the dominant ones) carry every configuration; other operand
+// layouts and precisions use configuration 0.
+template <int PREC>
+static int pick_cfg(omx_ctx ctx, int m, int n) {
+    if (ctx->gemm_cfg >= 0) return ctx->gemm_cfg;
+    (void)m;
+    (void)n;
+    return 0;
+}
+
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
+                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
+                    int64_t ldc) {
+    constexpr bool tunable = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
+    if constexpr (tunable) {
+        switch (pick_cfg<PREC>(ctx, m, n)) {
+            case 1:
+                return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg1>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                                C0, beta, C, ldc);
+            case 2:
+                return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg2>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                                C0, beta, C, ldc);
+            case 3:
+                return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg3>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                                C0, beta, C, ldc);
+            default:
+                break;
+        }
+    }
+    return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg0>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta,
+                                                    C, ldc);
+}
+
 template <typename TS, int PREC>
 static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
                         int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
     const TS *a = (const TS *)A, *b = (const TS *)B;
     TS *c = (TS *)C;
-    if (!trA && !trB) return run_gemm<TS, PREC, false, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
-    if (!trA && trB) return run_gemm<TS, PREC, false, true, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
-    if (trA && !trB) return run_gemm<TS, PREC, true, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    if (!trA && !trB)
+        return run_gemm<TS, PREC, false, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    if (!trA && trB)
+        return run_gemm<TS, PREC, false, true, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
+    if (trA && !trB)
+        return run_gemm<TS, PREC, true, false, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
     return run_gemm<TS, PREC, true, true, 1>(ctx, m, n, k, alpha, a, a, lda, b, b, ldb, 0.0, nullptr, beta, c, ldc);
 }
 
 int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
                 int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
     if (dt == OMX_F64) {
-        if (prec == OMX_PREC_F64) return dispatch_ops<double, OMX_PREC_F64>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-        if (prec == OMX_PREC_F32) return dispatch_ops<double, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-        if (prec == OMX_PREC_BF16) return dispatch_ops<double, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_F64)
+            return dispatch_ops<double, OMX_PREC_F64>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_F32)
+            return dispatch_ops<double, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_BF16)
+            return dispatch_ops<double, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     } else if (dt == OMX_F32) {
-        if (prec == OMX_PREC_F32) return dispatch_ops<float, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-        if (prec == OMX_PREC_BF16) return dispatch_ops<float, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_F32)
+            return dispatch_ops<float, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        if (prec == OMX_PREC_BF16)
+            return dispatch_ops<float, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     }
     set_error("omx_gemm: unsupported (dt=%d, prec=%d)", dt, prec);
     return OMX_ERR_UNSUPPORTED;
@@ -407,27 +518,32 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc) {
-    int bk = (prec == OMX_PREC_F64) ? Prec<OMX_PREC_F64>::BK
-             : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK
-                                      : Prec<OMX_PREC_BF16>::BK;
+    const int bk = (prec == OMX_PREC_F64)   ? Prec<OMX_PREC_F64>::BK
+                   : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK
+                                            : Prec<OMX_PREC_BF16>::BK;
     // The pair boundary must fall on a K-slice boundary; otherwise the caller runs separate passes.
     if (k % bk != 0) return OMX_ERR_UNSUPPORTED;
     if (dt == OMX_F64) {
         const double *a1 = (const double *)A1, *a2 = (const double *)A2, *b1 = (const double *)B1,
                      *b2 = (const double *)B2, *c0 = (const double *)C0;
         if (prec == OMX_PREC_F64)
-            return run_gemm<double, OMX_PREC_F64, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+            return run_gemm<double, OMX_PREC_F64, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb,
+                                                                   beta0, c0, beta, (double *)C, ldc);
         if (prec == OMX_PREC_F32)
-            return run_gemm<double, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+            return run_gemm<double, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb,
+                                                                   beta0, c0, beta, (double *)C, ldc);
         if (prec == OMX_PREC_BF16)
-            return run_gemm<double, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (double *)C, ldc);
+            return run_gemm<double, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb,
+                                                                    beta0, c0, beta, (double *)C, ldc);
     } else if (dt == OMX_F32) {
         const float *a1 = (const float *)A1, *a2 = (const float *)A2, *b1 = (const float *)B1,
                     *b2 = (const float *)B2, *c0 = (const float *)C0;
         if (prec == OMX_PREC_F32)
-            return run_gemm<float, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (float *)C, ldc);
+            return run_gemm<float, OMX_PREC_F32, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0,
+                                                                  c0, beta, (float *)C, ldc);
         if (prec == OMX_PREC_BF16)
-            return run_gemm<float, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb, beta0, c0, beta, (float *)C, ldc);
+            return run_gemm<float, OMX_PREC_BF16, false, false, 2>(ctx, m, n, k, alpha, a1, a2, lda, b1, b2, ldb,
+                                                                   beta0, c0, beta, (float *)C, ldc);
     }
     set_error("calc_G: unsupported (dt=%d, prec=%d)", dt, prec);
     return OMX_ERR_UNSUPPORTED;
@@ -451,10 +567,18 @@ extern "C" int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m
     return launch_gemm(ctx, prec, dt, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+extern "C" int omx_gemm_set_tile(omx_ctx ctx, int cfg) {
+    OMX_REQUIRE(ctx && cfg >= -1 && cfg < kNumCfg, "omx_gemm_set_tile: configuration %d not in [-1, %d)", cfg,
+                kNumCfg);
+    ctx->gemm_cfg = cfg;
+    return OMX_OK;
+}
+
 extern "C" int omx_omm_calc_G(omx_ctx ctx, int prec, int dt, int n, int m, const void *HW, const void *SW,
                               int64_t ldw, const void *HC, const void *SC, void *G, int64_t ldc) {
     OMX_REQUIRE(ctx && HW && SW && HC && SC && G, "omx_omm_calc_G: null argument");
     OMX_REQUIRE(n > 0 && m > 0 && ldw >= n && ldc >= n, "omx_omm_calc_G: bad shape");
+    OMX_REQUIRE(G != HC && G != SC, "omx_omm_calc_G: G must not alias HC or SC");
     // G = 4*HC - 2*(SW*HC + HW*SC): pair product with the 4*HC term folded into the epilogue.
     int rc = launch_gemm_pair(ctx, prec, dt, n, m, n, -2.0, SW, HW, ldw, HC, SC, ldc, 4.0, HC, 0.0, G, ldc);
     if (rc != OMX_ERR_UNSUPPORTED) return rc;

@@ -61,6 +61,7 @@ struct omx_ctx_s {
     double prof_flops[OMX_K_COUNT] = {0};  // algorithmic flops of the timed launches
     double prof_bytes[OMX_K_COUNT] = {0};  // algorithmic (minimum) HBM bytes of the timed launches
     int num_cus = 256;
+    int gemm_cfg = -1;  // forced GEMM tile configuration (-1: chosen by shape)
 };
 
 struct omx_bsr_s {

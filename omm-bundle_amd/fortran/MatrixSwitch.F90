@@ -59,7 +59,7 @@ module MatrixSwitch
 
   public :: m_allocate, m_deallocate, m_copy, mm_multiply, m_add, m_trace, mm_trace, m_scale, m_set
   public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr
-  public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download
+  public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download, ms_convert
   public :: ms_label_prec, die
 
   type(c_ptr), save :: g_ctx = c_null_ptr
@@ -338,6 +338,20 @@ contains
     call chk(rc, 'm_register_gbsr')
     m_name%is_initialized = .true.
   end subroutine m_register_gbsr
+
+  !> Copy a dense device matrix into another allocated one of the same shape, converting the
+  !> element type if they differ (e.g. FP64 <-> FP32 working sets).
+  subroutine ms_convert(src, dst)
+    type(matrix), intent(in) :: src
+    type(matrix), intent(inout) :: dst
+    integer(c_int) :: rc
+    if (.not. (src%is_device .and. dst%is_device) .or. src%is_sparse .or. dst%is_sparse) &
+      call die('ms_convert: dense device matrices expected')
+    if (src%dim1 /= dst%dim1 .or. src%dim2 /= dst%dim2) call die('ms_convert: shape mismatch')
+    rc = omx_convert(ctx(), int(src%dim1, c_int), int(src%dim2, c_int), src%dptr, src%dtype, src%ld, dst%dptr, &
+                     dst%dtype, dst%ld)
+    call chk(rc, 'ms_convert')
+  end subroutine ms_convert
 
   !> Host <-> device transfers of a dense matrix.
   subroutine ms_upload(m_name, A)

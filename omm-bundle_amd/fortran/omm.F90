@@ -30,7 +30,8 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   integer, intent(in) :: m, n, flavour, np, ip
   real(dp), intent(in) :: eta, cg_tol, scale_T
   real(dp), intent(out) :: e_min
-  type(matrix), intent(inout) :: H, S, D_min, C_min, T
+  type(matrix), intent(inout) :: H, S, D_min, T
+  type(matrix), intent(inout), target :: C_min
 
   logical :: no_S, conv, ls_fail, dev, mixed
   logical, save :: log_start = .false.
@@ -42,8 +43,12 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   real(dp), allocatable, save :: x_min(:)
   real(dp), allocatable :: cinit(:,:)
   character(3) :: prod
-  type(matrix) :: SWd, SWdd, G, G_p, D, HC, HG, work1
-  type(matrix), allocatable, save :: HW(:), HWd(:), HWdd(:), SW(:), SC(:), SG(:), work2(:), QW(:), CD(:)
+  type(matrix), target :: SWd, SWdd, G, G_p, D, HC, HG, work1
+  type(matrix), allocatable, save, target :: HW(:), HWd(:), HWdd(:), SW(:), SC(:), SG(:), work2(:), QW(:), CD(:)
+  ! working set of the low-precision phase of 'hmx' (FP32 storage)
+  type(matrix), target :: lC, lHW, lHWd, lHWdd, lSW, lSC, lSG, lwork2
+  ! the matrices the CG phase works on: the FP64 state of this ip, or the low-precision set
+  type(matrix), pointer :: pC, pHW, pHWd, pHWdd, pSW, pSC, pSG, pwork2
   integer(c_int) :: rc
 
   if (log_start) then
@@ -100,15 +105,7 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   if (.not. SC(ip)%is_initialized) call m_allocate(SC(ip), n, m, m_storage)
   if (.not. SG(ip)%is_initialized) call m_allocate(SG(ip), n, m, m_storage)
   if (.not. work2(ip)%is_initialized) call m_allocate(work2(ip), n, m, m_storage)
-  call m_allocate(SWd, n, n, m_storage)
-  call m_allocate(SWdd, n, n, m_storage)
-  call m_allocate(G, n, m, m_storage)
-  call m_allocate(G_p, n, m, m_storage)
-  call m_allocate(D, n, m, m_storage)
-  call m_allocate(HC, n, m, m_storage)
-  call m_allocate(HG, n, m, m_storage)
-  call m_allocate(work1, n, n, m_storage)
-  dev = opt_fused .and. C_min%is_device .and. G%is_device
+  dev = opt_fused .and. C_min%is_device .and. (m_storage(1:1) == 'g')
 
   ! random initial guess, normalised (omm.F90:329-343)
   if (first_call(ip) .and. (.not. init_C)) then
@@ -149,26 +146,33 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   end if
 
   if (mixed) then
-    ! low-precision phase, then FP64 from the resulting coefficients
-    call initial_energy(opt_lowp_label, first_call(ip) .or. init_C .or. new_S)
+    if (.not. C_min%is_device) call omm_die('omm: mixed precision needs device storage')
+    ! Low-precision phase on an FP32 copy of everything (products in BF16 or FP32), started from
+    ! C_min and recomputed from scratch; then FP64 from the resulting coefficients, again with
+    ! every quantity recomputed exactly (the incremental updates of the low-precision phase carry
+    ! its rounding).
+    call m_allocate(lC, n, m, 'gsden')
+    call ms_convert(C_min, lC)
+    call point_lowp()
+    call alloc_locals('gsden')
+    call initial_energy(opt_lowp_label, .true.)
     call cg(opt_lowp_label, opt_lowp_tol, opt_lowp_max_iter)
     icg_lowp = icg
+    call ms_convert(lC, C_min)
+    call free_locals()
+    call free_lowp()
+    call point_state()
+    call alloc_locals(m_storage)
     call initial_energy('hip', .true.)
     call cg('hip', cg_tol_internal, opt_max_iter)
   else
+    call point_state()
+    call alloc_locals(m_storage)
     call initial_energy(prod, first_call(ip) .or. init_C .or. new_S)
     call cg(prod, cg_tol_internal, opt_max_iter)
   end if
   if ((mpi_rank0()) .and. long_out) write(log_unit, '(a)') '+---------------------------------------------+'
-
-  call m_deallocate(work1)
-  call m_deallocate(HG)
-  call m_deallocate(HC)
-  call m_deallocate(D)
-  call m_deallocate(G_p)
-  call m_deallocate(G)
-  call m_deallocate(SWdd)
-  call m_deallocate(SWd)
+  call free_locals()
 
   if (.not. QW(ip)%is_initialized) call m_allocate(QW(ip), n, n, m_storage)
   if (.not. CD(ip)%is_initialized) call m_allocate(CD(ip), n, m, m_storage)
@@ -251,31 +255,31 @@ contains
   subroutine initial_energy(lab, recompute_S)
     character(3), intent(in) :: lab
     logical, intent(in) :: recompute_S
-    call calc_AW(H, C_min, HW(ip), HC, lab)
+    call calc_AW(H, pC, pHW, HC, lab)
     if (no_S) then
-      if (recompute_S) call mm_multiply(C_min, 'n', C_min, 'c', SW(ip), 1.0_dp, 0.0_dp, lab)
+      if (recompute_S) call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, lab)
     else
       if (recompute_S) then
-        call calc_AW(S, C_min, SW(ip), SC(ip), lab)
+        call calc_AW(S, pC, pSW, pSC, lab)
       else
-        call m_add(SG(ip), 'n', SC(ip), x_min(ip), 1.0_dp, scale_label())
+        call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
       end if
     end if
     if (no_S) then
-      call calc_G(HW(ip), SW(ip), G, HC, C_min, lab)
-      call mm_multiply(C_min, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+      call calc_G(pHW, pSW, G, HC, pC, lab)
+      call mm_multiply(pC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     else
-      call calc_G(HW(ip), SW(ip), G, HC, SC(ip), lab)
-      call mm_multiply(SC(ip), 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+      call calc_G(pHW, pSW, G, HC, pSC, lab)
+      call mm_multiply(pSC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     end if
-    call mm_multiply(HC, 'n', G, 'c', HWd(ip), 1.0_dp, 0.0_dp, lab)
-    call calc_AW(H, G, HWdd(ip), HG, lab)
+    call mm_multiply(HC, 'n', G, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
+    call calc_AW(H, G, pHWdd, HG, lab)
     if (no_S) then
       call mm_multiply(G, 'n', G, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
     else
-      call calc_AW(S, G, SWdd, SG(ip), lab)
+      call calc_AW(S, G, SWdd, pSG, lab)
     end if
-    call calc_coeff(HW(ip), SW(ip), HWd(ip), SWd, HWdd(ip), SWdd, work1, coeff, lab)
+    call calc_coeff(pHW, pSW, pHWd, SWd, pHWdd, SWdd, work1, coeff, lab)
     e_min = coeff(0)
   end subroutine initial_energy
 
@@ -305,36 +309,36 @@ contains
         e_min_old = e_min
         ! after the first step of a phase, line-search data along the new direction (omm.F90:448-464)
         if (steps > 0) then
-          call mm_multiply(HC, 'n', D, 'c', HWd(ip), 1.0_dp, 0.0_dp, lab)
+          call mm_multiply(HC, 'n', D, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
           if (no_S) then
-            call mm_multiply(C_min, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+            call mm_multiply(pC, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
           else
-            call mm_multiply(SC(ip), 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+            call mm_multiply(pSC, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
           end if
-          call calc_AW(H, D, HWdd(ip), HG, lab)
+          call calc_AW(H, D, pHWdd, HG, lab)
           if (no_S) then
             call mm_multiply(D, 'n', D, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
           else
-            call calc_AW(S, D, SWdd, SG(ip), lab)
+            call calc_AW(S, D, SWdd, pSG, lab)
           end if
-          call calc_coeff(HW(ip), SW(ip), HWd(ip), SWd, HWdd(ip), SWdd, work1, coeff, lab)
+          call calc_coeff(pHW, pSW, pHWd, SWd, pHWdd, SWdd, work1, coeff, lab)
         end if
         call omm_solve_quartic(coeff(0:4), x_min(ip), ls_fail)
         if (ls_fail) then
           if (mpi_rank0()) write(log_unit, '(a)') '| WARNING: Rescaling coefficients!            |'
           e_min = 3.0_dp*e_min
-          call m_scale(C_min, 0.5_dp, scale_label())
+          call m_scale(pC, 0.5_dp, scale_label())
           ls_conv = .false.
           nfail = nfail + 1
         else
           e_min = coeff(4)*x_min(ip)**4 + coeff(3)*x_min(ip)**3 + coeff(2)*x_min(ip)**2 + coeff(1)*x_min(ip) + coeff(0)
-          call m_add(D, 'n', C_min, x_min(ip), 1.0_dp, scale_label())
+          call m_add(D, 'n', pC, x_min(ip), 1.0_dp, scale_label())
           ls_conv = .true.
         end if
         if (ls_fail) then
-          call m_scale(SW(ip), 0.25_dp, scale_label())
+          call m_scale(pSW, 0.25_dp, scale_label())
         else
-          call update_W(SW(ip), SWd, SWdd, x_min(ip))
+          call update_W(pSW, SWd, SWdd, x_min(ip))
         end if
         e_diff = 2.0_dp*abs((e_min - e_min_old)/(e_min + e_min_old))
         if (mpi_rank0() .and. long_out) write(log_unit, '(a,2(1x,i5),2(1x,es15.7e3),1x,a)') '|', outer, inner, &
@@ -355,22 +359,22 @@ contains
           exit
         end if
         if (ls_fail) then
-          call m_scale(HW(ip), 0.25_dp, scale_label())
+          call m_scale(pHW, 0.25_dp, scale_label())
         else
-          call update_W(HW(ip), HWd(ip), HWdd(ip), x_min(ip))
+          call update_W(pHW, pHWd, pHWdd, x_min(ip))
         end if
         if (ls_fail) then
           call m_scale(HC, 0.5_dp, scale_label())
-          if (.not. no_S) call m_scale(SC(ip), 0.5_dp, scale_label())
+          if (.not. no_S) call m_scale(pSC, 0.5_dp, scale_label())
           call m_add(G_p, 'n', G, 1.0_dp, 0.0_dp, scale_label())
           call m_add(HC, 'n', G, 1.5_dp, 1.0_dp, scale_label())
         else
           call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
           if (no_S) then
-            call calc_G(HW(ip), SW(ip), G, HC, C_min, lab)
+            call calc_G(pHW, pSW, G, HC, pC, lab)
           else
-            call m_add(SG(ip), 'n', SC(ip), x_min(ip), 1.0_dp, scale_label())
-            call calc_G(HW(ip), SW(ip), G, HC, SC(ip), lab)
+            call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
+            call calc_G(pHW, pSW, G, HC, pSC, lab)
           end if
         end if
         if (ls_conv) then
@@ -380,9 +384,9 @@ contains
             lambda_n = dots(1)
             lambda_d = dots(2)
           else
-            call m_add(G, 'n', work2(ip), 1.0_dp, 0.0_dp, scale_label())
-            call m_add(G_p, 'n', work2(ip), -1.0_dp, 1.0_dp, scale_label())
-            call mm_trace(G, work2(ip), lambda_n, scale_label())
+            call m_add(G, 'n', pwork2, 1.0_dp, 0.0_dp, scale_label())
+            call m_add(G_p, 'n', pwork2, -1.0_dp, 1.0_dp, scale_label())
+            call mm_trace(G, pwork2, lambda_n, scale_label())
             call mm_trace(G_p, G_p, lambda_d, scale_label())
           end if
           lambda = lambda_n/lambda_d
@@ -396,6 +400,72 @@ contains
       if (mpi_rank0()) write(log_unit, '(a)') '| WARNING: OMM failed to converge!            |'
     end if
   end subroutine cg
+
+  !> Point the CG working set at the FP64 state of this ip and at C_min.
+  subroutine point_state()
+    pC => C_min
+    pHW => HW(ip)
+    pHWd => HWd(ip)
+    pHWdd => HWdd(ip)
+    pSW => SW(ip)
+    pSC => SC(ip)
+    pSG => SG(ip)
+    pwork2 => work2(ip)
+  end subroutine point_state
+
+  !> Point the CG working set at a fresh FP32 set (the low-precision phase of 'hmx').
+  subroutine point_lowp()
+    call m_allocate(lHW, n, n, 'gsden')
+    call m_allocate(lHWd, n, n, 'gsden')
+    call m_allocate(lHWdd, n, n, 'gsden')
+    call m_allocate(lSW, n, n, 'gsden')
+    call m_allocate(lSC, n, m, 'gsden')
+    call m_allocate(lSG, n, m, 'gsden')
+    call m_allocate(lwork2, n, m, 'gsden')
+    pC => lC
+    pHW => lHW
+    pHWd => lHWd
+    pHWdd => lHWdd
+    pSW => lSW
+    pSC => lSC
+    pSG => lSG
+    pwork2 => lwork2
+  end subroutine point_lowp
+
+  subroutine free_lowp()
+    call m_deallocate(lHW)
+    call m_deallocate(lHWd)
+    call m_deallocate(lHWdd)
+    call m_deallocate(lSW)
+    call m_deallocate(lSC)
+    call m_deallocate(lSG)
+    call m_deallocate(lwork2)
+    call m_deallocate(lC)
+  end subroutine free_lowp
+
+  !> Per-call work matrices (omm.F90:292-303), zero-filled as m_allocate does.
+  subroutine alloc_locals(label)
+    character(5), intent(in) :: label
+    call m_allocate(SWd, n, n, label)
+    call m_allocate(SWdd, n, n, label)
+    call m_allocate(G, n, m, label)
+    call m_allocate(G_p, n, m, label)
+    call m_allocate(D, n, m, label)
+    call m_allocate(HC, n, m, label)
+    call m_allocate(HG, n, m, label)
+    call m_allocate(work1, n, n, label)
+  end subroutine alloc_locals
+
+  subroutine free_locals()
+    if (work1%is_initialized) call m_deallocate(work1)
+    if (HG%is_initialized) call m_deallocate(HG)
+    if (HC%is_initialized) call m_deallocate(HC)
+    if (D%is_initialized) call m_deallocate(D)
+    if (G_p%is_initialized) call m_deallocate(G_p)
+    if (G%is_initialized) call m_deallocate(G)
+    if (SWdd%is_initialized) call m_deallocate(SWdd)
+    if (SWd%is_initialized) call m_deallocate(SWd)
+  end subroutine free_locals
 
   subroutine free_all()
     integer :: q

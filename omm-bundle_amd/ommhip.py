@@ -48,6 +48,7 @@ SIGNATURES = {
     "omx_bsr_update_values": (_i, [_vp, _vp]),
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
+    "omx_gemm_set_tile": (_i, [_vp, _i]),
     "omx_add": (_i, [_vp, _i, _c, _i, _i, _d, _vp, _i64, _d, _vp, _i64]),
     "omx_scale": (_i, [_vp, _i, _i, _i, _d, _vp, _i64]),
     "omx_set": (_i, [_vp, _i, _c, _i, _i, _d, _d, _vp, _i64]),
