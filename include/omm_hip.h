@@ -116,6 +116,9 @@ int omx_bsr_destroy(omx_bsr a);
 int omx_bsr_info(omx_bsr a, int *nb, int64_t *nnzb, int *is_complex);
 /* Replace the block values (same pattern), e.g. after a parameter change. */
 int omx_bsr_update_values(omx_bsr a, const double *bval);
+/* Copy the block values (16 per stored block, column-major) / the block pattern to the host. */
+int omx_bsr_get_values(omx_bsr a, double *bval);
+int omx_bsr_get_pattern(omx_bsr a, int *bcolptr, int *browidx);
 
 /* ==== MatrixSwitch operations on device matrices =========================================== */
 
@@ -129,6 +132,15 @@ int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, i
              void *Y, int64_t ldy);
 
 /*
+ * Cp := alpha*(A^T B) on the stored blocks of Cp + beta*Cp, A and B no x nb dense (dt OMX_F64 or
+ * OMX_F32), Cp a BSR-4 pattern (its values are overwritten): the dense x dense -> sparse product
+ * mm_multiply(A,'c',B,'n',Cp) (MatrixSwitch/src/MatrixSwitch_mm_multiply.F90:334-398).  With
+ * A = C and B = (2I - SW) C it is the density matrix on the pattern of H, which is also dE/dH.
+ */
+int omx_sddmm_bsr(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *A, int64_t lda, const void *B,
+                  int64_t ldb, double beta, omx_bsr Cp);
+
+/*
  * C := alpha*op(A)*op(B) + beta*C  (mm_multiply on dense operands,
  * MatrixSwitch/src/MatrixSwitch.F90:1046-1390; 'ref' semantics of
  * MatrixSwitch_mm_multiply.F90:19-79).  op = 'n', 't' or 'c' ('c' = 't' for real data).
@@ -137,6 +149,10 @@ int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, i
  */
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
              int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+/* SpMM launch shape of later products on this context: blocks whose loads are issued together
+ * (1, 4 or 8; 0 = default 8) and block columns per workgroup (0 = chosen by size).  Tuning hook;
+ * results do not depend on it. */
+int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper);
 /* Force the GEMM tile configuration of later products on this context (-1 restores the choice
  * by shape) for the FP64 Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
  * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256.  Used by the tile autotuner; results do

@@ -420,6 +420,26 @@ int omx_bsr_update_values(omx_bsr a, const double *bval) {
     return OMX_OK;
 }
 
+int omx_bsr_get_values(omx_bsr a, double *bval) {
+    OMX_REQUIRE(a && (bval || a->nnzb == 0), "omx_bsr_get_values: bad arguments");
+    size_t vbytes = sizeof(double) * 16 * (size_t)a->nnzb * (a->is_complex ? 2 : 1);
+    if (!vbytes) return OMX_OK;
+    OMX_CHECK_HIP(hipMemcpyAsync(bval, a->d_val, vbytes, hipMemcpyDeviceToHost, a->ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+    return OMX_OK;
+}
+
+int omx_bsr_get_pattern(omx_bsr a, int *bcolptr, int *browidx) {
+    OMX_REQUIRE(a && bcolptr && (browidx || a->nnzb == 0), "omx_bsr_get_pattern: bad arguments");
+    OMX_CHECK_HIP(hipMemcpyAsync(bcolptr, a->d_colptr, sizeof(int) * (a->nbc + 1), hipMemcpyDeviceToHost,
+                                 a->ctx->stream));
+    if (a->nnzb)
+        OMX_CHECK_HIP(hipMemcpyAsync(browidx, a->d_rowidx, sizeof(int) * a->nnzb, hipMemcpyDeviceToHost,
+                                     a->ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+    return OMX_OK;
+}
+
 // ---- element access --------------------------------------------------------------------------
 
 int omx_set_element(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha, double beta) {

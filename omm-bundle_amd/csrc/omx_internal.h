@@ -61,7 +61,9 @@ struct omx_ctx_s {
     double prof_flops[OMX_K_COUNT] = {0};  // algorithmic flops of the timed launches
     double prof_bytes[OMX_K_COUNT] = {0};  // algorithmic (minimum) HBM bytes of the timed launches
     int num_cus = 256;
-    int gemm_cfg = -1;  // forced GEMM tile configuration (-1: chosen by shape)
+    int gemm_cfg = -1;    // forced GEMM tile configuration (-1: chosen by shape)
+    int spmm_batch = 0;   // SpMM blocks per load batch (0: default 8)
+    int spmm_jper = 0;    // SpMM block columns per workgroup (0: chosen by size)
 };
 
 struct omx_bsr_s {

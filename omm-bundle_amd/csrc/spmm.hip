@@ -9,7 +9,10 @@
 //
 // Mapping: a workgroup owns a chunk of ROWS consecutive states (VEC per thread, loaded as one
 // 16-byte vector per basis column) and a run of JPER block columns.  Each thread keeps 4*VEC
-// accumulators; the 16 block values are wave-uniform and come through the scalar cache.  The
+// accumulators and issues the loads of a whole batch of blocks before using them (memory-level
+// parallelism); the 16 block values are wave-uniform and come through the scalar cache.  A zero
+// entry stored inside a block contributes an exact zero, so the per-element sum equals the
+// reference's CSC sum over its stored entries.  The
 // workgroup id is remapped so that each XCD owns a contiguous band of state chunks and walks
 // the block columns in order: the neighbouring basis columns a block column gathers were read
 // a few block columns earlier by the same XCD and are still in its L2 (lattice neighbours in
@@ -59,7 +62,7 @@ __device__ inline void vstore(T *p, const T (&in)[VEC]) {
 
 constexpr int SPMM_NT = 256;
 
-template <typename T, int VEC>
+template <typename T, int VEC, int NB>
 __global__ __launch_bounds__(SPMM_NT) void k_spmm_bsr4(int no, int nb, int nbc, double alpha,
                                                        const T *__restrict__ X, int64_t ldx,
                                                        const int *__restrict__ colptr, const int *__restrict__ rowidx,
@@ -91,29 +94,42 @@ __global__ __launch_bounds__(SPMM_NT) void k_spmm_bsr4(int no, int nb, int nbc, 
 #pragma unroll
             for (int q = 0; q < VEC; ++q) acc[c][q] = (T)0;
         const int lb = colptr[J], le = colptr[J + 1];
-        for (int l = lb; l < le; ++l) {
-            const int I = rowidx[l];
-            const double *v = val + 16 * (int64_t)l;
-            double bv[16];
+        // Blocks are taken in batches of up to NB: all their basis columns are requested before
+        // any is used, so each thread keeps 4*NB vector loads in flight; the accumulation then
+        // runs in ascending (block, r) order.
+        for (int l0 = lb; l0 < le; l0 += NB) {
+            const int nblk = min(NB, le - l0);
+            T x[NB][4][VEC];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) bv[e] = v[e];
+            for (int b = 0; b < NB; ++b) {
+                if (b < nblk) {
+                    const int I = rowidx[l0 + b];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int kcol = 4 * I + r;
-                if (kcol >= nb) break;  // partial last block (nb % 4 != 0)
-                T x[VEC];
-                const T *src = X + (int64_t)kcol * ldx + i0;
-                if (full) {
-                    vload<T, VEC>(src, x);
-                } else {
+                    for (int r = 0; r < 4; ++r) {
+                        const int kcol = 4 * I + r;
+                        const T *src = X + (int64_t)kcol * ldx + i0;
+                        if (kcol < nb && full) {
+                            vload<T, VEC>(src, x[b][r]);
+                        } else {
 #pragma unroll
-                    for (int q = 0; q < VEC; ++q) x[q] = (any && i0 + q < no) ? src[q] : (T)0;
+                            for (int q = 0; q < VEC; ++q) x[b][r][q] = (kcol < nb && any && i0 + q < no) ? src[q] : (T)0;
+                        }
+                    }
                 }
+            }
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const T a = (T)bv[r + 4 * c];
+            for (int b = 0; b < NB; ++b) {
+                if (b < nblk) {
+                    const double *v = val + 16 * (int64_t)(l0 + b);
 #pragma unroll
-                    for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[q], a, acc[c][q]);
+                    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const T a = (T)v[r + 4 * c];
+#pragma unroll
+                            for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[b][r][q], a, acc[c][q]);
+                        }
+                    }
                 }
             }
         }
@@ -154,18 +170,31 @@ static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64
     constexpr int ROWS = SPMM_NT * VEC;
     int rchunks = (no + ROWS - 1) / ROWS;
     // Enough workgroups to cover the chip several times; each walks jper block columns.
-    int64_t target = 16LL * ctx->num_cus;
     int jper = 1;
-    while ((int64_t)rchunks * ((A->nbc + jper * 2 - 1) / (jper * 2)) >= target && jper < 64) jper *= 2;
+    if (ctx->spmm_jper > 0) {
+        jper = ctx->spmm_jper;
+    } else {
+        int64_t target = 16LL * ctx->num_cus;
+        while ((int64_t)rchunks * ((A->nbc + jper * 2 - 1) / (jper * 2)) >= target && jper < 64) jper *= 2;
+    }
     int njg = (A->nbc + jper - 1) / jper;
     int64_t total = (int64_t)rchunks * njg;
     OMX_REQUIRE(total < (1LL << 31), "omx_spmm: grid too large");
+    const int batch = ctx->spmm_batch > 0 ? ctx->spmm_batch : 8;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_SPMM, &ev);
     prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * A->nnzb * no,
               sizeof(T) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (16.0 * 8 + 4) + 4.0 * (A->nbc + 1));
-    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC>), dim3((unsigned)total), dim3(SPMM_NT), 0, ctx->stream, no, nb, A->nbc,
-                       alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, rchunks, jper);
+#define OMX_SPMM_LAUNCH(NBV)                                                                                         \
+    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NBV>), dim3((unsigned)total), dim3(SPMM_NT), 0, ctx->stream, no, nb,     \
+                       A->nbc, alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, rchunks, jper)
+    if (batch >= 8)
+        OMX_SPMM_LAUNCH(8);
+    else if (batch >= 4)
+        OMX_SPMM_LAUNCH(4);
+    else
+        OMX_SPMM_LAUNCH(1);
+#undef OMX_SPMM_LAUNCH
     prof_end(ctx, OMX_K_SPMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
@@ -176,6 +205,13 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 }  // namespace omx
 
 using namespace omx;
+
+extern "C" int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper) {
+    OMX_REQUIRE(ctx && batch >= 0 && jper >= 0 && jper <= 4096, "omx_spmm_set_tile: bad arguments");
+    ctx->spmm_batch = batch;
+    ctx->spmm_jper = jper;
+    return OMX_OK;
+}
 
 extern "C" int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, int64_t ldx, omx_bsr A,
                         double beta, void *Y, int64_t ldy) {
@@ -197,5 +233,136 @@ extern "C" int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const
         return run_spmm<float, 1>(ctx, no, nb, alpha, (const float *)X, ldx, A, beta, (float *)Y, ldy);
     }
     set_error("omx_spmm: unsupported element type %d", dt);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+// =============================================================================================
+// Dense x dense -> block-sparse pattern (SDDMM): Cp := alpha * (A^T B) restricted to the stored
+// blocks of Cp + beta * Cp, with A, B no x nb column-major.  Block (I,J), entry (r,c) is the dot
+// product over the state index of basis columns 4I+r of A and 4J+c of B.  This is the
+// reference's dense*dense -> CSC product (MatrixSwitch/src/MatrixSwitch_mm_multiply.F90:334-398,
+// op = 'c','n'), used for the density matrix D = C^T (QW C) on the pattern of H
+// (libOMM/src/omm_ops.F90:104-124 with a sparse D_min) and for dE/dH.
+//
+// Mapping: a workgroup owns a chunk of SD_ROWS states and one block column J; it holds B's four
+// columns 4J..4J+3 in registers and, per stored block, reads A's four columns, forms the 16
+// products per thread and reduces them over the workgroup.  Per-chunk partial blocks go to the
+// workspace and a second kernel adds them in chunk order (deterministic).
+// =============================================================================================
+namespace omx {
+
+constexpr int SD_NT = 256;
+
+template <typename T, int VEC>
+__global__ __launch_bounds__(SD_NT) void k_sddmm_bsr4_partial(int no, int nb, const T *__restrict__ A, int64_t lda,
+                                                              const T *__restrict__ B, int64_t ldb,
+                                                              const int *__restrict__ colptr,
+                                                              const int *__restrict__ rowidx, int rchunks,
+                                                              int64_t nnzb, double *__restrict__ partial) {
+    constexpr int ROWS = SD_NT * VEC;
+    const int rc = blockIdx.x % rchunks, J = blockIdx.x / rchunks;
+    const int i0 = rc * ROWS + threadIdx.x * VEC;
+    const bool full = (i0 + VEC <= no);
+    __shared__ double red[16][SD_NT / 64];
+    double bcol[4][VEC];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int j = 4 * J + c;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) bcol[c][q] = (j < nb && i0 + q < no) ? (double)B[(int64_t)j * ldb + i0 + q] : 0.0;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int l = colptr[J]; l < colptr[J + 1]; ++l) {
+        const int I = rowidx[l];
+        double acol[4][VEC];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = 4 * I + r;
+            const T *src = A + (int64_t)k * lda + i0;
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) acol[r][q] = (k < nb && (full || i0 + q < no)) ? (double)src[q] : 0.0;
+        }
+        double s[16];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double t = 0.0;
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) t = fma(acol[r][q], bcol[c][q], t);
+                s[r + 4 * c] = t;
+            }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            double t = s[e];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            if (lane == 0) red[e][wave] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < SD_NT / 64; ++w) t += red[threadIdx.x][w];
+            partial[((int64_t)rc * nnzb + l) * 16 + threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_sddmm_bsr4_final(int64_t nnzb, int rchunks, const double *__restrict__ partial, double alpha,
+                                   double beta, double *__restrict__ vals) {
+    const int64_t total = nnzb * 16;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int rc = 0; rc < rchunks; ++rc) s += partial[(int64_t)rc * total + q];
+        vals[q] = (beta == 0.0) ? alpha * s : alpha * s + beta * vals[q];
+    }
+}
+
+template <typename T, int VEC>
+static int run_sddmm(omx_ctx ctx, int no, int nb, double alpha, const T *A, int64_t lda, const T *B, int64_t ldb,
+                     double beta, omx_bsr C) {
+    constexpr int ROWS = SD_NT * VEC;
+    const int rchunks = (no + ROWS - 1) / ROWS;
+    const int64_t grid = (int64_t)rchunks * C->nbc;
+    OMX_REQUIRE(grid < (1LL << 31), "omx_sddmm_bsr: grid too large");
+    const size_t bytes = sizeof(double) * 16 * (size_t)C->nnzb * rchunks;
+    int rc = ensure_ws(ctx, bytes);
+    if (rc) return rc;
+    double *partial = (double *)ctx->ws;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_SPMM, &ev);
+    prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * C->nnzb * no, sizeof(T) * 2.0 * (double)no * nb + C->nnzb * (16.0 * 8 + 4));
+    hipLaunchKernelGGL((k_sddmm_bsr4_partial<T, VEC>), dim3((unsigned)grid), dim3(SD_NT), 0, ctx->stream, no, nb, A,
+                       lda, B, ldb, C->d_colptr, C->d_rowidx, rchunks, C->nnzb, partial);
+    const int64_t total = C->nnzb * 16;
+    const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_sddmm_bsr4_final, dim3(g2), dim3(256), 0, ctx->stream, C->nnzb, rchunks, partial, alpha, beta,
+                       C->d_val);
+    prof_end(ctx, OMX_K_SPMM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+}  // namespace omx
+
+extern "C" int omx_sddmm_bsr(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *A, int64_t lda,
+                             const void *B, int64_t ldb, double beta, omx_bsr C) {
+    OMX_REQUIRE(ctx && C, "omx_sddmm_bsr: null argument");
+    OMX_REQUIRE(nb == C->nb && !C->is_complex, "omx_sddmm_bsr: pattern does not match nb");
+    OMX_REQUIRE(no >= 0 && lda >= std::max(1, no) && ldb >= std::max(1, no), "omx_sddmm_bsr: bad leading dimension");
+    if (C->nnzb == 0) return OMX_OK;
+    if (no == 0) {  // empty contraction: Cp := beta * Cp
+        const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((C->nnzb * 16 + 255) / 256, 8192));
+        hipLaunchKernelGGL(k_sddmm_bsr4_final, dim3(g2), dim3(256), 0, ctx->stream, C->nnzb, 0, (const double *)nullptr,
+                           alpha, beta, C->d_val);
+        OMX_CHECK_HIP(hipGetLastError());
+        return OMX_OK;
+    }
+    OMX_REQUIRE(A && B, "omx_sddmm_bsr: null operand");
+    if (dt == OMX_F64) return run_sddmm<double, 2>(ctx, no, nb, alpha, (const double *)A, lda, (const double *)B, ldb, beta, C);
+    if (dt == OMX_F32) return run_sddmm<float, 2>(ctx, no, nb, alpha, (const float *)A, lda, (const float *)B, ldb, beta, C);
+    set_error("omx_sddmm_bsr: unsupported element type %d", dt);
     return OMX_ERR_UNSUPPORTED;
 }

@@ -266,10 +266,10 @@ contains
       end if
     end if
     if (no_S) then
-      call calc_G(pHW, pSW, G, HC, pC, lab)
+      call calc_G(pHW, pSW, G, HC, pC, grad_label(lab))
       call mm_multiply(pC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     else
-      call calc_G(pHW, pSW, G, HC, pSC, lab)
+      call calc_G(pHW, pSW, G, HC, pSC, grad_label(lab))
       call mm_multiply(pSC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     end if
     call mm_multiply(HC, 'n', G, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
@@ -371,10 +371,10 @@ contains
         else
           call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
           if (no_S) then
-            call calc_G(pHW, pSW, G, HC, pC, lab)
+            call calc_G(pHW, pSW, G, HC, pC, grad_label(lab))
           else
             call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
-            call calc_G(pHW, pSW, G, HC, pSC, lab)
+            call calc_G(pHW, pSW, G, HC, pSC, grad_label(lab))
           end if
         end if
         if (ls_conv) then

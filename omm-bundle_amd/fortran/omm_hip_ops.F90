@@ -23,6 +23,7 @@ module omm_hip_ops
   integer, save :: opt_lowp_max_iter = 0     ! mixed precision: cap on low-precision steps (0 = none)
   character(3), save :: opt_lowp_label = 'hbf'
   logical, save :: opt_fused = .true.        ! use fused device kernels when all operands are on device
+  logical, save :: opt_grad_fp32 = .false.   ! BF16 phase: gradient products in FP32 (Grams stay BF16)
 
   ! ---- statistics of the last omm() call ----
   integer, save :: stat_icg = 0, stat_conv = 0, stat_nfail = 0, stat_icg_lowp = 0
@@ -130,6 +131,14 @@ contains
     coeff(4) = -TrHddSdd
   end subroutine calc_coeff
 
+  !> Product label of the gradient (calc_G) for a phase whose other products use `lab`.
+  function grad_label(lab) result(g)
+    character(3), intent(in) :: lab
+    character(3) :: g
+    g = lab
+    if (lab == 'hbf' .and. opt_grad_fp32) g = 'h32'
+  end function grad_label
+
   !> Reductions always accumulate in FP64; low-precision product labels map to 'hip' for them.
   function trace_label(lab) result(t)
     character(3), intent(in) :: lab
@@ -234,6 +243,8 @@ contains
       end if
     case ('fused')
       opt_fused = (value /= 0.0_dp)
+    case ('lowp_grad_fp32')
+      opt_grad_fp32 = (value /= 0.0_dp)
     case default
       call omm_die('omm_set_option: unknown option '//name)
     end select

@@ -46,9 +46,13 @@ SIGNATURES = {
     "omx_bsr_destroy": (_i, [_vp]),
     "omx_bsr_info": (_i, [_vp, _pi, C.POINTER(_i64), _pi]),
     "omx_bsr_update_values": (_i, [_vp, _vp]),
+    "omx_bsr_get_values": (_i, [_vp, _vp]),
+    "omx_bsr_get_pattern": (_i, [_vp, _vp, _vp]),
+    "omx_sddmm_bsr": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp]),
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     "omx_gemm_set_tile": (_i, [_vp, _i]),
+    "omx_spmm_set_tile": (_i, [_vp, _i, _i]),
     "omx_add": (_i, [_vp, _i, _c, _i, _i, _d, _vp, _i64, _d, _vp, _i64]),
     "omx_scale": (_i, [_vp, _i, _i, _i, _d, _vp, _i64]),
     "omx_set": (_i, [_vp, _i, _c, _i, _i, _d, _d, _vp, _i64]),
@@ -302,6 +306,11 @@ class Bsr:
         self.nnzb = nnzb.value
         return self
 
+    def values(self):
+        out = np.zeros((self.nnzb, 16))
+        check(lib().omx_bsr_get_values(self.h, _ptr(out)), "bsr_get_values")
+        return out
+
     def free(self):
         if self.h:
             lib().omx_bsr_destroy(self.h)
@@ -321,6 +330,11 @@ class Bsr:
 
 def spmm(ctx, X: DMat, A: Bsr, Y: DMat, alpha=1.0, beta=0.0):
     check(lib().omx_spmm(ctx.h, X.dt, X.rows, X.cols, alpha, X.p, X.ld, A.h, beta, Y.p, Y.ld), "omx_spmm")
+
+
+def sddmm(ctx, A: DMat, B: DMat, Cp: Bsr, alpha=1.0, beta=0.0):
+    """Cp := alpha*(A^T B) on Cp's blocks + beta*Cp."""
+    check(lib().omx_sddmm_bsr(ctx.h, A.dt, A.rows, A.cols, alpha, A.p, A.ld, B.p, B.ld, beta, Cp.h), "omx_sddmm_bsr")
 
 
 def gemm(ctx, opA, opB, A: DMat, B: DMat, Cm: DMat, alpha=1.0, beta=0.0, prec=PREC_F64):
