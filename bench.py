@@ -162,6 +162,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true", help="skip the BF16 / FP32 step-rate runs")
     ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
+    ap.add_argument("--ttc-log", default=None, help="write the per-step energies of the --ttc solves (JSON) here")
     args = ap.parse_args()
 
     if args.compute_eta:
@@ -251,7 +252,7 @@ def main():
             line[f"{key}_products_gemm_TFLOPs"] = round(g[2] / (g[0] * 1e-3) / 1e12, 2)
 
     if args.ttc:
-        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0)
+        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log)
 
     if not args.no_cpu:
         t = time.time()
@@ -263,10 +264,11 @@ def main():
     return 0
 
 
-def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10):
+def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None):
     H, S, Cm = mats
     out = {"cg_tol": tol}
     Dref = F.Mat(nb, nb)
+    F.set_option("progress", progress)
     Cm.upload(C0)
     F.sync()
     t = time.perf_counter()
@@ -277,7 +279,6 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10):
     log(f"bench: ttc FP64 {t64:.2f}s icg={r['icg']} e={r['e_min']:.12f}")
     Dmx = F.Mat(nb, nb)
     Cm.upload(C0)
-    F.set_option("lowp_tol", 1e-6)
     F.sync()
     t = time.perf_counter()
     rm = F.omm(nb, no, H, S, Cm, D_min=Dmx, init_C=True, eta=eta, cg_tol=tol, operation="hmx", dealloc=True)
@@ -289,13 +290,21 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10):
     oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
     fro = ctypes.c_double(0)
     oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
-    out["mixed"] = {"seconds": tmx, "icg": rm["icg"], "icg_bf16": rm["icg_lowp"], "e_min": rm["e_min"],
+    out["mixed"] = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
+                    "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
                     "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
                     "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
                     "speedup_vs_fp64": t64 / tmx}
-    log(f"bench: ttc mixed {tmx:.2f}s icg={rm['icg']} (bf16 {rm['icg_lowp']}) e={rm['e_min']:.12f}")
+    F.set_option("progress", 0)
+    log(f"bench: ttc mixed {tmx:.2f}s icg={rm['icg']} (low precision {rm['icg_lowp']}) e={rm['e_min']:.12f} "
+        f"speedup {t64 / tmx:.2f}x")
     Dref.free()
     Dmx.free()
+    if log_path:
+        with open(log_path, "w") as f:
+            json.dump({"fp64": {"e": r["e_log"].tolist(), "t": (r["t_log"] - r["t0"]).tolist()},
+                       "mixed": {"e": rm["e_log"].tolist(), "t": (rm["t_log"] - rm["t0"]).tolist(),
+                                 "icg_low_precision": rm["icg_lowp"]}}, f)
     return out
 
 

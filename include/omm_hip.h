@@ -119,6 +119,9 @@ int omx_bsr_update_values(omx_bsr a, const double *bval);
 /* Copy the block values (16 per stored block, column-major) / the block pattern to the host. */
 int omx_bsr_get_values(omx_bsr a, double *bval);
 int omx_bsr_get_pattern(omx_bsr a, int *bcolptr, int *browidx);
+/* Element (i, j), 1-based, of the operator (0 when its block is not stored): m_get_element on a
+ * sparse matrix (MatrixSwitch/src/MatrixSwitch.F90:2917-3086, 'sdcsc' branch). */
+int omx_bsr_get_element(omx_bsr a, int i, int j, double *out);
 
 /* ==== MatrixSwitch operations on device matrices =========================================== */
 
@@ -139,6 +142,15 @@ int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *X, i
  */
 int omx_sddmm_bsr(omx_ctx ctx, int dt, int no, int nb, double alpha, const void *A, int64_t lda, const void *B,
                   int64_t ldb, double beta, omx_bsr Cp);
+/*
+ * Adjoint contraction dE/dtheta_p = sum_e J_p(e) * Dp(e) over the stored values of Dp (dE/dH on
+ * H's pattern, from omx_sddmm_bsr or omm() with a 'gdcsc' D_min) for a Hamiltonian linear in
+ * theta.  seg (nparam+1), ent, coef: device arrays in the layout of omx_lattice_theta_jacobian.
+ * grad: host, nparam <= 256.  No counterpart in the reference (libOMM has no derivatives); it is
+ * the backward pass of omm() with respect to the Hamiltonian's parameters.
+ */
+int omx_bsr_param_grad(omx_ctx ctx, omx_bsr Dp, int nparam, const int64_t *seg, const int64_t *ent,
+                       const double *coef, double *grad);
 
 /*
  * C := alpha*op(A)*op(B) + beta*C  (mm_multiply on dense operands,
@@ -150,8 +162,8 @@ int omx_sddmm_bsr(omx_ctx ctx, int dt, int no, int nb, double alpha, const void 
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
              int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
 /* SpMM launch shape of later products on this context: blocks whose loads are issued together
- * (1, 4 or 8; 0 = default 8) and block columns per workgroup (0 = chosen by size).  Tuning hook;
- * results do not depend on it. */
+ * (1, 4 or 8; 0 = default 8) and block columns (one wave each) per workgroup (1, 2 or 4;
+ * 0 = default 4).  Tuning hook; results do not depend on it. */
 int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper);
 /* Force the GEMM tile configuration of later products on this context (-1 restores the choice
  * by shape) for the FP64 Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
@@ -231,6 +243,16 @@ int omx_lattice_bsr(int Lx, int Ly, int Lz, uint64_t seed, int which, int64_t *n
 /* H(theta): one hopping block theta[0..15] (column-major) on every bond, onsite diag(theta[16..19]). */
 int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *nnzb, int *bcolptr, int *browidx,
                           double *bval);
+/* Number of generator parameters theta (16 hopping-block entries + 4 onsite levels). */
+#define OMX_THETA_COUNT 20
+/*
+ * Jacobian of H(theta) on its block pattern (H is linear in theta): for parameter p, entries
+ * seg[p]..seg[p+1]-1 of (ent, coef) give the stored value index e = 16*l + r + 4*c and dH(e)/dtheta_p.
+ * First call with seg == NULL returns the entry count in *nent; second call fills seg
+ * (OMX_THETA_COUNT+1), ent and coef (*nent each).  No counterpart in the reference (its inputs
+ * are files); it is the generator side of the adjoint.
+ */
+int omx_lattice_theta_jacobian(int Lx, int Ly, int Lz, int64_t *nent, int64_t *seg, int64_t *ent, double *coef);
 /* Initial parameters: hopping U(-1,1) from the seed, onsite levels (-5, 1, 1, 1). */
 int omx_lattice_theta0(uint64_t seed, double *theta);
 /* C0(i,k) = U(-0.5,0.5)/sqrt(nb), no x nb column-major. */

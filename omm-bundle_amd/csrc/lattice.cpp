@@ -142,6 +142,31 @@ int omx_lattice_bsr(int Lx, int Ly, int Lz, uint64_t seed, int which, int64_t *n
     return fill(blocks, (int64_t)Lx * Ly * Lz, nnzb, bcolptr, browidx, bval);
 }
 
+// Jacobian of H(theta) (linear, no constant term): column p is H(e_p) on the same block pattern,
+// listed as its nonzero stored entries (index 16*l + r + 4*c into the block values).
+int omx_lattice_theta_jacobian(int Lx, int Ly, int Lz, int64_t *nent, int64_t *seg, int64_t *ent, double *coef) {
+    if (Lx < 1 || Ly < 1 || Lz < 1 || !nent) return OMX_ERR_ARG;
+    int64_t k = 0;
+    if (seg) seg[0] = 0;
+    for (int p = 0; p < OMX_THETA_COUNT; ++p) {
+        double th[OMX_THETA_COUNT] = {0};
+        th[p] = 1.0;
+        auto blocks = build(Lx, Ly, Lz, 0, 2, th);
+        for (size_t l = 0; l < blocks.size(); ++l)
+            for (int q = 0; q < 16; ++q)
+                if (blocks[l].v[q] != 0.0) {
+                    if (ent) {
+                        ent[k] = (int64_t)l * 16 + q;
+                        coef[k] = blocks[l].v[q];
+                    }
+                    ++k;
+                }
+        if (seg) seg[p + 1] = k;
+    }
+    *nent = k;
+    return 0;
+}
+
 int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *nnzb, int *bcolptr,
                           int *browidx, double *bval) {
     if (Lx < 1 || Ly < 1 || Lz < 1 || !theta || !nnzb) return OMX_ERR_ARG;

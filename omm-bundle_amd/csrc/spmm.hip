@@ -7,16 +7,18 @@
 //     Y(:,4J+c) = sum over blocks (I,J), r = 0..3 of X(:,4I+r) * A_IJ(r,c),
 // summed in ascending (I, r), the order of the reference's CSC loop.
 //
-// Mapping: a workgroup owns a chunk of ROWS consecutive states (VEC per thread, loaded as one
-// 16-byte vector per basis column) and a run of JPER block columns.  Each thread keeps 4*VEC
-// accumulators and issues the loads of a whole batch of blocks before using them (memory-level
-// parallelism); the 16 block values are wave-uniform and come through the scalar cache.  A zero
-// entry stored inside a block contributes an exact zero, so the per-element sum equals the
-// reference's CSC sum over its stored entries.  The
-// workgroup id is remapped so that each XCD owns a contiguous band of state chunks and walks
-// the block columns in order: the neighbouring basis columns a block column gathers were read
-// a few block columns earlier by the same XCD and are still in its L2 (lattice neighbours in
-// +-x, +-y) or in the shared memory-side cache (+-z), so HBM sees X about once.
+// Mapping (band-major): the state index is cut into bands of ROWS = 64*VEC rows.  One wave
+// computes one (band, block column): each lane keeps 4*VEC accumulators, issues the loads of a
+// whole batch of blocks before using them (memory-level parallelism), and reads the 16 block
+// values as wave-uniform scalars.  A workgroup holds WPG waves on WPG consecutive block columns
+// of one band.  Workgroup ids are dealt round-robin over the 8 XCDs; each XCD owns every 8th band
+// and walks all block columns of one band before the next.  The workgroups resident on an XCD
+// therefore cover a window of consecutive block columns of a single band, and the ROWS-row
+// segments of the basis columns a block column gathers were loaded shortly before by the same
+// XCD: for a banded pattern (the lattice in natural order reaches +-Lx*Ly sites) they are
+// still in that XCD's L2, so HBM sees X about once.  A zero entry stored inside a block
+// contributes an exact zero, so the per-element sum equals the reference's CSC sum over its
+// stored entries.
 #include <algorithm>
 
 #include "omx_internal.h"
@@ -60,141 +62,141 @@ __device__ inline void vstore(T *p, const T (&in)[VEC]) {
     *reinterpret_cast<V *>(p) = v;
 }
 
-constexpr int SPMM_NT = 256;
-
-template <typename T, int VEC, int NB>
-__global__ __launch_bounds__(SPMM_NT) void k_spmm_bsr4(int no, int nb, int nbc, double alpha,
-                                                       const T *__restrict__ X, int64_t ldx,
-                                                       const int *__restrict__ colptr, const int *__restrict__ rowidx,
-                                                       const double *__restrict__ val, double beta,
-                                                       T *__restrict__ Y, int64_t ldy, int rchunks, int jper) {
-    constexpr int ROWS = SPMM_NT * VEC;
-    const int njg = (nbc + jper - 1) / jper;
-    const int total = rchunks * njg;
-    int w = blockIdx.x;
-    int rc, jg;
-    if ((rchunks & 7) == 0 && (total & 7) == 0) {
-        const int per = rchunks >> 3;          // state chunks per XCD
+template <typename T, int VEC, int NB, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4(int no, int nb, int nbc, double alpha,
+                                                      const T *__restrict__ X, int64_t ldx,
+                                                      const int *__restrict__ colptr, const int *__restrict__ rowidx,
+                                                      const double *__restrict__ val, double beta,
+                                                      T *__restrict__ Y, int64_t ldy, int nbands, int jgroups) {
+    constexpr int ROWS = 64 * VEC;
+    const int w = blockIdx.x;
+    int band, jg;
+    if ((nbands & 7) == 0) {
         const int xcd = w & 7, local = w >> 3;  // dispatch is round-robin over 8 XCDs
-        rc = xcd * per + local % per;
-        jg = local / per;
+        band = (local / jgroups) * 8 + xcd;
+        jg = local % jgroups;
     } else {
-        rc = w % rchunks;
-        jg = w / rchunks;
+        band = w / jgroups;
+        jg = w % jgroups;
     }
-    const int i0 = rc * ROWS + threadIdx.x * VEC;
+    const int J = jg * WPG + (threadIdx.x >> 6);
+    if (J >= nbc) return;  // whole wave; the kernel has no barriers
+    const int i0 = band * ROWS + (threadIdx.x & 63) * VEC;
     const bool full = (i0 + VEC <= no);
     const bool any = (i0 < no);
-    const int jbeg = jg * jper, jend = min(nbc, jbeg + jper);
 
-    for (int J = jbeg; J < jend; ++J) {
-        T acc[4][VEC];
+    T acc[4][VEC];
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int q = 0; q < VEC; ++q) acc[c][q] = (T)0;
-        const int lb = colptr[J], le = colptr[J + 1];
-        // Blocks are taken in batches of up to NB: all their basis columns are requested before
-        // any is used, so each thread keeps 4*NB vector loads in flight; the accumulation then
-        // runs in ascending (block, r) order.
-        for (int l0 = lb; l0 < le; l0 += NB) {
-            const int nblk = min(NB, le - l0);
-            T x[NB][4][VEC];
+        for (int q = 0; q < VEC; ++q) acc[c][q] = (T)0;
+    const int lb = colptr[J], le = colptr[J + 1];
+    // Blocks are taken in batches of up to NB: all their basis columns are requested before any
+    // is used, so each lane keeps 4*NB vector loads in flight; the accumulation then runs in
+    // ascending (block, r) order.
+    for (int l0 = lb; l0 < le; l0 += NB) {
+        const int nblk = min(NB, le - l0);
+        T x[NB][4][VEC];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (b < nblk) {
-                    const int I = rowidx[l0 + b];
+        for (int b = 0; b < NB; ++b) {
+            if (b < nblk) {
+                const int I = rowidx[l0 + b];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int kcol = 4 * I + r;
-                        const T *src = X + (int64_t)kcol * ldx + i0;
-                        if (kcol < nb && full) {
-                            vload<T, VEC>(src, x[b][r]);
-                        } else {
+                for (int r = 0; r < 4; ++r) {
+                    const int kcol = 4 * I + r;
+                    const T *src = X + (int64_t)kcol * ldx + i0;
+                    if (kcol < nb && full) {
+                        vload<T, VEC>(src, x[b][r]);
+                    } else {
 #pragma unroll
-                            for (int q = 0; q < VEC; ++q) x[b][r][q] = (kcol < nb && any && i0 + q < no) ? src[q] : (T)0;
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (b < nblk) {
-                    const double *v = val + 16 * (int64_t)(l0 + b);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const T a = (T)v[r + 4 * c];
-#pragma unroll
-                            for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[b][r][q], a, acc[c][q]);
-                        }
+                        for (int q = 0; q < VEC; ++q) x[b][r][q] = (kcol < nb && any && i0 + q < no) ? src[q] : (T)0;
                     }
                 }
             }
         }
-        if (!any) continue;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int jcol = 4 * J + c;
-            if (jcol >= nb) break;
-            T *dst = Y + (int64_t)jcol * ldy + i0;
-            T out[VEC];
-            if (full) {
-                if (beta != 0.0) {
-                    T old[VEC];
-                    vload<T, VEC>(dst, old);
+        for (int b = 0; b < NB; ++b) {
+            if (b < nblk) {
+                const double *v = val + 16 * (int64_t)(l0 + b);
 #pragma unroll
-                    for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q] + beta * (double)old[q]);
-                } else {
+                for (int r = 0; r < 4; ++r) {
 #pragma unroll
-                    for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q]);
-                }
-                vstore<T, VEC>(dst, out);
-            } else {
+                    for (int c = 0; c < 4; ++c) {
+                        const T a = (T)v[r + 4 * c];
 #pragma unroll
-                for (int q = 0; q < VEC; ++q)
-                    if (i0 + q < no) {
-                        double o = alpha * (double)acc[c][q];
-                        if (beta != 0.0) o += beta * (double)dst[q];
-                        dst[q] = (T)o;
+                        for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[b][r][q], a, acc[c][q]);
                     }
+                }
             }
         }
     }
+    if (!any) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int jcol = 4 * J + c;
+        if (jcol >= nb) break;
+        T *dst = Y + (int64_t)jcol * ldy + i0;
+        T out[VEC];
+        if (full) {
+            if (beta != 0.0) {
+                T old[VEC];
+                vload<T, VEC>(dst, old);
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q] + beta * (double)old[q]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q]);
+            }
+            vstore<T, VEC>(dst, out);
+        } else {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q)
+                if (i0 + q < no) {
+                    double o = alpha * (double)acc[c][q];
+                    if (beta != 0.0) o += beta * (double)dst[q];
+                    dst[q] = (T)o;
+                }
+        }
+    }
+}
+
+template <typename T, int VEC, int NB>
+static void launch_spmm(omx_ctx ctx, int wpg, int no, int nb, double alpha, const T *X,
+                        int64_t ldx, omx_bsr A, double beta, T *Y, int64_t ldy, int nbands) {
+    const int jgroups = (A->nbc + wpg - 1) / wpg;
+    const unsigned grid = (unsigned)((int64_t)nbands * jgroups);
+#define OMX_SPMM_LAUNCH(W)                                                                                           \
+    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NB, W>), dim3(grid), dim3(64 * W), 0, ctx->stream, no, nb, A->nbc, alpha, \
+                       X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nbands, jgroups)
+    if (wpg == 1)
+        OMX_SPMM_LAUNCH(1);
+    else if (wpg == 2)
+        OMX_SPMM_LAUNCH(2);
+    else
+        OMX_SPMM_LAUNCH(4);
+#undef OMX_SPMM_LAUNCH
 }
 
 template <typename T, int VEC>
 static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64_t ldx, omx_bsr A, double beta, T *Y,
                     int64_t ldy) {
-    constexpr int ROWS = SPMM_NT * VEC;
-    int rchunks = (no + ROWS - 1) / ROWS;
-    // Enough workgroups to cover the chip several times; each walks jper block columns.
-    int jper = 1;
-    if (ctx->spmm_jper > 0) {
-        jper = ctx->spmm_jper;
-    } else {
-        int64_t target = 16LL * ctx->num_cus;
-        while ((int64_t)rchunks * ((A->nbc + jper * 2 - 1) / (jper * 2)) >= target && jper < 64) jper *= 2;
-    }
-    int njg = (A->nbc + jper - 1) / jper;
-    int64_t total = (int64_t)rchunks * njg;
+    constexpr int ROWS = 64 * VEC;
+    const int nbands = (no + ROWS - 1) / ROWS;
+    int wpg = ctx->spmm_jper > 0 ? ctx->spmm_jper : 4;
+    wpg = wpg >= 4 ? 4 : (wpg >= 2 ? 2 : 1);
+    const int64_t total = (int64_t)nbands * ((A->nbc + wpg - 1) / wpg);
     OMX_REQUIRE(total < (1LL << 31), "omx_spmm: grid too large");
     const int batch = ctx->spmm_batch > 0 ? ctx->spmm_batch : 8;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_SPMM, &ev);
     prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * A->nnzb * no,
               sizeof(T) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (16.0 * 8 + 4) + 4.0 * (A->nbc + 1));
-#define OMX_SPMM_LAUNCH(NBV)                                                                                         \
-    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NBV>), dim3((unsigned)total), dim3(SPMM_NT), 0, ctx->stream, no, nb,     \
-                       A->nbc, alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, rchunks, jper)
     if (batch >= 8)
-        OMX_SPMM_LAUNCH(8);
+        launch_spmm<T, VEC, 8>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
     else if (batch >= 4)
-        OMX_SPMM_LAUNCH(4);
+        launch_spmm<T, VEC, 4>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
     else
-        OMX_SPMM_LAUNCH(1);
-#undef OMX_SPMM_LAUNCH
+        launch_spmm<T, VEC, 1>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
     prof_end(ctx, OMX_K_SPMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;

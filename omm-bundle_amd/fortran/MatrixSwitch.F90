@@ -568,7 +568,16 @@ contains
 
     if (A%is_device .and. B%is_device .and. C%is_device) then
       call check_device_label(label, 'mm_dmultiply')
-      if (C%is_sparse .or. A%is_sparse) call die('mm_dmultiply: invalid implementation')
+      if (C%is_sparse) then
+        ! dense x dense -> sparse pattern, op = ('c','n') (MatrixSwitch_mm_multiply.F90:334-398): SDDMM
+        if (A%is_sparse .or. B%is_sparse .or. (.not. trA) .or. trB) call die('mm_dmultiply: invalid implementation')
+        if (A%dtype /= B%dtype) call die('mm_dmultiply: element types differ')
+        rc = omx_sddmm_bsr(ctx(), A%dtype, int(A%dim1, c_int), int(A%dim2, c_int), alpha, A%dptr, A%ld, B%dptr, &
+                           B%ld, beta, C%bsr)
+        call chk(rc, 'mm_dmultiply')
+        return
+      end if
+      if (A%is_sparse) call die('mm_dmultiply: invalid implementation')
       if (B%is_sparse) then
         ! dense x sparse operator: the SpMM (omm_ops.F90:61)
         if (trA .or. trB) call die('mm_dmultiply: only op = n for the sparse operator product')
@@ -615,6 +624,22 @@ contains
           k = B%iaux3(l)
           do i = 1, C%dim1
             C%dval(i,j) = C%dval(i,j) + alpha*aval(A, trA, i, k)*B%dval(l,1)
+          end do
+        end do
+      end do
+    else if (.not. A%is_sparse .and. .not. B%is_sparse .and. C%is_sparse) then
+      ! dense x dense -> CSC pattern of C (MatrixSwitch_mm_multiply.F90:334-398)
+      C%dval = beta*C%dval
+      if (trA) then
+        kk = A%dim1
+      else
+        kk = A%dim2
+      end if
+      do j = 1, C%dim2
+        do l = C%iaux4(j)+1, C%iaux4(j+1)
+          i = C%iaux3(l)
+          do k = 1, kk
+            C%dval(l,1) = C%dval(l,1) + alpha*aval(A, trA, i, k)*aval(B, trB, k, j)
           end do
         end do
       end do
@@ -819,7 +844,11 @@ contains
     integer :: l
     if (i < 1 .or. i > C%dim1 .or. j < 1 .or. j > C%dim2) call die('m_dget_element: element out of range')
     if (C%is_device) then
-      if (C%is_sparse) call die('m_dget_element: invalid implementation')
+      if (C%is_sparse) then
+        rc = omx_bsr_get_element(C%bsr, int(i, c_int), int(j, c_int), alpha)
+        call chk(rc, 'm_dget_element')
+        return
+      end if
       rc = omx_get_element(ctx(), C%dtype, C%dptr, C%ld, int(i, c_int), int(j, c_int), alpha)
       call chk(rc, 'm_dget_element')
       return

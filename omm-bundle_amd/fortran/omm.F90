@@ -10,9 +10,14 @@
 ! fused FP64 reductions; 'sdden' keeps the reference host semantics.                              !
 !                                                                                                  !
 ! m_operation: 'hip' FP64, 'h32' FP32 products, 'hbf' BF16 products (FP32 accumulate), and 'hmx': !
-! mixed precision -- CG with low-precision products until the relative energy change falls below  !
-! the 'lowp_tol' option, then FP64 CG from the resulting coefficients (all quantities recomputed   !
-! exactly) to cg_tol.  Traces, the line search and the energy are FP64 in every mode.              !
+! mixed precision -- a low-precision CG phase on an FP32 working set, then FP64 CG from the        !
+! resulting coefficients (all quantities recomputed exactly) to cg_tol.  In the low-precision      !
+! phase the line-search Grams (HWd, SWd, HWdd, SWdd) are BF16 products and the gradient is an FP32 !
+! product; HW and SW are updated incrementally from the BF16 Grams and recomputed from C in FP32   !
+! every 'lowp_rewin' steps, which bounds their drift by a multiple of the step length.  The phase  !
+! ends when the energy decrease predicted by a line search (its quartic, independent of the drift  !
+! of HW/SW) stays below 'lowp_tol' relative for 'lowp_consec' consecutive steps.  Traces, the line !
+! search and the energy are FP64 in every mode.                                                    !
 !                                                                                                  !
 ! Flavour 0 (basic) is implemented; flavours 1-4 (Cholesky / preconditioned) stop with die().      !
 !==================================================================================================!
@@ -156,7 +161,11 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
     call point_lowp()
     call alloc_locals('gsden')
     call initial_energy(opt_lowp_label, .true.)
-    call cg(opt_lowp_label, opt_lowp_tol, opt_lowp_max_iter)
+    if (opt_lowp_tol < 0.0_dp) then
+      call cg(opt_lowp_label, 0.3_dp*cg_tol_internal, opt_lowp_max_iter, .true.)
+    else
+      call cg(opt_lowp_label, opt_lowp_tol, opt_lowp_max_iter, .true.)
+    end if
     icg_lowp = icg
     call ms_convert(lC, C_min)
     call free_locals()
@@ -164,12 +173,12 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
     call point_state()
     call alloc_locals(m_storage)
     call initial_energy('hip', .true.)
-    call cg('hip', cg_tol_internal, opt_max_iter)
+    call cg('hip', cg_tol_internal, opt_max_iter, .false.)
   else
     call point_state()
     call alloc_locals(m_storage)
     call initial_energy(prod, first_call(ip) .or. init_C .or. new_S)
-    call cg(prod, cg_tol_internal, opt_max_iter)
+    call cg(prod, cg_tol_internal, opt_max_iter, .false.)
   end if
   if ((mpi_rank0()) .and. long_out) write(log_unit, '(a)') '+---------------------------------------------+'
   call free_locals()
@@ -285,15 +294,20 @@ contains
 
   !> Polak-Ribiere CG with exact quartic line search (omm.F90:429-555).
   !> Stops when e_diff <= tol, or after max_steps steps of this phase (0 = reference caps only).
-  subroutine cg(lab, tol, max_steps)
+  !> lowp: the low-precision phase of 'hmx' (periodic recomputation of the state from C, and the
+  !> stop test on the decrease predicted by the line search).
+  subroutine cg(lab, tol, max_steps, lowp)
     character(3), intent(in) :: lab
     real(dp), intent(in) :: tol
     integer, intent(in) :: max_steps
-    integer :: outer, inner, steps
-    logical :: ls_conv, capped
+    logical, intent(in) :: lowp
+    integer :: outer, inner, steps, n_below
+    logical :: ls_conv, capped, rewin_now
+    real(dp) :: e_pred
     conv = .false.
     capped = .false.
     steps = 0
+    n_below = 0
     stat_t0 = wall_seconds()  ! the coefficient readback that precedes this point left the stream idle
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
@@ -343,6 +357,13 @@ contains
         e_diff = 2.0_dp*abs((e_min - e_min_old)/(e_min + e_min_old))
         if (mpi_rank0() .and. long_out) write(log_unit, '(a,2(1x,i5),2(1x,es15.7e3),1x,a)') '|', outer, inner, &
           e_min, e_diff, '|'
+        if (opt_progress > 0) then
+          if (mod(icg + 1, opt_progress) == 0) then
+            write(0, '(a,a3,a,i6,a,es22.14,a,es10.3)') 'omm: ', lab, ' icg=', icg + 1, ' e_min=', e_min, &
+              ' e_diff=', e_diff
+            flush(0)
+          end if
+        end if
         if (stat_nlog < stat_log_cap) then
           stat_nlog = stat_nlog + 1
           stat_elog(stat_nlog) = e_min
@@ -350,17 +371,31 @@ contains
         end if
         icg = icg + 1
         steps = steps + 1
-        if (e_diff <= tol) then
+        if (lowp) then
+          ! relative decrease predicted by this step's quartic from the energy of the current state,
+          ! below tol on opt_lowp_consec consecutive steps
+          if (.not. ls_fail) then
+            e_pred = 2.0_dp*abs((e_min - coeff(0))/(e_min + coeff(0)))
+            if (e_pred <= tol) then
+              n_below = n_below + 1
+            else
+              n_below = 0
+            end if
+            if (n_below >= opt_lowp_consec) conv = .true.
+          end if
+        else if (e_diff <= tol) then
           conv = .true.
-          exit
         end if
+        if (conv) exit
         if (max_steps > 0 .and. steps >= max_steps) then
           capped = .true.
           exit
         end if
+        rewin_now = lowp .and. (.not. ls_fail) .and. opt_lowp_rewin > 0
+        if (rewin_now) rewin_now = (mod(steps, opt_lowp_rewin) == 0)
         if (ls_fail) then
           call m_scale(pHW, 0.25_dp, scale_label())
-        else
+        else if (.not. rewin_now) then
           call update_W(pHW, pHWd, pHWdd, x_min(ip))
         end if
         if (ls_fail) then
@@ -369,11 +404,21 @@ contains
           call m_add(G_p, 'n', G, 1.0_dp, 0.0_dp, scale_label())
           call m_add(HC, 'n', G, 1.5_dp, 1.0_dp, scale_label())
         else
-          call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
+          if (rewin_now) then
+            ! HC = C H, HW = HC C^T, SC = C S, SW = SC C^T at the gradient's precision
+            call calc_AW(H, pC, pHW, HC, grad_label(lab))
+            if (no_S) then
+              call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, grad_label(lab))
+            else
+              call calc_AW(S, pC, pSW, pSC, grad_label(lab))
+            end if
+          else
+            call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
+            if (.not. no_S) call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
+          end if
           if (no_S) then
             call calc_G(pHW, pSW, G, HC, pC, grad_label(lab))
           else
-            call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
             call calc_G(pHW, pSW, G, HC, pSC, grad_label(lab))
           end if
         end if

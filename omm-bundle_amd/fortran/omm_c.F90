@@ -275,6 +275,27 @@ contains
     omm_c_set_option = 0
   end function omm_c_set_option
 
+  !> dE/dtheta from a sparse device D (dE/dH on H's pattern) and a device Jacobian
+  !> (omm_param_grad); element (i, j) of any matrix through m_get_element.
+  integer(c_int) function omm_c_param_grad(hD, nparam, seg, ent, coef, grad) bind(c, name='omm_c_param_grad')
+    integer(c_int), value :: hD, nparam
+    type(c_ptr), value :: seg, ent, coef
+    real(c_double), intent(out) :: grad(*)
+    omm_c_param_grad = -1
+    if (.not. valid(hD)) return
+    call omm_param_grad(table(hD), int(nparam), seg, ent, coef, grad(1:nparam))
+    omm_c_param_grad = 0
+  end function omm_c_param_grad
+
+  integer(c_int) function omm_c_get_element(h, i, j, val) bind(c, name='omm_c_get_element')
+    integer(c_int), value :: h, i, j
+    real(c_double), intent(out) :: val
+    omm_c_get_element = -1
+    if (.not. valid(h)) return
+    call m_get_element(table(h), int(i), int(j), val)
+    omm_c_get_element = 0
+  end function omm_c_get_element
+
   integer(c_int) function omm_c_sync() bind(c, name='omm_c_sync')
     call ms_hip_sync()
     omm_c_sync = 0

@@ -19,11 +19,17 @@ module omm_hip_ops
 
   ! ---- options (set with omm_set_option; defaults reproduce the reference) ----
   integer, save :: opt_max_iter = 0          ! > 0: cap on CG steps per call (icg)
-  real(dp), save :: opt_lowp_tol = 1.0e-6_dp ! mixed precision: e_diff that ends the low-precision phase
+  ! mixed precision: predicted relative energy decrease of one line-search step that ends the
+  ! low-precision phase (< 0: 0.3*cg_tol)
+  real(dp), save :: opt_lowp_tol = -1.0_dp
   integer, save :: opt_lowp_max_iter = 0     ! mixed precision: cap on low-precision steps (0 = none)
   character(3), save :: opt_lowp_label = 'hbf'
   logical, save :: opt_fused = .true.        ! use fused device kernels when all operands are on device
-  logical, save :: opt_grad_fp32 = .false.   ! BF16 phase: gradient products in FP32 (Grams stay BF16)
+  integer, save :: opt_progress = 0          ! > 0: one line on stderr every opt_progress CG steps
+  logical, save :: opt_grad_fp32 = .true.    ! BF16 phase: gradient and state products in FP32
+  integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
+  integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
+                                             ! this many steps (0 = incremental updates only)
 
   ! ---- statistics of the last omm() call ----
   integer, save :: stat_icg = 0, stat_conv = 0, stat_nfail = 0, stat_icg_lowp = 0
@@ -224,6 +230,22 @@ contains
     stop 1
   end subroutine omm_die
 
+  !> Adjoint of the converged energy for a Hamiltonian linear in parameters theta:
+  !> grad(p) = sum_e J_p(e) * Dm(e), with Dm = dE/dH on H's pattern (the 'gdcsc' D_min of a
+  !> converged omm() call, C^T (2I - SW) C) and the Jacobian J as device arrays seg (nparam+1),
+  !> ent, coef in the layout of omx_lattice_theta_jacobian.  libOMM has no derivatives; this is
+  !> the backward pass of omm() with respect to the Hamiltonian's parameters.
+  subroutine omm_param_grad(Dm, nparam, seg, ent, coef, grad)
+    type(matrix), intent(in) :: Dm
+    integer, intent(in) :: nparam
+    type(c_ptr), intent(in) :: seg, ent, coef
+    real(dp), intent(out) :: grad(nparam)
+    integer(c_int) :: rc
+    if (.not. (Dm%is_device .and. Dm%is_sparse)) call omm_die('omm_param_grad: D must be a device sparse matrix')
+    rc = omx_bsr_param_grad(ms_hip_ctx(), Dm%bsr, int(nparam, c_int), seg, ent, coef, grad)
+    if (rc /= 0) call omm_die('omm_param_grad: '//omx_error_string())
+  end subroutine omm_param_grad
+
   !> Set a solver option by name (extensions beyond the omm() argument list).
   subroutine omm_set_option(name, value)
     character(*), intent(in) :: name
@@ -243,6 +265,12 @@ contains
       end if
     case ('fused')
       opt_fused = (value /= 0.0_dp)
+    case ('progress')
+      opt_progress = nint(value)
+    case ('lowp_consec')
+      opt_lowp_consec = max(1, nint(value))
+    case ('lowp_rewin')
+      opt_lowp_rewin = nint(value)
     case ('lowp_grad_fp32')
       opt_grad_fp32 = (value /= 0.0_dp)
     case default

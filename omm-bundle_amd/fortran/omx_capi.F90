@@ -88,6 +88,27 @@ module omx_capi
       real(c_double), value :: alpha, beta
       integer(c_int64_t), value :: ldx, ldy
     end function
+    integer(c_int) function omx_sddmm_bsr(ctx, dt, no, nb, alpha, A, lda, B, ldb, beta, Cp) &
+        bind(c, name='omx_sddmm_bsr')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx, A, B, Cp
+      integer(c_int), value :: dt, no, nb
+      real(c_double), value :: alpha, beta
+      integer(c_int64_t), value :: lda, ldb
+    end function
+    integer(c_int) function omx_bsr_get_element(a, i, j, out) bind(c, name='omx_bsr_get_element')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: a
+      integer(c_int), value :: i, j
+      real(c_double), intent(out) :: out
+    end function
+    integer(c_int) function omx_bsr_param_grad(ctx, Dp, nparam, seg, ent, coef, grad) &
+        bind(c, name='omx_bsr_param_grad')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, Dp, seg, ent, coef
+      integer(c_int), value :: nparam
+      real(c_double), intent(out) :: grad(*)
+    end function
     integer(c_int) function omx_gemm(ctx, prec, dt, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc) &
         bind(c, name='omx_gemm')
       import :: c_int, c_ptr, c_int64_t, c_double, c_char

@@ -49,6 +49,8 @@ SIGNATURES = {
     "omx_bsr_get_values": (_i, [_vp, _vp]),
     "omx_bsr_get_pattern": (_i, [_vp, _vp, _vp]),
     "omx_sddmm_bsr": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp]),
+    "omx_bsr_param_grad": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "omx_bsr_get_element": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     "omx_gemm_set_tile": (_i, [_vp, _i]),
@@ -72,6 +74,7 @@ SIGNATURES = {
     "omx_lattice_bsr": (_i, [_i, _i, _i, C.c_uint64, _i, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_lattice_bsr_theta": (_i, [_i, _i, _i, _vp, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_lattice_theta0": (_i, [C.c_uint64, _vp]),
+    "omx_lattice_theta_jacobian": (_i, [_i, _i, _i, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_random_c0": (_i, [_i, _i, C.c_uint64, _vp, _i64]),
     "omx_u01": (_d, [C.c_uint64, C.c_uint64, C.c_uint64]),
 }
@@ -119,6 +122,7 @@ def device_count() -> int:
 # ---------------------------------------------------------------------------------------------
 
 DEFAULT_SEED = 0x0EEDC0DE
+THETA_COUNT = 20
 
 
 def lattice_bsr(L, which: int, seed: int = DEFAULT_SEED):
@@ -149,6 +153,20 @@ def lattice_bsr_theta(L, theta):
     return cp, ri, bv
 
 
+def lattice_theta_jacobian(L):
+    """Jacobian of H(theta) on the lattice block pattern: (seg[21], ent, coef); parameter p owns
+    entries seg[p]..seg[p+1]-1, ent = index 16*l + r + 4*c into the block values."""
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    nent = C.c_int64(0)
+    check(lib().omx_lattice_theta_jacobian(Lx, Ly, Lz, C.byref(nent), None, None, None), "theta jacobian")
+    seg = np.zeros(THETA_COUNT + 1, np.int64)
+    ent = np.zeros(max(nent.value, 1), np.int64)
+    coef = np.zeros(max(nent.value, 1), np.float64)
+    check(lib().omx_lattice_theta_jacobian(Lx, Ly, Lz, C.byref(nent), _ptr(seg), _ptr(ent), _ptr(coef)),
+          "theta jacobian")
+    return seg, ent[:nent.value], coef[:nent.value]
+
+
 def lattice_theta0(seed: int = DEFAULT_SEED):
     th = np.zeros(20, np.float64)
     check(lib().omx_lattice_theta0(seed, _ptr(th)), "theta0")
@@ -163,6 +181,19 @@ def random_c0(no: int, nb: int, seed: int = DEFAULT_SEED):
 
 def u01(seed, stream, idx) -> float:
     return lib().omx_u01(seed, stream, idx)
+
+
+def param_grad(ctx, Dp: "Bsr", seg_d: "DMat", ent_d: "DMat", coef_d: "DMat", nparam: int) -> np.ndarray:
+    """dE/dtheta from dE/dH on the pattern (Dp) and a device Jacobian (omx_bsr_param_grad)."""
+    out = np.zeros(nparam)
+    check(lib().omx_bsr_param_grad(ctx.h, Dp.h, nparam, seg_d.p, ent_d.p, coef_d.p, _ptr(out)), "param_grad")
+    return out
+
+
+def bsr_get_element(Dp: "Bsr", i: int, j: int) -> float:
+    out = C.c_double(0)
+    check(lib().omx_bsr_get_element(Dp.h, i, j, C.byref(out)), "bsr_get_element")
+    return out.value
 
 
 def bsr_to_dense(nb: int, cp, ri, bv) -> np.ndarray:
@@ -310,6 +341,14 @@ class Bsr:
         out = np.zeros((self.nnzb, 16))
         check(lib().omx_bsr_get_values(self.h, _ptr(out)), "bsr_get_values")
         return out
+
+    def pattern(self):
+        """(block colptr, block row indices) of the stored blocks, 0-based."""
+        nbc = (self.nb + 3) // 4
+        cp = np.zeros(nbc + 1, np.int32)
+        ri = np.zeros(max(self.nnzb, 1), np.int32)
+        check(lib().omx_bsr_get_pattern(self.h, _ptr(cp), _ptr(ri)), "bsr_get_pattern")
+        return cp, ri[: self.nnzb]
 
     def free(self):
         if self.h:

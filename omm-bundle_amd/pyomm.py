@@ -41,6 +41,8 @@ def lib():
         L.omm_c_set_option.argtypes = [C.c_char_p, _d]
         L.omm_c_tlog.argtypes = [_vp, _i]
         L.omm_c_ctx.restype = _vp
+        L.omm_c_param_grad.argtypes = [_i, _i, _vp, _vp, _vp, _vp]
+        L.omm_c_get_element.argtypes = [_i, _i, _i, C.POINTER(_d)]
         _lib = L
     return _lib
 
@@ -104,6 +106,13 @@ class Mat:
         p = lib().omm_c_mat_dptr(self.h, C.byref(ld))
         return p, ld.value
 
+    def get_element(self, i, j):
+        """Element (i, j), 1-based (m_get_element; sparse device matrices included)."""
+        v = C.c_double(0)
+        if lib().omm_c_get_element(self.h, i, j, C.byref(v)) != 0:
+            raise RuntimeError("get_element failed")
+        return v.value
+
     def free(self):
         if self.h > 0:
             lib().omm_c_mat_free(self.h)
@@ -142,6 +151,14 @@ def omm(m, n, H: Mat, S: Mat | None, C_min: Mat, *, new_S=True, D_min: Mat | Non
     return {"e_min": e.value, "icg": icg.value, "converged": bool(conv.value), "nfail": nfail.value,
             "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy(), "t0": tbuf[0],
             "t_log": tbuf[1:kt].copy()}
+
+
+def param_grad(D: Mat, nparam: int, seg_p, ent_p, coef_p) -> np.ndarray:
+    """dE/dtheta (omm_param_grad) from a sparse device D_min and device Jacobian arrays."""
+    out = np.zeros(nparam)
+    if lib().omm_c_param_grad(D.h, nparam, seg_p, ent_p, coef_p, _p(out)) != 0:
+        raise RuntimeError("param_grad failed (bad handle)")
+    return out
 
 
 def ctx_handle():

@@ -203,3 +203,79 @@ def test_cg_updates(ctx):
     oh.omm_update_w(ctx, -0.37, dWd, dWdd, dW)
     ref = W - 0.37 * (Wd.T + Wd) + 0.37 ** 2 * Wdd
     assert np.max(np.abs(dW.download() - ref)) < 1e-14 * 20
+
+
+def test_bsr_roundtrip(ctx):
+    # values and pattern download return what was uploaded; update_values replaces the values
+    L = 3
+    _note("bsr_roundtrip", L)
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    A = oh.Bsr(ctx, 4 * L ** 3, cp, ri, bv)
+    gcp, gri = A.pattern()
+    assert np.array_equal(gcp, cp) and np.array_equal(gri, ri)
+    assert np.array_equal(A.values().ravel(), np.asarray(bv).ravel())
+    bv2 = rng.standard_normal(np.asarray(bv).shape)
+    from ctypes import c_void_p
+    oh.check(oh.lib().omx_bsr_update_values(A.h, bv2.ctypes.data_as(c_void_p)), "update_values")
+    assert np.array_equal(A.values().ravel(), bv2.ravel())
+
+
+def _pattern_mask(nb, cp, ri):
+    M = np.zeros((nb, nb), bool)
+    for jb in range(len(cp) - 1):
+        for l in range(cp[jb], cp[jb + 1]):
+            i0, j0 = 4 * ri[l], 4 * jb
+            M[i0:i0 + 4, j0:j0 + 4] = True
+    return M
+
+
+@pytest.mark.parametrize("L,no,beta", [(3, 54, 0.0), (4, 127, 0.5), (5, 250, -1.0)])
+def test_sddmm_lattice(ctx, L, no, beta):
+    # Cp := alpha*(A^T B) + beta*Cp on the stored blocks of the lattice H pattern
+    _note("sddmm", L, no, beta)
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    nb = 4 * L ** 3
+    P = oh.Bsr(ctx, nb, cp, ri, bv)
+    Hd = oh.bsr_to_dense(nb, cp, ri, bv)
+    A, B = rng.standard_normal((no, nb)), rng.standard_normal((no, nb))
+    dA, dB = oh.DMat.from_host(ctx, A), oh.DMat.from_host(ctx, B)
+    oh.sddmm(ctx, dA, dB, P, alpha=2.0, beta=beta)
+    mask = _pattern_mask(nb, cp, ri)
+    ref = np.where(mask, 2.0 * (A.T @ B) + beta * Hd, 0.0)
+    got = oh.bsr_to_dense(nb, cp, ri, P.values())
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.sqrt(no) * 10
+
+
+def test_sddmm_general_pattern(ctx):
+    # non-multiple-of-4 basis from 1-based CSC; padding rows/columns of edge blocks stay zero
+    nb, no = 37, 21
+    _note("sddmm_general", nb, no)
+    Am = rng.standard_normal((nb, nb)) * (rng.random((nb, nb)) < 0.2)
+    cols = [np.nonzero(Am[:, j])[0] for j in range(nb)]
+    colptr = np.cumsum([0] + [len(c) for c in cols]).astype(np.int32)
+    rows = np.concatenate(cols).astype(np.int32) + 1
+    vals = np.concatenate([Am[c, j] for j, c in enumerate(cols)])
+    P = oh.Bsr.from_csc(ctx, nb, colptr, rows, vals, ptrbase=0, rowbase=1)
+    X, Y = rng.standard_normal((no, nb)), rng.standard_normal((no, nb))
+    oh.sddmm(ctx, oh.DMat.from_host(ctx, X), oh.DMat.from_host(ctx, Y), P)
+    gcp, gri = P.pattern()
+    nbp = 4 * ((nb + 3) // 4)
+    got = oh.bsr_to_dense(nbp, gcp, gri, P.values())
+    mask = _pattern_mask(nbp, gcp, gri)[:nb, :nb]
+    ref = np.where(mask, X.T @ Y, 0.0)
+    assert np.max(np.abs(got[:nb, :nb] - ref)) < 1e-12 * 50
+    assert np.all(got[nb:, :] == 0) and np.all(got[:, nb:] == 0)
+
+
+def test_sddmm_f32(ctx):
+    L, no = 4, 200
+    _note("sddmm_f32", L, no)
+    cp, ri, bv = oh.lattice_bsr(L, 1)
+    nb = 4 * L ** 3
+    P = oh.Bsr(ctx, nb, cp, ri, bv)
+    A = rng.standard_normal((no, nb)).astype(np.float32)
+    B = rng.standard_normal((no, nb)).astype(np.float32)
+    oh.sddmm(ctx, oh.DMat.from_host(ctx, A, oh.OMX_F32), oh.DMat.from_host(ctx, B, oh.OMX_F32), P)
+    ref = np.where(_pattern_mask(nb, cp, ri), A.astype(np.float64).T @ B.astype(np.float64), 0.0)
+    got = oh.bsr_to_dense(nb, cp, ri, P.values())
+    assert np.max(np.abs(got - ref)) < 1e-4
