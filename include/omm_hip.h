@@ -166,9 +166,10 @@ int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, in
  * 0 = default 4).  Tuning hook; results do not depend on it. */
 int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper);
 /* Force the GEMM tile configuration of later products on this context (-1 restores the choice
- * by shape) for the FP64 Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
- * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256.  Used by the tile autotuner; results do
- * not depend on it. */
+ * by shape) for the Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
+ * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256 (FP64 products); 4 = 256x256 with 8 waves
+ * (FP32 and BF16 products on FP32 storage).  A configuration that does not apply to a product
+ * falls back to 0.  Used by the tile autotuner; results do not depend on it. */
 int omx_gemm_set_tile(omx_ctx ctx, int cfg);
 /* C := alpha*op(A) + beta*C  (m_add, MatrixSwitch.F90:1731-1902; MatrixSwitch_m_add.F90:95-175). */
 int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const void *A, int64_t lda, double beta,
@@ -253,7 +254,8 @@ int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *
  * are files); it is the generator side of the adjoint.
  */
 int omx_lattice_theta_jacobian(int Lx, int Ly, int Lz, int64_t *nent, int64_t *seg, int64_t *ent, double *coef);
-/* Initial parameters: hopping U(-1,1) from the seed, onsite levels (-5, 1, 1, 1). */
+/* Initial parameters: hopping U(-1,1) from the seed, onsite levels (-6, -6, 6, 6) (half filling
+ * then has a gap of about 3.8). */
 int omx_lattice_theta0(uint64_t seed, double *theta);
 /* C0(i,k) = U(-0.5,0.5)/sqrt(nb), no x nb column-major. */
 int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc);

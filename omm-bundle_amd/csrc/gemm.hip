@@ -19,6 +19,7 @@
 // so that each XCD (ids are dealt round-robin over the 8 XCDs) walks a contiguous band of tiles,
 // grouped GM tile-rows at a time, so operand panels are shared inside one XCD's L2.
 #include <algorithm>
+#include <type_traits>
 
 #include "omx_internal.h"
 
@@ -83,7 +84,8 @@ typedef Cfg<128, 128, 2, 2, 2> Cfg0;  // 4 waves, 64x64 per wave, two workgroups
 typedef Cfg<128, 128, 2, 2, 1> Cfg1;  // 4 waves, 64x64 per wave, one workgroup per CU
 typedef Cfg<256, 128, 2, 2, 1> Cfg2;  // 4 waves, 128x64 per wave
 typedef Cfg<128, 256, 2, 2, 1> Cfg3;  // 4 waves, 64x128 per wave
-constexpr int kNumCfg = 4;
+typedef Cfg<256, 256, 2, 4, 1> Cfg4;  // 8 waves, 128x64 per wave (BF16 / FP32 products)
+constexpr int kNumCfg = 5;
 
 // LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
 // lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
@@ -462,6 +464,12 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                     const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
                     int64_t ldc) {
     constexpr bool tunable = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
+    constexpr bool tunable_lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
+    if constexpr (tunable_lowp) {
+        if (pick_cfg<PREC>(ctx, m, n) == 4)
+            return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg4>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
+                                                            beta, C, ldc);
+    }
     if constexpr (tunable) {
         switch (pick_cfg<PREC>(ctx, m, n)) {
             case 1:

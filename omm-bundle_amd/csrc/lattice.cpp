@@ -177,8 +177,10 @@ int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *
 int omx_lattice_theta0(uint64_t seed, double *theta) {
     if (!theta) return OMX_ERR_ARG;
     for (int k = 0; k < 16; ++k) theta[k] = uab(seed, OMX_STREAM_THETA, (uint64_t)k, -1.0, 1.0);
-    theta[16] = -5.0;
-    for (int k = 17; k < 20; ++k) theta[k] = 1.0;
+    // two low and two high onsite levels: the half-filled system has a gap of about 3.8 (with
+    // one low level, e.g. (-5, 1, 1, 1), half filling lands inside a band)
+    theta[16] = theta[17] = -6.0;
+    theta[18] = theta[19] = 6.0;
     return 0;
 }
 

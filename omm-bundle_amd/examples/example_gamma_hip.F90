@@ -3,7 +3,9 @@
 ! (libOMM/examples/example_gamma.F90:134-245, flavour 0) with H and S moved to HBM as 'gdcsc'       !
 ! operators and all dense matrices allocated with the 'gdden' label, operations 'hip'.            !
 ! Two SCF steps (the second with H_ii -= 0.001 and init_C), then the energy-weighted density      !
-! matrix.  Checks e_min, D_11 and ED_11 against the reference's constants (tolerance 1e-4).       !
+! matrix.  Checks e_min, D_11 and ED_11 against the reference's constants (tolerance 1e-4; with    !
+! 'hmx' the density elements 3e-4: the low-precision phase ends on the energy, which is quadratic  !
+! in the error of C, while D is linear in it).                                                     !
 ! Usage: example_gamma_hip <path to example_gamma.dat> [operation label, default 'hip']            !
 !==================================================================================================!
 program example_gamma_hip
@@ -18,7 +20,7 @@ program example_gamma_hip
   logical :: new_S, init_C
   integer :: m, n, num_elements, i, j
   integer, allocatable :: idx(:,:)
-  real(dp) :: e_min, el
+  real(dp) :: e_min, el, d_tol
   real(dp), allocatable :: vals(:,:)
   type(matrix) :: Hh, Sh, H, S, D_min, ED_min, C_min, T
 
@@ -29,6 +31,8 @@ program example_gamma_hip
     m_operation = arg(1:3)
   end if
   m_storage = 'gdden'
+  d_tol = 1.0e-4_dp
+  if (m_operation == 'hmx') d_tol = 3.0e-4_dp
 
   open(10, file=trim(path))
   read(10, '(i2,1x,i2,1x,i4)') m, n, num_elements
@@ -65,17 +69,17 @@ program example_gamma_hip
     call omm(m, n, H, S, new_S, e_min, D_min, .false., 0.0_dp, C_min, init_C, T, 0.0_dp, 0, 1, 1, -1.0_dp, &
              .true., .false., m_storage, m_operation)
     print '(a,i1,a,f21.15)', 'e_min [', i, '] : ', e_min
-    call check(e_min, e_min_check(i))
+    call check(e_min, e_min_check(i), 1.0e-4_dp)
     call m_get_element(D_min, 1, 1, el)
     print '(a,i1,a,f21.15)', 'D_11 [', i, ']  : ', el
-    call check(el, D_el_check(i))
+    call check(el, D_el_check(i), d_tol)
   end do
 
   call omm(m, n, H, S, .false., e_min, ED_min, .true., 0.0_dp, C_min, .true., T, 0.0_dp, 0, 1, 1, -1.0_dp, &
            .true., .true., m_storage, m_operation)
   call m_get_element(ED_min, 1, 1, el)
   print '(a,f21.15)', 'ED_11     : ', el
-  call check(el, ED_el_check)
+  call check(el, ED_el_check, d_tol)
 
   call m_deallocate(C_min)
   call m_deallocate(ED_min)
@@ -88,9 +92,9 @@ program example_gamma_hip
 
 contains
 
-  subroutine check(a, b)
-    real(dp), intent(in) :: a, b
-    if (abs(a - b) > 1.0e-4_dp) then
+  subroutine check(a, b, tol)
+    real(dp), intent(in) :: a, b, tol
+    if (abs(a - b) > tol) then
       print '(a,2f21.15)', 'CHECK FAILED: ', a, b
       stop 1
     end if

@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 
 # mid-gap eta = (lambda_m + lambda_{m+1})/2 of H(theta0) x = lambda S x for the default seed
 # (scipy generalised eigensolver on the host), and the gap
-ETA_THETA0 = {3: (0.9778812205428649, 0.0017521087018307346), 4: (0.5345100577348022, 0.0007735727530415737),
-              12: (0.6202814590815537, 0.0003127111545859762)}
+ETA_THETA0 = {3: (0.5273898630346915, 4.144775489523899), 4: (0.6716930386128855, 3.845393787313842),
+              12: (0.6723397363468351, 3.8575065331396274)}
 
 
 def _note(*a):
