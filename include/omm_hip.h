@@ -112,6 +112,13 @@ int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, 
  */
 int omx_bsr_from_csc(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
                      int rowbase, omx_bsr *out);
+/* Complex (k-point) operators: 32 doubles per block, (re, im) interleaved, column-major inside
+ * the block; CSC values (re, im) interleaved per entry.  Same conventions as the real calls
+ * (szcsc / pzcsc storage, MatrixSwitch_m_copy.F90, pspMat.F90). */
+int omx_bsr_create_z(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval,
+                     omx_bsr *out);
+int omx_bsr_from_csc_z(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
+                       int rowbase, omx_bsr *out);
 int omx_bsr_destroy(omx_bsr a);
 int omx_bsr_info(omx_bsr a, int *nb, int64_t *nnzb, int *is_complex);
 /* Replace the block values (same pattern), e.g. after a parameter change. */
@@ -157,6 +164,8 @@ int omx_bsr_param_grad(omx_ctx ctx, omx_bsr Dp, int nparam, const int64_t *seg, 
  * MatrixSwitch/src/MatrixSwitch.F90:1046-1390; 'ref' semantics of
  * MatrixSwitch_mm_multiply.F90:19-79).  op = 'n', 't' or 'c' ('c' = 't' for real data).
  * A, B and C share the element type dt (OMX_F64 / OMX_F32); prec selects the MFMA path.
+ * Complex dt (OMX_C128 on FP64 MFMA, OMX_C64 on FP32 MFMA; prec ignored): 'c' is the conjugate
+ * transpose (mm_zmultiply, MatrixSwitch.F90:1395-1726).  alpha, beta are real.
  * beta == 0 overwrites C (BLAS convention).
  */
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
@@ -179,7 +188,8 @@ int omx_scale(omx_ctx ctx, int dt, int m, int n, double beta, void *C, int64_t l
 /* C_ij := alpha (i /= j), beta (i == j) on 'a'll, 'u'pper or 'l'ower part
  * (m_set, MatrixSwitch.F90:2747-2827; MatrixSwitch_m_set.F90:19-75). */
 int omx_set(omx_ctx ctx, int dt, char seC, int m, int n, double alpha, double beta, void *C, int64_t ldc);
-/* alpha := tr(A), FP64 accumulation (m_trace, MatrixSwitch.F90:2060-2156). Synchronises. */
+/* alpha := tr(A), FP64 accumulation (m_trace, MatrixSwitch.F90:2060-2156; complex: out = (re, im),
+ * :2161-2248). Synchronises. */
 int omx_trace(omx_ctx ctx, int dt, int n, const void *A, int64_t lda, double *out);
 /* alpha := sum_ij A_ij*B_ij = tr(A^T B), FP64 accumulation (mm_trace, MatrixSwitch.F90:2253-2377;
  * complex: sum conj(A_ij)*B_ij, :2471, out = (re, im)). Synchronises. */
@@ -187,8 +197,12 @@ int omx_mm_trace(omx_ctx ctx, int dt, int m, int n, const void *A, int64_t lda, 
                  double *out);
 /* C_ij := alpha + beta*C_ij for one element, 1-based (m_set_element, MatrixSwitch.F90:2917-3086). */
 int omx_set_element(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha, double beta);
-/* alpha := C_ij, 1-based (m_get_element, MatrixSwitch.F90:3309-3417). Synchronises. */
+/* alpha := C_ij, 1-based (m_get_element, MatrixSwitch.F90:3309-3417); complex types write
+ * (re, im) to out[0..1]. Synchronises. */
 int omx_get_element(omx_ctx ctx, int dt, const void *C, int64_t ldc, int i, int j, double *out);
+/* C_ij := alpha + beta*C_ij with complex alpha, 1-based (m_set_element on complex matrices). */
+int omx_set_element_z(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha_re, double alpha_im,
+                      double beta);
 
 /* ==== fused libOMM kernels ================================================================== */
 
@@ -218,6 +232,27 @@ int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const
 /* W := W + x*(Wd^T + Wd) + x^2*Wdd in one pass (omm.F90:496-502 and 511-517). */
 int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const void *Wd, const void *Wdd, void *W);
 
+/* ==== dense factorisations (Cholesky and preconditioned flavours) ========================= */
+/* All FP64 column-major; U is upper triangular and only its upper triangle is read. */
+
+/* A = U^T U in place, upper (m_factorize, libOMM/src/omm_ops.F90:246-315, dpotrf 'u').  *info > 0
+ * names the first non-positive pivot (1-based).  The strictly lower triangle is used as scratch.
+ * Synchronises. */
+int omx_potrf(omx_ctx ctx, int dt, int n, void *A, int64_t lda, int *info);
+/* B := alpha*op(U)^-1*B (side 'l') or alpha*B*op(U)^-1 (side 'r'), op 'n' or 't' (dtrsm; used by
+ * m_back_transform, omm_ops.F90:829-897). */
+int omx_trsm(omx_ctx ctx, int dt, char side, char trans, int m, int n, double alpha, const void *U, int64_t ldu,
+             void *B, int64_t ldb);
+/* B := alpha*op(U)*B (side 'l') or alpha*B*op(U) (side 'r') (dtrmm; m_transform, omm_ops.F90:662-730). */
+int omx_trmm(omx_ctx ctx, int dt, char side, char trans, int m, int n, double alpha, const void *U, int64_t ldu,
+             void *B, int64_t ldb);
+/* H := U^-T H U^-1, symmetric result in both triangles (m_reduce: dsygst itype 1 'u' and the
+ * mirror copy, omm_ops.F90:417-524). */
+int omx_sygst(omx_ctx ctx, int dt, int n, void *H, int64_t ldh, const void *U, int64_t ldu);
+/* A := A^-1 for symmetric positive definite A (m_inverse of the preconditioner S + T/scale_T,
+ * omm_ops.F90:992-1106; the reference factors with dsytrf, here Cholesky).  Synchronises. */
+int omx_inverse_spd(omx_ctx ctx, int dt, int n, void *A, int64_t lda, int *info);
+
 /* ==== event timing ========================================================================= */
 
 /* Start (enable=1) or stop accumulating HIP-event durations of launches of each kernel class. */
@@ -241,6 +276,13 @@ int omx_prof_work(omx_ctx ctx, int kclass, double *flops, double *bytes);
  */
 int omx_lattice_bsr(int Lx, int Ly, int Lz, uint64_t seed, int which, int64_t *nnzb, int *bcolptr, int *browidx,
                     double *bval);
+/* Bloch operators H(k) (which = 0) / S(k) (which = 1) of the periodic cell: the blocks of
+ * omx_lattice_bsr, with the bonds that cross the cell boundary along d multiplied by
+ * exp(2 pi i phi[d]) (forward) and exp(-2 pi i phi[d]) (backward), i.e. sum_R H_R exp(i k.R) with
+ * k.R = 2 pi phi[d] per cell translation (a uniform N-point grid along d: phi = n/N).  Hermitian.
+ * Values: 32 doubles per block, (re, im) interleaved.  Two-call protocol as omx_lattice_bsr. */
+int omx_lattice_bsr_k(int Lx, int Ly, int Lz, uint64_t seed, int which, const double *phi, int64_t *nnzb,
+                      int *bcolptr, int *browidx, double *bval);
 /* H(theta): one hopping block theta[0..15] (column-major) on every bond, onsite diag(theta[16..19]). */
 int omx_lattice_bsr_theta(int Lx, int Ly, int Lz, const double *theta, int64_t *nnzb, int *bcolptr, int *browidx,
                           double *bval);

@@ -163,6 +163,17 @@ template <typename T>
 __global__ void k_get_element(const T *C, int64_t off, double *out) {
     out[0] = (double)C[off];
 }
+// Complex element (re, im) at element offset off: C := (alpha_re + beta*re, alpha_im + beta*im).
+template <typename R>
+__global__ void k_set_element_z(R *C, int64_t off, double are, double aim, double beta) {
+    C[2 * off] = (R)(are + beta * (double)C[2 * off]);
+    C[2 * off + 1] = (R)(aim + beta * (double)C[2 * off + 1]);
+}
+template <typename R>
+__global__ void k_get_element_z(const R *C, int64_t off, double *out) {
+    out[0] = (double)C[2 * off];
+    out[1] = (double)C[2 * off + 1];
+}
 
 }  // namespace omx
 
@@ -343,6 +354,64 @@ static int bsr_upload(omx_ctx ctx, int nb, const std::vector<int> &colptr, const
     return OMX_OK;
 }
 
+static int bsr_create_common(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval,
+                             int is_complex, omx_bsr *out) {
+    OMX_REQUIRE(ctx && out && bcolptr && nb > 0, "omx_bsr_create: bad arguments");
+    int nbc = (nb + 3) / 4;
+    OMX_REQUIRE(bcolptr[0] == 0, "omx_bsr_create: bcolptr must be 0-based");
+    for (int j = 0; j < nbc; ++j)
+        OMX_REQUIRE(bcolptr[j + 1] >= bcolptr[j], "omx_bsr_create: bcolptr not monotone at %d", j);
+    int64_t nnzb = bcolptr[nbc];
+    OMX_REQUIRE(nnzb == 0 || (browidx && bval), "omx_bsr_create: null block arrays");
+    for (int j = 0; j < nbc; ++j)
+        for (int l = bcolptr[j]; l < bcolptr[j + 1]; ++l) {
+            OMX_REQUIRE(browidx[l] >= 0 && browidx[l] < nbc, "omx_bsr_create: block row %d out of range", browidx[l]);
+            OMX_REQUIRE(l == bcolptr[j] || browidx[l] > browidx[l - 1],
+                        "omx_bsr_create: block rows of column %d not strictly ascending", j);
+        }
+    std::vector<int> cp(bcolptr, bcolptr + nbc + 1), ri(browidx, browidx + nnzb);
+    return bsr_upload(ctx, nb, cp, ri, bval, is_complex, out);
+}
+
+int omx_bsr_create_z(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval,
+                     omx_bsr *out) {
+    return bsr_create_common(ctx, nb, bcolptr, browidx, bval, 1, out);
+}
+
+int omx_bsr_from_csc_z(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
+                       int rowbase, omx_bsr *out) {
+    OMX_REQUIRE(ctx && out && colptr && nb > 0, "omx_bsr_from_csc_z: bad arguments");
+    OMX_REQUIRE((ptrbase == 0 || ptrbase == 1) && (rowbase == 0 || rowbase == 1), "omx_bsr_from_csc_z: bad base");
+    int nbc = (nb + 3) / 4;
+    int64_t nnz = (int64_t)colptr[nb] - ptrbase;
+    OMX_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val)), "omx_bsr_from_csc_z: bad nnz");
+    std::vector<int> cp(nbc + 1, 0), ri;
+    std::vector<double> bv;
+    for (int J = 0; J < nbc; ++J) {
+        std::map<int, std::vector<std::pair<int, int64_t>>> blocks;  // I -> (r + 4c, entry)
+        for (int c = 0; c < 4; ++c) {
+            int j = 4 * J + c;
+            if (j >= nb) break;
+            for (int64_t l = colptr[j] - ptrbase; l < colptr[j + 1] - ptrbase; ++l) {
+                int i = rowidx[l] - rowbase;
+                OMX_REQUIRE(i >= 0 && i < nb, "omx_bsr_from_csc_z: row index %d out of range", i + rowbase);
+                blocks[i / 4].push_back({(i % 4) + 4 * c, l});
+            }
+        }
+        for (auto &kv : blocks) {
+            ri.push_back(kv.first);
+            size_t base = bv.size();
+            bv.resize(base + 32, 0.0);
+            for (auto &e : kv.second) {
+                bv[base + 2 * e.first] += val[2 * e.second];
+                bv[base + 2 * e.first + 1] += val[2 * e.second + 1];
+            }
+        }
+        cp[J + 1] = (int)ri.size();
+    }
+    return bsr_upload(ctx, nb, cp, ri, bv.data(), 1, out);
+}
+
 int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, const double *bval, omx_bsr *out) {
     OMX_REQUIRE(ctx && out && bcolptr && nb > 0, "omx_bsr_create: bad arguments");
     int nbc = (nb + 3) / 4;
@@ -449,6 +518,10 @@ int omx_set_element(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, dou
         hipLaunchKernelGGL(k_set_element<double>, dim3(1), dim3(1), 0, ctx->stream, (double *)C, off, alpha, beta);
     else if (dt == OMX_F32)
         hipLaunchKernelGGL(k_set_element<float>, dim3(1), dim3(1), 0, ctx->stream, (float *)C, off, alpha, beta);
+    else if (dt == OMX_C128)
+        hipLaunchKernelGGL(k_set_element_z<double>, dim3(1), dim3(1), 0, ctx->stream, (double *)C, off, alpha, 0.0, beta);
+    else if (dt == OMX_C64)
+        hipLaunchKernelGGL(k_set_element_z<float>, dim3(1), dim3(1), 0, ctx->stream, (float *)C, off, alpha, 0.0, beta);
     else {
         set_error("omx_set_element: unsupported type %d", dt);
         return OMX_ERR_UNSUPPORTED;
@@ -466,14 +539,40 @@ int omx_get_element(omx_ctx ctx, int dt, const void *C, int64_t ldc, int i, int 
     else if (dt == OMX_F32)
         hipLaunchKernelGGL(k_get_element<float>, dim3(1), dim3(1), 0, ctx->stream, (const float *)C, off,
                            ctx->d_scalars);
+    else if (dt == OMX_C128)
+        hipLaunchKernelGGL(k_get_element_z<double>, dim3(1), dim3(1), 0, ctx->stream, (const double *)C, off,
+                           ctx->d_scalars);
+    else if (dt == OMX_C64)
+        hipLaunchKernelGGL(k_get_element_z<float>, dim3(1), dim3(1), 0, ctx->stream, (const float *)C, off,
+                           ctx->d_scalars);
     else {
         set_error("omx_get_element: unsupported type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
     OMX_CHECK_HIP(hipGetLastError());
-    int rc = read_scalars(ctx, 1);
+    const bool z = (dt == OMX_C128 || dt == OMX_C64);
+    int rc = read_scalars(ctx, z ? 2 : 1);
     if (rc) return rc;
-    *out = ctx->h_scalars[0];
+    out[0] = ctx->h_scalars[0];
+    if (z) out[1] = ctx->h_scalars[1];
+    return OMX_OK;
+}
+
+int omx_set_element_z(omx_ctx ctx, int dt, void *C, int64_t ldc, int i, int j, double alpha_re, double alpha_im,
+                      double beta) {
+    OMX_REQUIRE(ctx && C && i >= 1 && j >= 1 && ldc >= i, "omx_set_element_z: bad arguments");
+    int64_t off = (int64_t)(i - 1) + (int64_t)(j - 1) * ldc;
+    if (dt == OMX_C128)
+        hipLaunchKernelGGL(k_set_element_z<double>, dim3(1), dim3(1), 0, ctx->stream, (double *)C, off, alpha_re,
+                           alpha_im, beta);
+    else if (dt == OMX_C64)
+        hipLaunchKernelGGL(k_set_element_z<float>, dim3(1), dim3(1), 0, ctx->stream, (float *)C, off, alpha_re,
+                           alpha_im, beta);
+    else {
+        set_error("omx_set_element_z: complex element type expected, got %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
 

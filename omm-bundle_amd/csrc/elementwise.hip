@@ -51,48 +51,94 @@ __global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const do
     block_reduce_store<NV>(v, out);
 }
 
+// ---- element types -----------------------------------------------------------------------------
+// Real (double, float) and complex (Cx<double>, Cx<float>: the Fortran (re, im) layout).  Every
+// kernel below is written once over El<T>; for real types the imaginary terms are compiled out,
+// so the real arithmetic (and its rounding) is exactly the real-only one.
+
+template <typename R_>
+struct Cx {
+    R_ x, y;
+};
+template <typename T>
+struct El {
+    static constexpr bool cplx = false;
+    __device__ static inline double re(const T &v) { return (double)v; }
+    __device__ static inline double im(const T &) { return 0.0; }
+    __device__ static inline T make(double r, double) { return (T)r; }
+};
+template <typename R_>
+struct El<Cx<R_>> {
+    static constexpr bool cplx = true;
+    __device__ static inline double re(const Cx<R_> &v) { return (double)v.x; }
+    __device__ static inline double im(const Cx<R_> &v) { return (double)v.y; }
+    __device__ static inline Cx<R_> make(double r, double i) { return Cx<R_>{(R_)r, (R_)i}; }
+};
+
 // ---- m_add / m_scale / m_set -----------------------------------------------------------------
 
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_add_n(int m, int n, double alpha, const T *__restrict__ A, int64_t lda,
                                                  double beta, T *__restrict__ C, int64_t ldc) {
+    typedef El<T> E;
     int64_t total = (int64_t)m * n;
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m;
-        double a = alpha * (double)A[i + j * lda];
-        C[i + j * ldc] = (T)(beta == 0.0 ? a : beta * (double)C[i + j * ldc] + a);
+        const T a = A[i + j * lda];
+        T *p = C + i + j * ldc;
+        double ar = alpha * E::re(a), ai = 0.0;
+        if constexpr (E::cplx) ai = alpha * E::im(a);
+        if (beta == 0.0) {
+            *p = E::make(ar, ai);
+        } else {
+            const T c = *p;
+            double cr = beta * E::re(c) + ar, ci = 0.0;
+            if constexpr (E::cplx) ci = beta * E::im(c) + ai;
+            *p = E::make(cr, ci);
+        }
     }
 }
 
-// C(i,j) := beta*C(i,j) + alpha*A(j,i): 32x32 tiles through LDS.
+// C(i,j) := beta*C(i,j) + alpha*op(A)(i,j), op(A)(i,j) = A(j,i) or conj(A(j,i)): 32x32 tiles
+// through LDS.
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_add_t(int m, int n, double alpha, const T *__restrict__ A, int64_t lda,
-                                                 double beta, T *__restrict__ C, int64_t ldc) {
+                                                 double beta, T *__restrict__ C, int64_t ldc, int conj) {
+    typedef El<T> E;
     __shared__ T tile[TT][TT + 1];
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;  // 32 x 8
     const int ti = blockIdx.x * TT, tj = blockIdx.y * TT;     // C tile origin
     // A is n x m; read A(tj + r, ti + c) for the C tile (ti.., tj..).
     for (int c = ty; c < TT; c += EW_NT / TT) {
         int ar = tj + tx, ac = ti + c;
-        tile[c][tx] = (ar < n && ac < m) ? A[(int64_t)ar + (int64_t)ac * lda] : (T)0;
+        tile[c][tx] = (ar < n && ac < m) ? A[(int64_t)ar + (int64_t)ac * lda] : E::make(0.0, 0.0);
     }
     __syncthreads();
     for (int c = ty; c < TT; c += EW_NT / TT) {
         int i = ti + tx, j = tj + c;
         if (i < m && j < n) {
-            double a = alpha * (double)tile[tx][c];
+            const T a = tile[tx][c];
+            double vr = alpha * E::re(a), vi = 0.0;
+            if constexpr (E::cplx) vi = alpha * (conj ? -E::im(a) : E::im(a));
             T *p = C + (int64_t)i + (int64_t)j * ldc;
-            *p = (T)(beta == 0.0 ? a : beta * (double)*p + a);
+            if (beta != 0.0) {
+                const T o = *p;
+                vr = beta * E::re(o) + vr;
+                if constexpr (E::cplx) vi = beta * E::im(o) + vi;
+            }
+            *p = E::make(vr, vi);
         }
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_scale(int m, int n, double beta, T *__restrict__ C, int64_t ldc) {
+    typedef El<T> E;
     int64_t total = (int64_t)m * n;
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m;
-        C[i + j * ldc] = (T)(beta * (double)C[i + j * ldc]);
+        const T c = C[i + j * ldc];
+        C[i + j * ldc] = E::make(beta * E::re(c), beta * E::im(c));
     }
 }
 
@@ -104,60 +150,91 @@ __global__ __launch_bounds__(EW_NT) void k_set(int lu, int m, int n, double alph
         int64_t i = q % m, j = q / m;
         if (lu == 1 && j < i) continue;
         if (lu == 2 && j > i) continue;
-        C[i + j * ldc] = (T)(i != j ? alpha : beta);
+        C[i + j * ldc] = El<T>::make(i != j ? alpha : beta, 0.0);
     }
 }
 
 // ---- traces ----------------------------------------------------------------------------------
 
+// (re, im) of tr(A)
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_trace_partial(int n, const T *__restrict__ A, int64_t lda,
                                                          double *__restrict__ partial) {
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * EW_NT + threadIdx.x; i < n; i += gridDim.x * EW_NT) v[0] += ld(A + (int64_t)i * (lda + 1));
-    block_reduce_store<1>(v, partial + blockIdx.x);
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * EW_NT + threadIdx.x; i < n; i += gridDim.x * EW_NT) {
+        const T a = A[(int64_t)i * (lda + 1)];
+        v[0] += El<T>::re(a);
+        if constexpr (El<T>::cplx) v[1] += El<T>::im(a);
+    }
+    block_reduce_store<2>(v, partial + 2 * (int64_t)blockIdx.x);
 }
 
+// (re, im) of sum_ij conj(A_ij)*B_ij
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_dot_partial(int m, int n, const T *__restrict__ A, int64_t lda,
                                                        const T *__restrict__ B, int64_t ldb,
                                                        double *__restrict__ partial) {
-    double v[1] = {0.0};
+    typedef El<T> E;
+    double v[2] = {0.0, 0.0};
     int64_t total = (int64_t)m * n;
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m;
-        v[0] += (double)A[i + j * lda] * (double)B[i + j * ldb];
+        const T a = A[i + j * lda], b = B[i + j * ldb];
+        if constexpr (E::cplx) {
+            v[0] += E::re(a) * E::re(b) + E::im(a) * E::im(b);
+            v[1] += E::re(a) * E::im(b) - E::im(a) * E::re(b);
+        } else {
+            v[0] += E::re(a) * E::re(b);
+        }
     }
-    block_reduce_store<1>(v, partial + blockIdx.x);
+    block_reduce_store<2>(v, partial + 2 * (int64_t)blockIdx.x);
 }
 
-// <G, G - G_p> and <G_p, G_p> (Polak-Ribiere numerator and denominator).
+// Re <G, G - G_p> and <G_p, G_p> (Polak-Ribiere numerator and denominator; G - G_p rounded to
+// the storage type first, as the reference's work2 copy).
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_pr_partial(int64_t total_m, int m, const T *__restrict__ G,
                                                       const T *__restrict__ Gp, int64_t ld,
                                                       double *__restrict__ partial) {
+    typedef El<T> E;
     double v[2] = {0.0, 0.0};
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total_m; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m;
-        double g = (double)G[i + j * ld], p = (double)Gp[i + j * ld];
-        double w = (double)(T)(g - p);
-        v[0] += g * w;
+        const T gt = G[i + j * ld], pt = Gp[i + j * ld];
+        const double g = E::re(gt), p = E::re(pt);
+        const T wt = E::make(g - p, E::im(gt) - E::im(pt));
+        v[0] += g * E::re(wt);
         v[1] += p * p;
+        if constexpr (E::cplx) {
+            const double gi = E::im(gt), pi = E::im(pt);
+            v[0] += gi * E::im(wt);
+            v[1] += pi * pi;
+        }
     }
     block_reduce_store<2>(v, partial + 2 * (int64_t)blockIdx.x);
 }
 
 // ---- fused line-search coefficients (calc_coeff, libOMM/src/omm_ops.F90:156-219) --------------
-// Thirteen sums over n x n matrices; tile (I,J) also reads the SWd tile (J,I) for the
-// transposed term TrHdSd = sum_ij SWd(j,i)*HWd(i,j).
+// Thirteen sums over n x n matrices (real parts; mm_trace(A,B) = Re sum conj(A_ij) B_ij); tile
+// (I,J) also reads the SWd tile (J,I) for the transposed term TrHdSd = Re sum_ij SWd(j,i)*HWd(i,j).
 // Partial layout per block: [TrH, TrHS, TrHd, TrHdS, TrHSd, TrHdd, TrHddS, TrHSdd, TrHdSd,
 //                            TrHdSdH, TrHddSd, TrHdSdd, TrHddSdd]
+template <typename T>
+__device__ inline double cdot(const T &a, const T &b) {
+    if constexpr (El<T>::cplx)
+        return El<T>::re(a) * El<T>::re(b) + El<T>::im(a) * El<T>::im(b);
+    else
+        return El<T>::re(a) * El<T>::re(b);
+}
+
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, const T *__restrict__ HW,
                                                          const T *__restrict__ SW, const T *__restrict__ HWd,
                                                          const T *__restrict__ SWd, const T *__restrict__ HWdd,
                                                          const T *__restrict__ SWdd, double *__restrict__ partial) {
+    typedef El<T> E;
     __shared__ double swdT[TT][TT + 1];
+    __shared__ double swdTi[El<T>::cplx ? TT : 1][TT + 1];
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
     const int tiles = (n + TT - 1) / TT;
     const int I = blockIdx.x % tiles, J = blockIdx.x / tiles;
@@ -165,7 +242,10 @@ __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, con
     // SWd tile (J,I): element (j0 + a, i0 + b) -> swdT[a][b]
     for (int b = ty; b < TT; b += EW_NT / TT) {
         int r = j0 + tx, c = i0 + b;
-        swdT[tx][b] = (r < n && c < n) ? (double)SWd[(int64_t)r + (int64_t)c * ldm] : 0.0;
+        const bool in = (r < n && c < n);
+        const T w = in ? SWd[(int64_t)r + (int64_t)c * ldm] : E::make(0.0, 0.0);
+        swdT[tx][b] = E::re(w);
+        if constexpr (E::cplx) swdTi[tx][b] = E::im(w);
     }
     __syncthreads();
     double v[13];
@@ -175,22 +255,25 @@ __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, con
         const int i = i0 + tx, j = j0 + c;
         if (i >= n || j >= n) continue;
         const int64_t o = (int64_t)i + (int64_t)j * ldm;
-        const double hw = HW[o], sw = SW[o], hwd = HWd[o], swd = SWd[o], hwdd = HWdd[o], swdd = SWdd[o];
+        const T hw = HW[o], sw = SW[o], hwd = HWd[o], swd = SWd[o], hwdd = HWdd[o], swdd = SWdd[o];
         if (i == j) {
-            v[0] += hw;
-            v[2] += hwd;
-            v[5] += hwdd;
+            v[0] += E::re(hw);
+            v[2] += E::re(hwd);
+            v[5] += E::re(hwdd);
         }
-        v[1] += hw * sw;
-        v[3] += sw * hwd;
-        v[4] += hw * swd;
-        v[6] += hwdd * sw;
-        v[7] += hw * swdd;
-        v[8] += swdT[c][tx] * hwd;  // SWd(j,i) * HWd(i,j)
-        v[9] += swd * hwd;
-        v[10] += hwdd * swd;
-        v[11] += swdd * hwd;
-        v[12] += hwdd * swdd;
+        v[1] += cdot(hw, sw);
+        v[3] += cdot(sw, hwd);
+        v[4] += cdot(hw, swd);
+        v[6] += cdot(hwdd, sw);
+        v[7] += cdot(hw, swdd);
+        if constexpr (E::cplx)
+            v[8] += swdT[c][tx] * E::re(hwd) - swdTi[c][tx] * E::im(hwd);  // Re SWd(j,i) * HWd(i,j)
+        else
+            v[8] += swdT[c][tx] * E::re(hwd);
+        v[9] += cdot(swd, hwd);
+        v[10] += cdot(hwdd, swd);
+        v[11] += cdot(swdd, hwd);
+        v[12] += cdot(hwdd, swdd);
     }
     block_reduce_store<13>(v, partial + 13 * (int64_t)blockIdx.x);
 }
@@ -225,26 +308,41 @@ __global__ void k_coeff_final(int nblocks, const double *__restrict__ partial, d
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_cg_direction(int m, int n, double lambda, const T *__restrict__ G,
                                                         T *__restrict__ D, T *__restrict__ Gp, int64_t ld) {
+    typedef El<T> E;
     int64_t total = (int64_t)m * n;
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m, o = i + j * ld;
-        double g = (double)G[o];
-        double d = (lambda == 0.0) ? g : lambda * (double)D[o] + g;
-        D[o] = (T)d;
-        Gp[o] = (T)g;
+        const T gt = G[o];
+        const double g = E::re(gt), gi = E::im(gt);
+        double d, di = 0.0;
+        if (lambda == 0.0) {
+            d = g;
+            di = gi;
+        } else {
+            const T dt = D[o];
+            d = lambda * E::re(dt) + g;
+            if constexpr (E::cplx) di = lambda * E::im(dt) + gi;
+        }
+        D[o] = E::make(d, di);
+        Gp[o] = gt;
     }
 }
 
-// W := W + x*Wd^T + x*Wd + x^2*Wdd, accumulated in that order (omm.F90:499-501, 514-516).
+// W := W + x*Wd^H + x*Wd + x^2*Wdd, accumulated in that order (omm.F90:499-501, 514-516).
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_update_w(int n, int64_t ldm, double x, const T *__restrict__ Wd,
                                                     const T *__restrict__ Wdd, T *__restrict__ W) {
+    typedef El<T> E;
     __shared__ double tileT[TT][TT + 1];
+    __shared__ double tileTi[El<T>::cplx ? TT : 1][TT + 1];
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
     const int i0 = blockIdx.x * TT, j0 = blockIdx.y * TT;
     for (int b = ty; b < TT; b += EW_NT / TT) {
         int r = j0 + tx, c = i0 + b;  // Wd(j0+tx, i0+b)
-        tileT[tx][b] = (r < n && c < n) ? (double)Wd[(int64_t)r + (int64_t)c * ldm] : 0.0;
+        const bool in = (r < n && c < n);
+        const T w = in ? Wd[(int64_t)r + (int64_t)c * ldm] : E::make(0.0, 0.0);
+        tileT[tx][b] = E::re(w);
+        if constexpr (E::cplx) tileTi[tx][b] = -E::im(w);  // conjugate transpose
     }
     __syncthreads();
     const double x2 = x * x;
@@ -252,11 +350,31 @@ __global__ __launch_bounds__(EW_NT) void k_update_w(int n, int64_t ldm, double x
         int i = i0 + tx, j = j0 + c;
         if (i >= n || j >= n) continue;
         int64_t o = (int64_t)i + (int64_t)j * ldm;
-        double w = (double)W[o];
+        const T wt = W[o], wdt = Wd[o], wddt = Wdd[o];
+        double w = E::re(wt);
         w = w + x * tileT[c][tx];
-        w = w + x * (double)Wd[o];
-        w = w + x2 * (double)Wdd[o];
-        W[o] = (T)w;
+        w = w + x * E::re(wdt);
+        w = w + x2 * E::re(wddt);
+        double wi = 0.0;
+        if constexpr (E::cplx) {
+            wi = E::im(wt);
+            wi = wi + x * tileTi[c][tx];
+            wi = wi + x * E::im(wdt);
+            wi = wi + x2 * E::im(wddt);
+        }
+        W[o] = E::make(w, wi);
+    }
+}
+
+// Calls f(T{}) with T the element type of dt; false for an unknown dt.
+template <typename F>
+inline bool with_type(int dt, F &&f) {
+    switch (dt) {
+        case OMX_F64: f(double{}); return true;
+        case OMX_F32: f(float{}); return true;
+        case OMX_C128: f(Cx<double>{}); return true;
+        case OMX_C64: f(Cx<float>{}); return true;
+        default: return false;
     }
 }
 
@@ -280,38 +398,29 @@ int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const voi
             void *C, int64_t ldc) {
     OMX_REQUIRE(ctx && valid_op(opA) && m >= 0 && n >= 0, "omx_add: bad arguments");
     bool tr = is_trans(opA);
+    const int conj = (opA == 'c' || opA == 'C');
     OMX_REQUIRE(ldc >= std::max(1, m) && lda >= std::max(1, tr ? n : m), "omx_add: bad leading dimension");
     if (!m || !n) return OMX_OK;
     OMX_REQUIRE(A && C, "omx_add: null operand");
     OMX_REQUIRE(!(tr && A == C), "omx_add: in-place transpose is not supported");
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_ELEM, &ev);
-    if (!tr) {
-        int grid = grid_for((int64_t)m * n);
-        if (dt == OMX_F64)
-            hipLaunchKernelGGL(k_add_n<double>, dim3(grid), dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const double *)A,
-                               lda, beta, (double *)C, ldc);
-        else if (dt == OMX_F32)
-            hipLaunchKernelGGL(k_add_n<float>, dim3(grid), dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const float *)A,
-                               lda, beta, (float *)C, ldc);
-        else {
-            set_error("omx_add: unsupported element type %d", dt);
-            return OMX_ERR_UNSUPPORTED;
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        if (!tr) {
+            hipLaunchKernelGGL(k_add_n<T>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, alpha,
+                               (const T *)A, lda, beta, (T *)C, ldc);
+        } else {
+            dim3 grid((m + TT - 1) / TT, (n + TT - 1) / TT);
+            hipLaunchKernelGGL(k_add_t<T>, grid, dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const T *)A, lda, beta,
+                               (T *)C, ldc, conj);
         }
-    } else {
-        dim3 grid((m + TT - 1) / TT, (n + TT - 1) / TT);
-        if (dt == OMX_F64)
-            hipLaunchKernelGGL(k_add_t<double>, grid, dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const double *)A, lda,
-                               beta, (double *)C, ldc);
-        else if (dt == OMX_F32)
-            hipLaunchKernelGGL(k_add_t<float>, grid, dim3(EW_NT), 0, ctx->stream, m, n, alpha, (const float *)A, lda,
-                               beta, (float *)C, ldc);
-        else {
-            set_error("omx_add: unsupported element type %d", dt);
-            return OMX_ERR_UNSUPPORTED;
-        }
-    }
+    });
     prof_end(ctx, OMX_K_ELEM, ev);
+    if (!ok) {
+        set_error("omx_add: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
@@ -320,13 +429,12 @@ int omx_scale(omx_ctx ctx, int dt, int m, int n, double beta, void *C, int64_t l
     OMX_REQUIRE(ctx && m >= 0 && n >= 0 && ldc >= std::max(1, m), "omx_scale: bad arguments");
     if (!m || !n) return OMX_OK;
     OMX_REQUIRE(C, "omx_scale: null operand");
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_scale<double>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, beta,
-                           (double *)C, ldc);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_scale<float>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, beta,
-                           (float *)C, ldc);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, m, n, beta,
+                           (T *)C, ldc);
+    });
+    if (!ok) {
         set_error("omx_scale: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
@@ -339,13 +447,12 @@ int omx_set(omx_ctx ctx, int dt, char seC, int m, int n, double alpha, double be
     if (!m || !n) return OMX_OK;
     OMX_REQUIRE(C, "omx_set: null operand");
     int lu = (seC == 'l' || seC == 'L') ? 2 : ((seC == 'u' || seC == 'U') ? 1 : 0);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_set<double>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, lu, m, n, alpha,
-                           beta, (double *)C, ldc);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_set<float>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, lu, m, n, alpha,
-                           beta, (float *)C, ldc);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_set<T>, dim3(grid_for((int64_t)m * n)), dim3(EW_NT), 0, ctx->stream, lu, m, n, alpha,
+                           beta, (T *)C, ldc);
+    });
+    if (!ok) {
         set_error("omx_set: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
@@ -353,31 +460,34 @@ int omx_set(omx_ctx ctx, int dt, char seC, int m, int n, double alpha, double be
     return OMX_OK;
 }
 
+static inline bool is_complex_dt(int dt) { return dt == OMX_C128 || dt == OMX_C64; }
+
 int omx_trace(omx_ctx ctx, int dt, int n, const void *A, int64_t lda, double *out) {
     OMX_REQUIRE(ctx && out && n >= 0 && lda >= std::max(1, n), "omx_trace: bad arguments");
     if (!n) {
-        *out = 0.0;
+        out[0] = 0.0;
+        if (is_complex_dt(dt)) out[1] = 0.0;
         return OMX_OK;
     }
     OMX_REQUIRE(A, "omx_trace: null operand");
     int blocks = std::min(256, (n + EW_NT - 1) / EW_NT);
-    int rc = ensure_ws(ctx, sizeof(double) * blocks);
+    int rc = ensure_ws(ctx, sizeof(double) * 2 * blocks);
     if (rc) return rc;
     double *partial = (double *)ctx->ws;
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_trace_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, n, (const double *)A,
-                           lda, partial);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_trace_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, n, (const float *)A, lda,
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_trace_partial<T>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, n, (const T *)A, lda,
                            partial);
-    else {
+    });
+    if (!ok) {
         set_error("omx_trace: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
     OMX_CHECK_HIP(hipGetLastError());
-    rc = finish_reduce<1>(ctx, blocks, partial);
+    rc = finish_reduce<2>(ctx, blocks, partial);
     if (rc) return rc;
-    *out = ctx->h_scalars[0];
+    out[0] = ctx->h_scalars[0];
+    if (is_complex_dt(dt)) out[1] = ctx->h_scalars[1];
     return OMX_OK;
 }
 
@@ -386,31 +496,32 @@ int omx_mm_trace(omx_ctx ctx, int dt, int m, int n, const void *A, int64_t lda, 
     OMX_REQUIRE(ctx && out && m >= 0 && n >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, m),
                 "omx_mm_trace: bad arguments");
     if (!m || !n) {
-        *out = 0.0;
+        out[0] = 0.0;
+        if (is_complex_dt(dt)) out[1] = 0.0;
         return OMX_OK;
     }
     OMX_REQUIRE(A && B, "omx_mm_trace: null operand");
     int blocks = std::min(2048, grid_for((int64_t)m * n));
-    int rc = ensure_ws(ctx, sizeof(double) * blocks);
+    int rc = ensure_ws(ctx, sizeof(double) * 2 * blocks);
     if (rc) return rc;
     double *partial = (double *)ctx->ws;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_DOT, &ev);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_dot_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, m, n, (const double *)A,
-                           lda, (const double *)B, ldb, partial);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_dot_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, m, n, (const float *)A, lda,
-                           (const float *)B, ldb, partial);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_dot_partial<T>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, m, n, (const T *)A, lda,
+                           (const T *)B, ldb, partial);
+    });
+    prof_end(ctx, OMX_K_DOT, ev);
+    if (!ok) {
         set_error("omx_mm_trace: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
-    prof_end(ctx, OMX_K_DOT, ev);
     OMX_CHECK_HIP(hipGetLastError());
-    rc = finish_reduce<1>(ctx, blocks, partial);
+    rc = finish_reduce<2>(ctx, blocks, partial);
     if (rc) return rc;
-    *out = ctx->h_scalars[0];
+    out[0] = ctx->h_scalars[0];
+    if (is_complex_dt(dt)) out[1] = ctx->h_scalars[1];
     return OMX_OK;
 }
 
@@ -426,16 +537,16 @@ int omx_omm_coeff(omx_ctx ctx, int dt, int n, int64_t ldm, const void *HW, const
     double *partial = (double *)ctx->ws;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_COEFF, &ev);
-    prof_work(ctx, OMX_K_COEFF, 26.0 * n * (double)n, (dt == OMX_F64 ? 8.0 : 4.0) * 6.0 * n * (double)n);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_coeff_partial<double>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
-                           (const double *)HW, (const double *)SW, (const double *)HWd, (const double *)SWd,
-                           (const double *)HWdd, (const double *)SWdd, partial);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_coeff_partial<float>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
-                           (const float *)HW, (const float *)SW, (const float *)HWd, (const float *)SWd,
-                           (const float *)HWdd, (const float *)SWdd, partial);
-    else {
+    prof_work(ctx, OMX_K_COEFF, (is_complex_dt(dt) ? 4.0 : 1.0) * 26.0 * n * (double)n,
+              dt_size(dt) * 6.0 * n * (double)n);
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_coeff_partial<T>, dim3((unsigned)blocks), dim3(EW_NT), 0, ctx->stream, n, ldm,
+                           (const T *)HW, (const T *)SW, (const T *)HWd, (const T *)SWd, (const T *)HWdd,
+                           (const T *)SWdd, partial);
+    });
+    if (!ok) {
+        prof_end(ctx, OMX_K_COEFF, ev);
         set_error("omx_omm_coeff: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
@@ -457,17 +568,16 @@ int omx_omm_pr_dots(omx_ctx ctx, int dt, int n, int m, const void *G, const void
     double *partial = (double *)ctx->ws;
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_DOT, &ev);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_pr_partial<double>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n,
-                           (const double *)G, (const double *)G_p, ld, partial);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_pr_partial<float>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n, (const float *)G,
-                           (const float *)G_p, ld, partial);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_pr_partial<T>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n, (const T *)G,
+                           (const T *)G_p, ld, partial);
+    });
+    prof_end(ctx, OMX_K_DOT, ev);
+    if (!ok) {
         set_error("omx_omm_pr_dots: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
-    prof_end(ctx, OMX_K_DOT, ev);
     OMX_CHECK_HIP(hipGetLastError());
     rc = finish_reduce<2>(ctx, blocks, partial);
     if (rc) return rc;
@@ -482,17 +592,16 @@ int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const
     int grid = grid_for((int64_t)n * m);
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_ELEM, &ev);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_cg_direction<double>, dim3(grid), dim3(EW_NT), 0, ctx->stream, n, m, lambda,
-                           (const double *)G, (double *)D, (double *)G_p, ld);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_cg_direction<float>, dim3(grid), dim3(EW_NT), 0, ctx->stream, n, m, lambda,
-                           (const float *)G, (float *)D, (float *)G_p, ld);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_cg_direction<T>, dim3(grid), dim3(EW_NT), 0, ctx->stream, n, m, lambda, (const T *)G,
+                           (T *)D, (T *)G_p, ld);
+    });
+    prof_end(ctx, OMX_K_ELEM, ev);
+    if (!ok) {
         set_error("omx_omm_cg_direction: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
-    prof_end(ctx, OMX_K_ELEM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
@@ -503,17 +612,16 @@ int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const voi
     dim3 grid((n + TT - 1) / TT, (n + TT - 1) / TT);
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_ELEM, &ev);
-    if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_update_w<double>, grid, dim3(EW_NT), 0, ctx->stream, n, ld, x, (const double *)Wd,
-                           (const double *)Wdd, (double *)W);
-    else if (dt == OMX_F32)
-        hipLaunchKernelGGL(k_update_w<float>, grid, dim3(EW_NT), 0, ctx->stream, n, ld, x, (const float *)Wd,
-                           (const float *)Wdd, (float *)W);
-    else {
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_update_w<T>, grid, dim3(EW_NT), 0, ctx->stream, n, ld, x, (const T *)Wd, (const T *)Wdd,
+                           (T *)W);
+    });
+    prof_end(ctx, OMX_K_ELEM, ev);
+    if (!ok) {
         set_error("omx_omm_update_w: unsupported element type %d", dt);
         return OMX_ERR_UNSUPPORTED;
     }
-    prof_end(ctx, OMX_K_ELEM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

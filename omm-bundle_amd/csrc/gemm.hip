@@ -268,20 +268,6 @@ __device__ inline void mma_slice(const typename Prec<PREC>::lds_t *As, const typ
     }
 }
 
-// Tile index for workgroup id w: XCD-contiguous bands of tiles, grouped raster inside.
-__device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn) {
-    int T = tiles_m * tiles_n;
-    int t = w;
-    if ((T & 7) == 0) t = (w & 7) * (T >> 3) + (w >> 3);
-    constexpr int GM = 8;
-    int group = t / (GM * tiles_n);
-    int first_m = group * GM;
-    int gm = min(tiles_m - first_m, GM);
-    int in = t % (GM * tiles_n);
-    tm = first_m + in % gm;
-    tn = in / gm;
-}
-
 // Pair mode (NPAIR = 2): C := alpha*(A1*B1 + A2*B2) + beta0*C0 + beta*C over a K range made of
 // the two contractions back to back (calc_G).  NPAIR = 1: the plain product.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR, class CF>
@@ -572,6 +558,8 @@ extern "C" int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m
     if (m == 0 || n == 0) return OMX_OK;
     if (k == 0) return omx_scale(ctx, dt, m, n, beta, C, ldc);
     OMX_REQUIRE(A && B && C, "omx_gemm: null operand");
+    if (dt == OMX_C128 || dt == OMX_C64)  // complex: FP64 / FP32 MFMA by element type
+        return launch_zgemm(ctx, dt, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     return launch_gemm(ctx, prec, dt, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
@@ -587,12 +575,19 @@ extern "C" int omx_omm_calc_G(omx_ctx ctx, int prec, int dt, int n, int m, const
     OMX_REQUIRE(ctx && HW && SW && HC && SC && G, "omx_omm_calc_G: null argument");
     OMX_REQUIRE(n > 0 && m > 0 && ldw >= n && ldc >= n, "omx_omm_calc_G: bad shape");
     OMX_REQUIRE(G != HC && G != SC, "omx_omm_calc_G: G must not alias HC or SC");
+    const bool cplx = (dt == OMX_C128 || dt == OMX_C64);
     // G = 4*HC - 2*(SW*HC + HW*SC): pair product with the 4*HC term folded into the epilogue.
-    int rc = launch_gemm_pair(ctx, prec, dt, n, m, n, -2.0, SW, HW, ldw, HC, SC, ldc, 4.0, HC, 0.0, G, ldc);
+    int rc = cplx ? launch_zgemm_pair(ctx, dt, n, m, n, -2.0, SW, HW, ldw, HC, SC, ldc, 4.0, HC, 0.0, G, ldc)
+                  : launch_gemm_pair(ctx, prec, dt, n, m, n, -2.0, SW, HW, ldw, HC, SC, ldc, 4.0, HC, 0.0, G, ldc);
     if (rc != OMX_ERR_UNSUPPORTED) return rc;
     // Unaligned n: three passes (G = 4HC; G -= 2 SW HC; G -= 2 HW SC), as calc_G in omm_ops.F90:35-37.
     rc = omx_add(ctx, dt, 'n', n, m, 4.0, HC, ldc, 0.0, G, ldc);
     if (rc) return rc;
+    if (cplx) {
+        rc = launch_zgemm(ctx, dt, 'n', 'n', n, m, n, -2.0, SW, ldw, HC, ldc, 1.0, G, ldc);
+        if (rc) return rc;
+        return launch_zgemm(ctx, dt, 'n', 'n', n, m, n, -2.0, HW, ldw, SC, ldc, 1.0, G, ldc);
+    }
     rc = launch_gemm(ctx, prec, dt, false, false, n, m, n, -2.0, SW, ldw, HC, ldc, 1.0, G, ldc);
     if (rc) return rc;
     return launch_gemm(ctx, prec, dt, false, false, n, m, n, -2.0, HW, ldw, SC, ldc, 1.0, G, ldc);

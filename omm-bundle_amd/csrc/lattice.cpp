@@ -118,6 +118,78 @@ std::vector<Contrib> build(int Lx, int Ly, int Lz, uint64_t seed, int which, con
     return merged;
 }
 
+// Bloch operator H(k) (which = 0) or S(k) (which = 1) of the periodic Lx x Ly x Lz cell: the
+// same blocks as build(), with every bond that crosses the cell boundary in direction d
+// multiplied by the phase exp(2 pi i phi_d) (forward block) and its conjugate (backward block),
+// i.e. H(k) = sum_R H_R exp(i k.R) with k.R = 2 pi phi_d for a translation by one cell along d.
+// Values are complex, (re, im) interleaved: 32 doubles per block.
+struct ContribZ {
+    int64_t I, J;
+    double v[32];
+};
+
+std::vector<ContribZ> build_k(int Lx, int Ly, int Lz, uint64_t seed, int which, const double *phi) {
+    const int64_t ns = (int64_t)Lx * Ly * Lz;
+    const int L[3] = {Lx, Ly, Lz};
+    std::vector<ContribZ> out;
+    out.reserve((size_t)ns * 7);
+    for (int64_t s = 0; s < ns; ++s) {
+        ContribZ c{};
+        c.I = c.J = s;
+        if (which == 0) {
+            double A[16];
+            for (int k = 0; k < 16; ++k) A[k] = uab(seed, OMX_STREAM_ONSITE, (uint64_t)s * 16 + k, -0.1, 0.1);
+            for (int r = 0; r < 4; ++r)
+                for (int q = 0; q < 4; ++q) {
+                    double sym = 0.5 * (A[4 * r + q] + A[4 * q + r]);
+                    double diag = (r == q) ? (r == 0 ? -5.0 : 1.0) : 0.0;
+                    c.v[2 * (r + 4 * q)] = diag + sym;
+                }
+        } else {
+            for (int r = 0; r < 4; ++r) c.v[2 * (r + 4 * r)] = 1.0;
+        }
+        out.push_back(c);
+        const int64_t coord[3] = {s % Lx, (s / Lx) % Ly, s / ((int64_t)Lx * Ly)};
+        for (int d = 0; d < 3; ++d) {
+            const int64_t t = neighbour(s, d, Lx, Ly, Lz);
+            const bool wraps = (coord[d] == L[d] - 1);
+            const double ang = wraps ? 2.0 * M_PI * phi[d] : 0.0;
+            const double cr = std::cos(ang), ci = std::sin(ang);
+            ContribZ f{}, b{};
+            f.I = s;
+            f.J = t;
+            b.I = t;
+            b.J = s;
+            for (int r = 0; r < 4; ++r)
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t idx = ((uint64_t)s * 3 + d) * 16 + 4 * r + q;
+                    const double v = (which == 0) ? uab(seed, OMX_STREAM_HOP, idx, -1.0, 1.0)
+                                                  : 0.03 * uab(seed, OMX_STREAM_OVL, idx, -1.0, 1.0);
+                    // forward block (s -> t): T e^{i ang}; backward block (t -> s): T^T e^{-i ang}
+                    f.v[2 * (r + 4 * q)] = v * cr;
+                    f.v[2 * (r + 4 * q) + 1] = v * ci;
+                    b.v[2 * (q + 4 * r)] = v * cr;
+                    b.v[2 * (q + 4 * r) + 1] = -v * ci;
+                }
+            out.push_back(f);
+            out.push_back(b);
+        }
+    }
+    std::stable_sort(out.begin(), out.end(), [](const ContribZ &a, const ContribZ &b) {
+        return a.J != b.J ? a.J < b.J : a.I < b.I;
+    });
+    std::vector<ContribZ> merged;
+    merged.reserve(out.size());
+    for (const ContribZ &c : out) {
+        if (!merged.empty() && merged.back().I == c.I && merged.back().J == c.J) {
+            for (int k = 0; k < 32; ++k) merged.back().v[k] += c.v[k];
+        } else {
+            merged.push_back(c);
+        }
+    }
+    return merged;
+}
+
 int fill(const std::vector<Contrib> &blocks, int64_t nbc, int64_t *nnzb, int *bcolptr, int *browidx, double *bval) {
     *nnzb = (int64_t)blocks.size();
     if (!bcolptr) return 0;
@@ -164,6 +236,23 @@ int omx_lattice_theta_jacobian(int Lx, int Ly, int Lz, int64_t *nent, int64_t *s
         if (seg) seg[p + 1] = k;
     }
     *nent = k;
+    return 0;
+}
+
+int omx_lattice_bsr_k(int Lx, int Ly, int Lz, uint64_t seed, int which, const double *phi, int64_t *nnzb,
+                      int *bcolptr, int *browidx, double *bval) {
+    if (Lx < 1 || Ly < 1 || Lz < 1 || (which != 0 && which != 1) || !phi || !nnzb) return OMX_ERR_ARG;
+    auto blocks = build_k(Lx, Ly, Lz, seed, which, phi);
+    *nnzb = (int64_t)blocks.size();
+    if (!bcolptr) return 0;
+    const int64_t nbc = (int64_t)Lx * Ly * Lz;
+    std::fill(bcolptr, bcolptr + nbc + 1, 0);
+    for (const ContribZ &c : blocks) bcolptr[c.J + 1]++;
+    for (int64_t j = 0; j < nbc; ++j) bcolptr[j + 1] += bcolptr[j];
+    for (size_t l = 0; l < blocks.size(); ++l) {
+        browidx[l] = (int)blocks[l].I;
+        std::memcpy(bval + 32 * l, blocks[l].v, sizeof(double) * 32);
+    }
     return 0;
 }
 

@@ -112,11 +112,35 @@ inline bool valid_op(char op) { return op == 'n' || op == 'N' || is_trans(op); }
 // Scalar readback: copy `n` doubles from ctx->d_scalars to ctx->h_scalars and wait.
 int read_scalars(omx_ctx ctx, int n);
 
+// Tile index for workgroup id w of a tiles_m x tiles_n product: workgroup ids are dealt
+// round-robin over the 8 XCDs, so each XCD is given a contiguous band of tiles; inside a band
+// the tiles go in groups of 8 tile rows, column by column, so operand panels are shared in one
+// XCD's L2.
+__device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn) {
+    int T = tiles_m * tiles_n;
+    int t = w;
+    if ((T & 7) == 0) t = (w & 7) * (T >> 3) + (w >> 3);
+    constexpr int GM = 8;
+    int group = t / (GM * tiles_n);
+    int first_m = group * GM;
+    int gm = min(tiles_m - first_m, GM);
+    int in = t % (GM * tiles_n);
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+
 // Launch helpers implemented in the .hip files.
 int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha,
                 const void *A, int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc);
+// Complex products (OMX_C128 on FP64 MFMA, OMX_C64 on FP32 MFMA); op 'n', 't' or 'c' (conjugate
+// transpose); real alpha, beta.
+int launch_zgemm(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+                 int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+int launch_zgemm_pair(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,
+                      int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0, const void *C0,
+                      double beta, void *C, int64_t ldc);
 
 }  // namespace omx

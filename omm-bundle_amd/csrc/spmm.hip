@@ -43,6 +43,10 @@ template <>
 struct VecT<float, 1> {
     typedef float type;
 };
+template <>
+struct VecT<float, 2> {
+    typedef float2 type;
+};
 
 template <typename T, int VEC>
 __device__ inline void vload(const T *p, T (&out)[VEC]) {
@@ -202,6 +206,123 @@ static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64
     return OMX_OK;
 }
 
+// Complex operator and complex X, Y (interleaved (re, im), element type R per part): the same
+// band-major mapping; each lane holds VEC complex states.
+template <typename R, int VEC, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4_z(int no, int nb, int nbc, double alpha,
+                                                        const R *__restrict__ X, int64_t ldx,
+                                                        const int *__restrict__ colptr, const int *__restrict__ rowidx,
+                                                        const double *__restrict__ val, double beta,
+                                                        R *__restrict__ Y, int64_t ldy, int nbands, int jgroups) {
+    constexpr int ROWS = 64 * VEC;
+    constexpr int NB = 4;
+    const int w = blockIdx.x;
+    int band, jg;
+    if ((nbands & 7) == 0) {
+        const int xcd = w & 7, local = w >> 3;
+        band = (local / jgroups) * 8 + xcd;
+        jg = local % jgroups;
+    } else {
+        band = w / jgroups;
+        jg = w % jgroups;
+    }
+    const int J = jg * WPG + (threadIdx.x >> 6);
+    if (J >= nbc) return;  // whole wave; the kernel has no barriers
+    const int i0 = band * ROWS + (threadIdx.x & 63) * VEC;
+    const bool full = (i0 + VEC <= no);
+    const bool any = (i0 < no);
+    R ar[4][VEC], ai[4][VEC];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) ar[c][q] = ai[c][q] = (R)0;
+    const int lb = colptr[J], le = colptr[J + 1];
+    for (int l0 = lb; l0 < le; l0 += NB) {
+        const int nblk = min(NB, le - l0);
+        R xr[NB][4][VEC], xi[NB][4][VEC];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (b < nblk) {
+                const int I = rowidx[l0 + b];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int kcol = 4 * I + r;
+                    const R *src = X + 2 * ((int64_t)kcol * ldx + i0);
+                    R tmp[2 * VEC];
+                    if (kcol < nb && full) {
+                        vload<R, 2 * VEC>(src, tmp);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 2 * VEC; ++q) tmp[q] = (kcol < nb && any && i0 + q / 2 < no) ? src[q] : (R)0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < VEC; ++q) {
+                        xr[b][r][q] = tmp[2 * q];
+                        xi[b][r][q] = tmp[2 * q + 1];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (b < nblk) {
+                const double *v = val + 32 * (int64_t)(l0 + b);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const R vr = (R)v[2 * (r + 4 * c)], vi = (R)v[2 * (r + 4 * c) + 1];
+#pragma unroll
+                        for (int q = 0; q < VEC; ++q) {
+                            ar[c][q] = fma(xr[b][r][q], vr, ar[c][q]);
+                            ar[c][q] = fma(-xi[b][r][q], vi, ar[c][q]);
+                            ai[c][q] = fma(xr[b][r][q], vi, ai[c][q]);
+                            ai[c][q] = fma(xi[b][r][q], vr, ai[c][q]);
+                        }
+                    }
+            }
+        }
+    }
+    if (!any) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int jcol = 4 * J + c;
+        if (jcol >= nb) break;
+        R *dst = Y + 2 * ((int64_t)jcol * ldy + i0);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            if (!full && i0 + q >= no) continue;
+            double orr = alpha * (double)ar[c][q], oii = alpha * (double)ai[c][q];
+            if (beta != 0.0) {
+                orr += beta * (double)dst[2 * q];
+                oii += beta * (double)dst[2 * q + 1];
+            }
+            dst[2 * q] = (R)orr;
+            dst[2 * q + 1] = (R)oii;
+        }
+    }
+}
+
+template <typename R, int VEC>
+static int run_spmm_z(omx_ctx ctx, int no, int nb, double alpha, const R *X, int64_t ldx, omx_bsr A, double beta,
+                      R *Y, int64_t ldy) {
+    constexpr int ROWS = 64 * VEC, WPG = 4;
+    const int nbands = (no + ROWS - 1) / ROWS;
+    const int jgroups = (A->nbc + WPG - 1) / WPG;
+    const int64_t total = (int64_t)nbands * jgroups;
+    OMX_REQUIRE(total < (1LL << 31), "omx_spmm: grid too large");
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_SPMM, &ev);
+    prof_work(ctx, OMX_K_SPMM, 8.0 * 16.0 * A->nnzb * no,
+              2.0 * sizeof(R) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (32.0 * 8 + 4) +
+                  4.0 * (A->nbc + 1));
+    hipLaunchKernelGGL((k_spmm_bsr4_z<R, VEC, WPG>), dim3((unsigned)total), dim3(64 * WPG), 0, ctx->stream, no, nb,
+                       A->nbc, alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nbands, jgroups);
+    prof_end(ctx, OMX_K_SPMM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace omx
@@ -220,10 +341,18 @@ extern "C" int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const
     OMX_REQUIRE(ctx && A, "omx_spmm: null argument");
     OMX_REQUIRE(nb == A->nb, "omx_spmm: operator is %d x %d but X has %d columns", A->nb, A->nb, nb);
     OMX_REQUIRE(no >= 0 && ldx >= std::max(1, no) && ldy >= std::max(1, no), "omx_spmm: bad leading dimension");
-    OMX_REQUIRE(!A->is_complex, "omx_spmm: complex operator needs a complex element type");
+    const bool zdt = (dt == OMX_C128 || dt == OMX_C64);
+    OMX_REQUIRE(!A->is_complex || zdt, "omx_spmm: complex operator needs a complex element type");
     if (no == 0) return OMX_OK;
     OMX_REQUIRE(X && Y, "omx_spmm: null operand");
     OMX_REQUIRE(X != Y, "omx_spmm: X and Y must not alias");
+    if (zdt) {
+        OMX_REQUIRE(A->is_complex, "omx_spmm: complex X needs a complex operator");
+        if (dt == OMX_C128) return run_spmm_z<double, 1>(ctx, no, nb, alpha, (const double *)X, ldx, A, beta, (double *)Y, ldy);
+        const bool vec = (no % 2 == 0) && (ldx % 2 == 0) && (ldy % 2 == 0) && aligned16(X) && aligned16(Y);
+        if (vec) return run_spmm_z<float, 2>(ctx, no, nb, alpha, (const float *)X, ldx, A, beta, (float *)Y, ldy);
+        return run_spmm_z<float, 1>(ctx, no, nb, alpha, (const float *)X, ldx, A, beta, (float *)Y, ldy);
+    }
     if (dt == OMX_F64) {
         bool vec = (no % 2 == 0) && (ldx % 2 == 0) && (ldy % 2 == 0) && aligned16(X) && aligned16(Y);
         if (vec) return run_spmm<double, 2>(ctx, no, nb, alpha, (const double *)X, ldx, A, beta, (double *)Y, ldy);

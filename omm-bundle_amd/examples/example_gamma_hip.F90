@@ -6,7 +6,9 @@
 ! matrix.  Checks e_min, D_11 and ED_11 against the reference's constants (tolerance 1e-4; with    !
 ! 'hmx' the density elements 3e-4: the low-precision phase ends on the energy, which is quadratic  !
 ! in the error of C, while D is linear in it).                                                     !
-! Usage: example_gamma_hip <path to example_gamma.dat> [operation label, default 'hip']            !
+! Flavours 1 (Cholesky) and 3 (preconditioning) run on dense device H and S ('gdden'), as the     !
+! reference's example runs them on 'sdden' (example_gamma.F90:119-132).                            !
+! Usage: example_gamma_hip <example_gamma.dat> [operation label, default 'hip'] [flavour, default 0]!
 !==================================================================================================!
 program example_gamma_hip
   use MatrixSwitch
@@ -18,7 +20,7 @@ program example_gamma_hip
   character(3) :: m_operation
   character(len=512) :: path, arg
   logical :: new_S, init_C
-  integer :: m, n, num_elements, i, j
+  integer :: m, n, num_elements, i, j, flavour
   integer, allocatable :: idx(:,:)
   real(dp) :: e_min, el, d_tol
   real(dp), allocatable :: vals(:,:)
@@ -31,6 +33,11 @@ program example_gamma_hip
     m_operation = arg(1:3)
   end if
   m_storage = 'gdden'
+  flavour = 0
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    read(arg, *) flavour
+  end if
   d_tol = 1.0e-4_dp
   if (m_operation == 'hmx') d_tol = 3.0e-4_dp
 
@@ -56,7 +63,11 @@ program example_gamma_hip
       end do
       new_S = .true.
       init_C = .false.
-      call m_copy(S, Sh, 'gdcsc', 0.0_dp)
+      if (flavour == 0) then
+        call m_copy(S, Sh, 'gdcsc', 0.0_dp)
+      else
+        call m_copy(S, Sh, 'gdden')
+      end if
     else
       do j = 1, m
         call m_set_element(Hh, j, j, -0.001_dp, 1.0_dp)
@@ -65,8 +76,12 @@ program example_gamma_hip
       init_C = .true.
       call m_deallocate(H)
     end if
-    call m_copy(H, Hh, 'gdcsc', 0.0_dp)
-    call omm(m, n, H, S, new_S, e_min, D_min, .false., 0.0_dp, C_min, init_C, T, 0.0_dp, 0, 1, 1, -1.0_dp, &
+    if (flavour == 0) then
+      call m_copy(H, Hh, 'gdcsc', 0.0_dp)
+    else
+      call m_copy(H, Hh, 'gdden')
+    end if
+    call omm(m, n, H, S, new_S, e_min, D_min, .false., 0.0_dp, C_min, init_C, T, 0.0_dp, flavour, 1, 1, -1.0_dp, &
              .true., .false., m_storage, m_operation)
     print '(a,i1,a,f21.15)', 'e_min [', i, '] : ', e_min
     call check(e_min, e_min_check(i), 1.0e-4_dp)

@@ -56,9 +56,21 @@ module MatrixSwitch
   interface m_allocate
     module procedure m_allocate_elements
   end interface
+  interface m_set_element
+    module procedure m_dset_element, m_zset_element
+  end interface
+  interface m_get_element
+    module procedure m_dget_element, m_zget_element
+  end interface
+  interface ms_upload
+    module procedure ms_upload_d, ms_upload_z
+  end interface
+  interface ms_download
+    module procedure ms_download_d, ms_download_z
+  end interface
 
   public :: m_allocate, m_deallocate, m_copy, mm_multiply, m_add, m_trace, mm_trace, m_scale, m_set
-  public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr
+  public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr, m_register_gbsr_z
   public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download, ms_convert
   public :: ms_label_prec, die
 
@@ -151,10 +163,19 @@ contains
       esize = 8
     case (OMX_F32)
       esize = 4
+    case (OMX_C128)
+      esize = 16
+    case (OMX_C64)
+      esize = 8
     case default
       esize = 8
     end select
   end function esize
+
+  logical function is_cplx_dt(dt)
+    integer(c_int), intent(in) :: dt
+    is_cplx_dt = (dt == OMX_C128 .or. dt == OMX_C64)
+  end function is_cplx_dt
 
   !================================================================================================!
   ! allocation                                                                                     !
@@ -202,7 +223,11 @@ contains
       m_name%dtype = OMX_F32
     case ('z')
       m_name%is_real = .false.
-      if (m_name%is_device) call die('m_allocate_elements: complex device storage is not available')
+      m_name%dtype = OMX_C128
+    case ('c')
+      if (.not. m_name%is_device) call die('m_allocate_elements: invalid label')
+      m_name%is_real = .false.
+      m_name%dtype = OMX_C64
     case default
       call die('m_allocate_elements: invalid label')
     end select
@@ -339,6 +364,27 @@ contains
     m_name%is_initialized = .true.
   end subroutine m_register_gbsr
 
+  !> m_register_gbsr for a complex operator: 32 doubles per block, (re, im) interleaved ('gzcsc').
+  subroutine m_register_gbsr_z(m_name, nb, bcolptr, browidx, bval)
+    type(matrix), intent(inout) :: m_name
+    integer, intent(in) :: nb
+    integer(c_int), intent(in), target :: bcolptr(:), browidx(:)
+    real(dp), intent(in), target :: bval(:)
+    integer(c_int) :: rc
+    m_name%str_type = 'csc'
+    m_name%is_serial = .true.
+    m_name%is_real = .false.
+    m_name%is_sparse = .true.
+    m_name%is_device = .true.
+    m_name%dtype = OMX_C128
+    m_name%dim1 = nb
+    m_name%dim2 = nb
+    m_name%is_square = .true.
+    rc = omx_bsr_create_z(ctx(), int(nb, c_int), c_loc(bcolptr), c_loc(browidx), c_loc(bval), m_name%bsr)
+    call chk(rc, 'm_register_gbsr_z')
+    m_name%is_initialized = .true.
+  end subroutine m_register_gbsr_z
+
   !> Copy a dense device matrix into another allocated one of the same shape, converting the
   !> element type if they differ (e.g. FP64 <-> FP32 working sets).
   subroutine ms_convert(src, dst)
@@ -348,19 +394,90 @@ contains
     if (.not. (src%is_device .and. dst%is_device) .or. src%is_sparse .or. dst%is_sparse) &
       call die('ms_convert: dense device matrices expected')
     if (src%dim1 /= dst%dim1 .or. src%dim2 /= dst%dim2) call die('ms_convert: shape mismatch')
-    rc = omx_convert(ctx(), int(src%dim1, c_int), int(src%dim2, c_int), src%dptr, src%dtype, src%ld, dst%dptr, &
-                     dst%dtype, dst%ld)
+    if (is_cplx_dt(src%dtype) .neqv. is_cplx_dt(dst%dtype)) call die('ms_convert: real and complex storage')
+    if (is_cplx_dt(src%dtype)) then
+      ! complex m x n with leading dimension ld = real 2m x n with leading dimension 2*ld
+      rc = omx_convert(ctx(), int(2*src%dim1, c_int), int(src%dim2, c_int), src%dptr, real_dt(src%dtype), &
+                       2*src%ld, dst%dptr, real_dt(dst%dtype), 2*dst%ld)
+    else
+      rc = omx_convert(ctx(), int(src%dim1, c_int), int(src%dim2, c_int), src%dptr, src%dtype, src%ld, dst%dptr, &
+                       dst%dtype, dst%ld)
+    end if
     call chk(rc, 'ms_convert')
   end subroutine ms_convert
 
-  !> Host <-> device transfers of a dense matrix.
-  subroutine ms_upload(m_name, A)
+  integer(c_int) function real_dt(dt)
+    integer(c_int), intent(in) :: dt
+    real_dt = dt
+    if (dt == OMX_C128) real_dt = OMX_F64
+    if (dt == OMX_C64) real_dt = OMX_F32
+  end function real_dt
+
+  !> Host <-> device transfers of a dense complex matrix (C128 directly; C64 staged in C128 on the
+  !> device and converted there).
+  subroutine ms_upload_z(m_name, A)
+    type(matrix), intent(inout) :: m_name
+    complex(dp), intent(in), target, contiguous :: A(:,:)
+    type(c_ptr) :: tmp
+    integer(c_int) :: rc
+    if (.not. (m_name%is_device .and. m_name%str_type == 'den')) call die('ms_upload: not a dense device matrix')
+    if (.not. is_cplx_dt(m_name%dtype)) call die('ms_upload: complex data into real storage')
+    if (size(A,1) /= m_name%dim1 .or. size(A,2) /= m_name%dim2) call die('ms_upload: shape mismatch')
+    if (m_name%dtype == OMX_C128) then
+      rc = omx_h2d_2d(ctx(), m_name%dptr, m_name%ld, c_loc(A), int(max(m_name%dim1,1), c_int64_t), &
+                      int(m_name%dim1, c_int), int(m_name%dim2, c_int), 16_c_int)
+      call chk(rc, 'ms_upload')
+    else
+      rc = omx_malloc(ctx(), int(max(m_name%dim1,1), c_size_t) * int(max(m_name%dim2,1), c_size_t) * 16_c_size_t, tmp)
+      call chk(rc, 'ms_upload')
+      rc = omx_h2d_2d(ctx(), tmp, int(max(m_name%dim1,1), c_int64_t), c_loc(A), int(max(m_name%dim1,1), c_int64_t), &
+                      int(m_name%dim1, c_int), int(m_name%dim2, c_int), 16_c_int)
+      call chk(rc, 'ms_upload')
+      rc = omx_convert(ctx(), int(2*m_name%dim1, c_int), int(m_name%dim2, c_int), tmp, OMX_F64, &
+                       int(2*max(m_name%dim1,1), c_int64_t), m_name%dptr, OMX_F32, 2*m_name%ld)
+      call chk(rc, 'ms_upload')
+      rc = omx_free(ctx(), tmp)
+    end if
+  end subroutine ms_upload_z
+
+  subroutine ms_download_z(m_name, A)
+    type(matrix), intent(in) :: m_name
+    complex(dp), intent(inout), target, contiguous :: A(:,:)
+    type(c_ptr) :: tmp
+    integer(c_int) :: rc
+    if (.not. (m_name%is_device .and. m_name%str_type == 'den')) call die('ms_download: not a dense device matrix')
+    if (.not. is_cplx_dt(m_name%dtype)) call die('ms_download: real storage into complex data')
+    if (size(A,1) /= m_name%dim1 .or. size(A,2) /= m_name%dim2) call die('ms_download: shape mismatch')
+    if (m_name%dtype == OMX_C128) then
+      rc = omx_d2h_2d(ctx(), c_loc(A), int(max(m_name%dim1,1), c_int64_t), m_name%dptr, m_name%ld, &
+                      int(m_name%dim1, c_int), int(m_name%dim2, c_int), 16_c_int)
+      call chk(rc, 'ms_download')
+    else
+      rc = omx_malloc(ctx(), int(max(m_name%dim1,1), c_size_t) * int(max(m_name%dim2,1), c_size_t) * 16_c_size_t, tmp)
+      call chk(rc, 'ms_download')
+      rc = omx_convert(ctx(), int(2*m_name%dim1, c_int), int(m_name%dim2, c_int), m_name%dptr, OMX_F32, &
+                       2*m_name%ld, tmp, OMX_F64, int(2*max(m_name%dim1,1), c_int64_t))
+      call chk(rc, 'ms_download')
+      rc = omx_d2h_2d(ctx(), c_loc(A), int(max(m_name%dim1,1), c_int64_t), tmp, int(max(m_name%dim1,1), c_int64_t), &
+                      int(m_name%dim1, c_int), int(m_name%dim2, c_int), 16_c_int)
+      call chk(rc, 'ms_download')
+      rc = omx_free(ctx(), tmp)
+    end if
+  end subroutine ms_download_z
+
+  !> Host <-> device transfers of a dense matrix (real data into complex storage get a zero
+  !> imaginary part).
+  subroutine ms_upload_d(m_name, A)
     type(matrix), intent(inout) :: m_name
     real(dp), intent(in), target, contiguous :: A(:,:)
     type(c_ptr) :: tmp
     integer(c_int) :: rc
     if (.not. (m_name%is_device .and. m_name%str_type == 'den')) call die('ms_upload: not a dense device matrix')
     if (size(A,1) /= m_name%dim1 .or. size(A,2) /= m_name%dim2) call die('ms_upload: shape mismatch')
+    if (is_cplx_dt(m_name%dtype)) then
+      call ms_upload_z(m_name, cmplx(A, 0.0_dp, dp))
+      return
+    end if
     if (m_name%dtype == OMX_F64) then
       rc = omx_h2d_2d(ctx(), m_name%dptr, m_name%ld, c_loc(A), int(max(m_name%dim1,1), c_int64_t), &
                       int(m_name%dim1, c_int), int(m_name%dim2, c_int), 8_c_int)
@@ -377,15 +494,16 @@ contains
       call chk(rc, 'ms_upload')
       rc = omx_free(ctx(), tmp)
     end if
-  end subroutine ms_upload
+  end subroutine ms_upload_d
 
-  subroutine ms_download(m_name, A)
+  subroutine ms_download_d(m_name, A)
     type(matrix), intent(in) :: m_name
     real(dp), intent(inout), target, contiguous :: A(:,:)
     type(c_ptr) :: tmp
     integer(c_int) :: rc
     if (.not. (m_name%is_device .and. m_name%str_type == 'den')) call die('ms_download: not a dense device matrix')
     if (size(A,1) /= m_name%dim1 .or. size(A,2) /= m_name%dim2) call die('ms_download: shape mismatch')
+    if (is_cplx_dt(m_name%dtype)) call die('ms_download: complex storage into real data')
     if (m_name%dtype == OMX_F64) then
       rc = omx_d2h_2d(ctx(), c_loc(A), int(max(m_name%dim1,1), c_int64_t), m_name%dptr, m_name%ld, &
                       int(m_name%dim1, c_int), int(m_name%dim2, c_int), 8_c_int)
@@ -401,7 +519,7 @@ contains
       call chk(rc, 'ms_download')
       rc = omx_free(ctx(), tmp)
     end if
-  end subroutine ms_download
+  end subroutine ms_download_d
 
   !================================================================================================!
   ! m_copy (MatrixSwitch.F90:630-963): host copies, CSC construction, host <-> device moves       !
@@ -425,11 +543,16 @@ contains
     else
       if (A%is_device) then
         lab(1:1) = 'g'
-        if (A%dtype == OMX_F32) then
+        select case (A%dtype)
+        case (OMX_F32)
           lab(2:2) = 's'
-        else
+        case (OMX_C128)
+          lab(2:2) = 'z'
+        case (OMX_C64)
+          lab(2:2) = 'c'
+        case default
           lab(2:2) = 'd'
-        end if
+        end select
       else
         lab(1:1) = 's'
         lab(2:2) = 'd'
@@ -443,7 +566,10 @@ contains
     if (present(threshold_is_soft)) then
       if (threshold_is_soft) soft = thr
     end if
-    if (.not. A%is_real) call die('m_copy: complex matrices are not available in this build')
+    if (.not. A%is_real) then
+      call m_copy_z(m_name, A, lab, thr, present(threshold))
+      return
+    end if
     if (A%is_sparse .and. lab(3:5) == 'den') call die('m_copy: sparse to dense copy is not available')
 
     if (lab(3:5) == 'den') then
@@ -536,6 +662,81 @@ contains
     end if
   end subroutine m_copy
 
+  !> m_copy of complex matrices: dense host <-> device moves, and dense -> device sparse operator
+  !> ('gzcsc', entries with |a| > threshold; MatrixSwitch_m_copy.F90:495-629 for the CSC rules).
+  subroutine m_copy_z(m_name, A, lab, thr, use_thr)
+    type(matrix), intent(inout) :: m_name
+    type(matrix), intent(inout) :: A
+    character(5), intent(in) :: lab
+    real(dp), intent(in) :: thr
+    logical, intent(in) :: use_thr
+    complex(dp), allocatable, target :: tmp(:,:)
+    integer, allocatable, target :: cp(:), ri(:)
+    real(dp), allocatable, target :: cv(:)
+    integer :: i, j, k
+    integer(c_int) :: rc
+    if (lab(2:2) /= 'z' .and. lab(2:2) /= 'c') call die('m_copy: complex source needs a complex label')
+    if (A%is_sparse) call die('m_copy: complex sparse source is not available')
+    if (lab(3:5) == 'den') then
+      call m_allocate(m_name, A%dim1, A%dim2, lab)
+      if (.not. A%is_device .and. .not. m_name%is_device) then
+        m_name%zval = A%zval
+        if (use_thr) where (abs(m_name%zval) <= thr) m_name%zval = (0.0_dp, 0.0_dp)
+      else if (.not. A%is_device .and. m_name%is_device) then
+        allocate(tmp(A%dim1, A%dim2))
+        tmp = A%zval
+        if (use_thr) where (abs(tmp) <= thr) tmp = (0.0_dp, 0.0_dp)
+        call ms_upload_z(m_name, tmp)
+      else if (A%is_device .and. .not. m_name%is_device) then
+        allocate(tmp(A%dim1, A%dim2))
+        call ms_download_z(A, tmp)
+        if (use_thr) where (abs(tmp) <= thr) tmp = (0.0_dp, 0.0_dp)
+        m_name%zval = tmp
+      else
+        call ms_convert(A, m_name)
+      end if
+    else if (lab(3:5) == 'csc') then
+      if (lab(1:1) /= 'g') call die('m_copy: complex host CSC is not available')
+      if (A%dim1 /= A%dim2) call die('m_copy: device sparse storage needs a square operator')
+      allocate(tmp(A%dim1, A%dim2))
+      if (A%is_device) then
+        call ms_download_z(A, tmp)
+      else
+        tmp = A%zval
+      end if
+      k = count(abs(tmp) > thr)
+      allocate(cp(A%dim2+1), ri(max(k,1)), cv(2*max(k,1)))
+      k = 0
+      cp(1) = 0
+      do i = 1, A%dim2
+        do j = 1, A%dim1
+          if (abs(tmp(j,i)) > thr) then
+            k = k + 1
+            ri(k) = j
+            cv(2*k-1) = real(tmp(j,i), dp)
+            cv(2*k) = aimag(tmp(j,i))
+          end if
+        end do
+        cp(i+1) = k
+      end do
+      m_name%dim1 = A%dim1
+      m_name%dim2 = A%dim2
+      m_name%is_square = .true.
+      m_name%is_real = .false.
+      m_name%is_serial = .true.
+      m_name%is_sparse = .true.
+      m_name%is_device = .true.
+      m_name%str_type = 'csc'
+      m_name%dtype = OMX_C128
+      rc = omx_bsr_from_csc_z(ctx(), int(A%dim2, c_int), c_loc(cp), c_loc(ri), c_loc(cv), 0_c_int, 1_c_int, &
+                              m_name%bsr)
+      call chk(rc, 'm_copy')
+      m_name%is_initialized = .true.
+    else
+      call die('m_copy: invalid label')
+    end if
+  end subroutine m_copy_z
+
   !================================================================================================!
   ! mm_multiply (MatrixSwitch.F90:1046-1390)                                                      !
   !================================================================================================!
@@ -564,7 +765,12 @@ contains
       if ((A%dim2/=C%dim1) .or. (B%dim1/=C%dim2) .or. (A%dim1/=B%dim2)) &
         call die('mm_dmultiply: matrices A, B and C are not compatible')
     end if
-    if (.not. (A%is_real .and. B%is_real .and. C%is_real)) call die('mm_dmultiply: complex matrices are not available')
+    if (.not. (A%is_real .and. B%is_real .and. C%is_real)) then
+      if (.not. (A%is_device .and. B%is_device .and. C%is_device)) &
+        call die('mm_zmultiply: complex matrices are available in device storage only')
+      if ((A%is_real .neqv. C%is_real) .or. (B%is_real .neqv. C%is_real)) &
+        call die('mm_multiply: mixed real and complex operands')
+    end if
 
     if (A%is_device .and. B%is_device .and. C%is_device) then
       call check_device_label(label, 'mm_dmultiply')
@@ -754,6 +960,20 @@ contains
     end if
     call check_host_label(label, 'm_dset')
     if (C%is_sparse) call die('m_dset: invalid implementation')
+    if (.not. C%is_real) then
+      do i = 1, C%dim1
+        do j = 1, C%dim2
+          if ((seC == 'u' .or. seC == 'U') .and. j < i) cycle
+          if ((seC == 'l' .or. seC == 'L') .and. j > i) cycle
+          if (i /= j) then
+            C%zval(i,j) = cmplx(alpha, 0.0_dp, dp)
+          else
+            C%zval(i,j) = cmplx(beta, 0.0_dp, dp)
+          end if
+        end do
+      end do
+      return
+    end if
     lu = 0
     if (seC == 'u' .or. seC == 'U') lu = 1
     if (seC == 'l' .or. seC == 'L') lu = 2
@@ -777,15 +997,23 @@ contains
     character(3), intent(in), optional :: label
     integer :: i
     integer(c_int) :: rc
+    real(c_double) :: tr2(2)
     if (.not. A%is_square) call die('m_dtrace: matrix A is not square')
     if (A%is_device) then
       call check_device_label(label, 'm_dtrace')
-      rc = omx_trace(ctx(), A%dtype, int(A%dim1, c_int), A%dptr, A%ld, alpha)
+      rc = omx_trace(ctx(), A%dtype, int(A%dim1, c_int), A%dptr, A%ld, tr2)
       call chk(rc, 'm_dtrace')
+      alpha = tr2(1)  ! complex: the real part, as the reference's real-valued interface
       return
     end if
     call check_host_label(label, 'm_dtrace')
     alpha = 0.0_dp
+    if (.not. A%is_real) then
+      do i = 1, A%dim1
+        alpha = alpha + real(A%zval(i,i), dp)
+      end do
+      return
+    end if
     do i = 1, A%dim1
       alpha = alpha + A%dval(i,i)
     end do
@@ -798,17 +1026,27 @@ contains
     character(3), intent(in), optional :: label
     integer :: i, j
     integer(c_int) :: rc
+    real(c_double) :: tr2(2)
     if ((A%dim1/=B%dim1) .or. (A%dim2/=B%dim2)) call die('mm_dtrace: matrices A and B are not compatible')
     if (A%is_device .and. B%is_device) then
       call check_device_label(label, 'mm_dtrace')
       if (A%dtype /= B%dtype) call die('mm_dtrace: element types differ')
-      rc = omx_mm_trace(ctx(), A%dtype, int(A%dim1, c_int), int(A%dim2, c_int), A%dptr, A%ld, B%dptr, B%ld, alpha)
+      rc = omx_mm_trace(ctx(), A%dtype, int(A%dim1, c_int), int(A%dim2, c_int), A%dptr, A%ld, B%dptr, B%ld, tr2)
       call chk(rc, 'mm_dtrace')
+      alpha = tr2(1)  ! complex: Re sum conj(A_ij) B_ij
       return
     end if
     if (A%is_device .or. B%is_device) call die('mm_dtrace: invalid implementation')
     call check_host_label(label, 'mm_dtrace')
     alpha = 0.0_dp
+    if (.not. A%is_real) then
+      do i = 1, A%dim1
+        do j = 1, A%dim2
+          alpha = alpha + real(conjg(A%zval(i,j))*B%zval(i,j), dp)
+        end do
+      end do
+      return
+    end if
     do i = 1, A%dim1
       do j = 1, A%dim2
         alpha = alpha + A%dval(i,j)*B%dval(i,j)
@@ -817,7 +1055,7 @@ contains
   end subroutine mm_trace
 
   !> C_ij := alpha + beta*C_ij (MatrixSwitch.F90:2917-3086).
-  subroutine m_set_element(C, i, j, alpha, beta, label)
+  subroutine m_dset_element(C, i, j, alpha, beta, label)
     type(matrix), intent(inout) :: C
     integer, intent(in) :: i, j
     real(dp), intent(in) :: alpha, beta
@@ -831,26 +1069,59 @@ contains
       return
     end if
     if (C%is_sparse) call die('m_dset_element: invalid implementation')
+    if (.not. C%is_real) then
+      C%zval(i,j) = cmplx(alpha, 0.0_dp, dp) + beta*C%zval(i,j)
+      return
+    end if
     C%dval(i,j) = alpha + beta*C%dval(i,j)
-  end subroutine m_set_element
+  end subroutine m_dset_element
 
-  !> alpha := C_ij (MatrixSwitch.F90:3309-3417).
-  subroutine m_get_element(C, i, j, alpha, label)
+  !> C_ij := alpha + beta*C_ij with complex alpha, beta (m_zset_element; real beta on device).
+  subroutine m_zset_element(C, i, j, alpha, beta, label)
+    type(matrix), intent(inout) :: C
+    integer, intent(in) :: i, j
+    complex(dp), intent(in) :: alpha, beta
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc
+    if (i < 1 .or. i > C%dim1 .or. j < 1 .or. j > C%dim2) call die('m_zset_element: element out of range')
+    if (C%is_real) call die('m_zset_element: real matrix')
+    if (C%is_device) then
+      if (C%is_sparse) call die('m_zset_element: invalid implementation')
+      if (aimag(beta) /= 0.0_dp) call die('m_zset_element: complex beta on device storage')
+      rc = omx_set_element_z(ctx(), C%dtype, C%dptr, C%ld, int(i, c_int), int(j, c_int), real(alpha, dp), &
+                             aimag(alpha), real(beta, dp))
+      call chk(rc, 'm_zset_element')
+      return
+    end if
+    if (C%is_sparse) call die('m_zset_element: invalid implementation')
+    C%zval(i,j) = alpha + beta*C%zval(i,j)
+  end subroutine m_zset_element
+
+  !> alpha := C_ij (MatrixSwitch.F90:3309-3417); the real part for complex matrices.
+  subroutine m_dget_element(C, i, j, alpha, label)
     type(matrix), intent(in) :: C
     integer, intent(in) :: i, j
     real(dp), intent(out) :: alpha
     character(3), intent(in), optional :: label
     integer(c_int) :: rc
     integer :: l
+    complex(dp) :: z
+    real(c_double) :: v2(2)
     if (i < 1 .or. i > C%dim1 .or. j < 1 .or. j > C%dim2) call die('m_dget_element: element out of range')
+    if (.not. C%is_real) then
+      call m_zget_element(C, i, j, z, label)
+      alpha = real(z, dp)
+      return
+    end if
     if (C%is_device) then
       if (C%is_sparse) then
         rc = omx_bsr_get_element(C%bsr, int(i, c_int), int(j, c_int), alpha)
         call chk(rc, 'm_dget_element')
         return
       end if
-      rc = omx_get_element(ctx(), C%dtype, C%dptr, C%ld, int(i, c_int), int(j, c_int), alpha)
+      rc = omx_get_element(ctx(), C%dtype, C%dptr, C%ld, int(i, c_int), int(j, c_int), v2)
       call chk(rc, 'm_dget_element')
+      alpha = v2(1)
       return
     end if
     if (C%is_sparse) then
@@ -861,7 +1132,33 @@ contains
       return
     end if
     alpha = C%dval(i,j)
-  end subroutine m_get_element
+  end subroutine m_dget_element
+
+  !> alpha := C_ij as a complex number (m_zget_element).
+  subroutine m_zget_element(C, i, j, alpha, label)
+    type(matrix), intent(in) :: C
+    integer, intent(in) :: i, j
+    complex(dp), intent(out) :: alpha
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc
+    real(c_double) :: v2(2)
+    real(dp) :: re
+    if (i < 1 .or. i > C%dim1 .or. j < 1 .or. j > C%dim2) call die('m_zget_element: element out of range')
+    if (C%is_real) then
+      call m_dget_element(C, i, j, re, label)
+      alpha = cmplx(re, 0.0_dp, dp)
+      return
+    end if
+    if (C%is_device) then
+      if (C%is_sparse) call die('m_zget_element: invalid implementation')
+      rc = omx_get_element(ctx(), C%dtype, C%dptr, C%ld, int(i, c_int), int(j, c_int), v2)
+      call chk(rc, 'm_zget_element')
+      alpha = cmplx(v2(1), v2(2), dp)
+      return
+    end if
+    if (C%is_sparse) call die('m_zget_element: invalid implementation')
+    alpha = C%zval(i,j)
+  end subroutine m_zget_element
 
   !> Fatal error with the reference behaviour (MatrixSwitch/src/MatrixSwitch_ops.F90:232-254).
   subroutine die(message)

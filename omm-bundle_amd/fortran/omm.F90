@@ -19,7 +19,10 @@
 ! of HW/SW) stays below 'lowp_tol' relative for 'lowp_consec' consecutive steps.  Traces, the line !
 ! search and the energy are FP64 in every mode.                                                    !
 !                                                                                                  !
-! Flavour 0 (basic) is implemented; flavours 1-4 (Cholesky / preconditioned) stop with die().      !
+! Flavours 0 (basic), 1 and 2 (Cholesky reduction to standard form, S given or already factored)  !
+! and 3 (preconditioner (S + T/scale_T)^-1) follow the reference's branches; their factorisations !
+! run on the device ('gdden' H and S).  Flavour 4 needs pspBLAS's kinetic-energy preconditioner   !
+! and stops with die(), as the reference does without pspBLAS.                                     !
 !==================================================================================================!
 subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, scale_T, flavour, np, ip, cg_tol, &
                long_out, dealloc, m_storage, m_operation)
@@ -38,7 +41,7 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   type(matrix), intent(inout) :: H, S, D_min, T
   type(matrix), intent(inout), target :: C_min
 
-  logical :: no_S, conv, ls_fail, dev, mixed
+  logical :: no_S, conv, ls_fail, dev, mixed, use_Cholesky, S_is_U, use_precon, std_form
   logical, save :: log_start = .false.
   logical, allocatable, save :: first_call(:)
   integer :: i, j, k, seed, icg, nfail, icg_lowp, log_unit
@@ -50,6 +53,9 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   character(3) :: prod
   type(matrix), target :: SWd, SWdd, G, G_p, D, HC, HG, work1
   type(matrix), allocatable, save, target :: HW(:), HWd(:), HWdd(:), SW(:), SC(:), SG(:), work2(:), QW(:), CD(:)
+  ! Cholesky-transformed coefficients and the preconditioner, per ip (omm.F90:102-106)
+  type(matrix), allocatable, save, target :: C_Chl(:), P(:)
+  type(matrix), target :: PG, PG_p
   ! working set of the low-precision phase of 'hmx' (FP32 storage)
   type(matrix), target :: lC, lHW, lHWd, lHWdd, lSW, lSC, lSG, lwork2
   ! the matrices the CG phase works on: the FP64 state of this ip, or the low-precision set
@@ -64,16 +70,30 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   end if
 
   no_S = .not. S%is_initialized
+  use_Cholesky = .false.
+  S_is_U = .false.
+  use_precon = .false.
   if (.not. no_S) then
-    if (flavour /= 0) then
-      if (flavour >= 1 .and. flavour <= 4) then
-        call omm_die('omm: flavours 1-4 (Cholesky / preconditioning) are not available in this build')
-      else
-        call omm_die('omm: illegal value of flavour')
-      end if
-    end if
+    ! omm.F90:125-155
+    select case (flavour)
+    case (0)
+    case (1)
+      use_Cholesky = .true.
+    case (2)
+      use_Cholesky = .true.
+      S_is_U = .true.
+    case (3)
+      use_precon = .true.
+    case (4)
+      call omm_die('omm: flavour 4 needs the kinetic-energy preconditioner of pspBLAS (calc_PW_precon)')
+    case default
+      call omm_die('omm: illegal value of flavour')
+    end select
   end if
+  ! standard form: the metric of the CG state is the identity (no S, or S reduced by Cholesky)
+  std_form = no_S .or. use_Cholesky
   mixed = (m_operation == 'hmx')
+  if (mixed .and. (use_Cholesky .or. use_precon)) call omm_die('omm: mixed precision is available for flavour 0')
   if (mixed) then
     prod = 'hip'
   else
@@ -111,6 +131,30 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   if (.not. SG(ip)%is_initialized) call m_allocate(SG(ip), n, m, m_storage)
   if (.not. work2(ip)%is_initialized) call m_allocate(work2(ip), n, m, m_storage)
   dev = opt_fused .and. C_min%is_device .and. (m_storage(1:1) == 'g')
+  if (use_Cholesky) then
+    if (.not. allocated(C_Chl)) allocate(C_Chl(np))
+    if (.not. C_Chl(ip)%is_initialized) call m_allocate(C_Chl(ip), n, m, m_storage)
+  end if
+  if (use_precon) then
+    if (.not. allocated(P)) allocate(P(np))
+    if (.not. P(ip)%is_initialized) call m_allocate(P(ip), m, m, m_storage)
+  end if
+
+  ! reduce the generalized problem to standard form (omm.F90:305-309)
+  if (use_Cholesky) then
+    if (new_S .and. (.not. S_is_U)) call m_factorize(S, prod)
+    call m_reduce(S, H, prod)
+  end if
+  ! preconditioner P = (S + T/scale_T)^-1 (omm.F90:312-324)
+  if (use_precon .and. new_S) then
+    if (T%is_initialized) then
+      call m_add(T, 'n', P(ip), 1.0_dp, 0.0_dp, prod)
+      call m_add(S, 'n', P(ip), 1.0_dp, 1.0_dp/scale_T, prod)
+    else
+      call m_add(S, 'n', P(ip), 1.0_dp, 0.0_dp, prod)
+    end if
+    call m_inverse(P(ip), prod)
+  end if
 
   ! random initial guess, normalised (omm.F90:329-343)
   if (first_call(ip) .and. (.not. init_C)) then
@@ -132,6 +176,10 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
     deallocate(cinit)
     call m_scale(C_min, 1.0e-2_dp/sqrt(real(m, dp)), scale_label())
   end if
+  if (use_Cholesky .and. (init_C .or. new_S)) then
+    call m_add(C_min, 'n', C_Chl(ip), 1.0_dp, 0.0_dp, scale_label())
+    call m_transform(S, C_Chl(ip), prod)
+  end if
 
   if (mpi_rank0()) then
     write(log_unit, '(a)') '+---------------------------------------------+'
@@ -152,6 +200,7 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
 
   if (mixed) then
     if (.not. C_min%is_device) call omm_die('omm: mixed precision needs device storage')
+    if (.not. C_min%is_real) call omm_die('omm: mixed precision is available for real matrices')
     ! Low-precision phase on an FP32 copy of everything (products in BF16 or FP32), started from
     ! C_min and recomputed from scratch; then FP64 from the resulting coefficients, again with
     ! every quantity recomputed exactly (the incremental updates of the low-precision phase carry
@@ -189,6 +238,11 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   ! density matrix D = C^T (2I - SW) C (omm.F90:575-593); skipped when D_min is not allocated
   call m_set(QW(ip), 'a', 0.0_dp, 2.0_dp, scale_label())
   call m_add(SW(ip), 'n', QW(ip), -1.0_dp, 1.0_dp, scale_label())
+  if (use_Cholesky) then
+    ! back to the original basis: C = C_Chl U^-T (omm.F90:578-581)
+    call m_add(C_Chl(ip), 'n', C_min, 1.0_dp, 0.0_dp, scale_label())
+    call m_back_transform(S, C_min, prod)
+  end if
   if (D_min%is_initialized) then
     call calc_A(QW(ip), C_min, D_min, CD(ip), prod)
   else
@@ -265,7 +319,7 @@ contains
     character(3), intent(in) :: lab
     logical, intent(in) :: recompute_S
     call calc_AW(H, pC, pHW, HC, lab)
-    if (no_S) then
+    if (std_form) then
       if (recompute_S) call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, lab)
     else
       if (recompute_S) then
@@ -274,19 +328,31 @@ contains
         call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
       end if
     end if
-    if (no_S) then
+    if (std_form) then
       call calc_G(pHW, pSW, G, HC, pC, grad_label(lab))
-      call mm_multiply(pC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     else
       call calc_G(pHW, pSW, G, HC, pSC, grad_label(lab))
-      call mm_multiply(pSC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
     end if
-    call mm_multiply(HC, 'n', G, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
-    call calc_AW(H, G, pHWdd, HG, lab)
-    if (no_S) then
-      call mm_multiply(G, 'n', G, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
+    if (use_precon) then
+      ! preconditioned search direction PG = G P (omm.F90:379-403)
+      call mm_multiply(G, 'n', P(ip), 'n', PG, 1.0_dp, 0.0_dp, lab)
+      call mm_multiply(HC, 'n', PG, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
+      call mm_multiply(pSC, 'n', PG, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+      call calc_AW(H, PG, pHWdd, HG, lab)
+      call calc_AW(S, PG, SWdd, pSG, lab)
     else
-      call calc_AW(S, G, SWdd, pSG, lab)
+      if (std_form) then
+        call mm_multiply(pC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+      else
+        call mm_multiply(pSC, 'n', G, 'c', SWd, 1.0_dp, 0.0_dp, lab)
+      end if
+      call mm_multiply(HC, 'n', G, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
+      call calc_AW(H, G, pHWdd, HG, lab)
+      if (std_form) then
+        call mm_multiply(G, 'n', G, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
+      else
+        call calc_AW(S, G, SWdd, pSG, lab)
+      end if
     end if
     call calc_coeff(pHW, pSW, pHWd, SWd, pHWdd, SWdd, work1, coeff, lab)
     e_min = coeff(0)
@@ -312,7 +378,11 @@ contains
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
       do inner = 1, n_step_max_inner
-        if (dev) then
+        if (use_precon) then
+          call m_add(PG, 'n', D, 1.0_dp, lambda, scale_label())
+          call m_add(G, 'n', G_p, 1.0_dp, 0.0_dp, scale_label())
+          call m_add(PG, 'n', PG_p, 1.0_dp, 0.0_dp, scale_label())
+        else if (dev) then
           rc = omx_omm_cg_direction(ms_hip_ctx(), G%dtype, int(n, c_int), int(m, c_int), lambda, G%dptr, D%dptr, &
                                     G_p%dptr, G%ld)
           if (rc /= 0) call omm_die('omm: '//omx_error_string())
@@ -324,13 +394,13 @@ contains
         ! after the first step of a phase, line-search data along the new direction (omm.F90:448-464)
         if (steps > 0) then
           call mm_multiply(HC, 'n', D, 'c', pHWd, 1.0_dp, 0.0_dp, lab)
-          if (no_S) then
+          if (std_form) then
             call mm_multiply(pC, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
           else
             call mm_multiply(pSC, 'n', D, 'c', SWd, 1.0_dp, 0.0_dp, lab)
           end if
           call calc_AW(H, D, pHWdd, HG, lab)
-          if (no_S) then
+          if (std_form) then
             call mm_multiply(D, 'n', D, 'c', SWdd, 1.0_dp, 0.0_dp, lab)
           else
             call calc_AW(S, D, SWdd, pSG, lab)
@@ -407,23 +477,29 @@ contains
           if (rewin_now) then
             ! HC = C H, HW = HC C^T, SC = C S, SW = SC C^T at the gradient's precision
             call calc_AW(H, pC, pHW, HC, grad_label(lab))
-            if (no_S) then
+            if (std_form) then
               call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, grad_label(lab))
             else
               call calc_AW(S, pC, pSW, pSC, grad_label(lab))
             end if
           else
             call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())
-            if (.not. no_S) call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
+            if (.not. std_form) call m_add(pSG, 'n', pSC, x_min(ip), 1.0_dp, scale_label())
           end if
-          if (no_S) then
+          if (std_form) then
             call calc_G(pHW, pSW, G, HC, pC, grad_label(lab))
           else
             call calc_G(pHW, pSW, G, HC, pSC, grad_label(lab))
           end if
         end if
+        if (use_precon) call mm_multiply(G, 'n', P(ip), 'n', PG, 1.0_dp, 0.0_dp, lab)
         if (ls_conv) then
-          if (dev) then
+          if (use_precon) then
+            call m_add(G, 'n', pwork2, 1.0_dp, 0.0_dp, scale_label())
+            call m_add(G_p, 'n', pwork2, -1.0_dp, 1.0_dp, scale_label())
+            call mm_trace(PG, pwork2, lambda_n, scale_label())
+            call mm_trace(PG_p, G_p, lambda_d, scale_label())
+          else if (dev) then
             rc = omx_omm_pr_dots(ms_hip_ctx(), G%dtype, int(n, c_int), int(m, c_int), G%dptr, G_p%dptr, G%ld, dots)
             if (rc /= 0) call omm_die('omm: '//omx_error_string())
             lambda_n = dots(1)
@@ -448,7 +524,11 @@ contains
 
   !> Point the CG working set at the FP64 state of this ip and at C_min.
   subroutine point_state()
-    pC => C_min
+    if (use_Cholesky) then
+      pC => C_Chl(ip)
+    else
+      pC => C_min
+    end if
     pHW => HW(ip)
     pHWd => HWd(ip)
     pHWdd => HWdd(ip)
@@ -499,9 +579,15 @@ contains
     call m_allocate(HC, n, m, label)
     call m_allocate(HG, n, m, label)
     call m_allocate(work1, n, n, label)
+    if (use_precon) then
+      call m_allocate(PG, n, m, label)
+      call m_allocate(PG_p, n, m, label)
+    end if
   end subroutine alloc_locals
 
   subroutine free_locals()
+    if (PG_p%is_initialized) call m_deallocate(PG_p)
+    if (PG%is_initialized) call m_deallocate(PG)
     if (work1%is_initialized) call m_deallocate(work1)
     if (HG%is_initialized) call m_deallocate(HG)
     if (HC%is_initialized) call m_deallocate(HC)
@@ -527,6 +613,18 @@ contains
         if (HW(q)%is_initialized) call m_deallocate(HW(q))
       end do
       deallocate(HW, HWd, HWdd, SW, SC, SG, work2, QW, CD)
+    end if
+    if (allocated(C_Chl)) then
+      do q = 1, size(C_Chl)
+        if (C_Chl(q)%is_initialized) call m_deallocate(C_Chl(q))
+      end do
+      deallocate(C_Chl)
+    end if
+    if (allocated(P)) then
+      do q = 1, size(P)
+        if (P(q)%is_initialized) call m_deallocate(P(q))
+      end do
+      deallocate(P)
     end if
     if (allocated(x_min)) deallocate(x_min)
     if (allocated(first_call)) deallocate(first_call)

@@ -150,6 +150,87 @@ contains
     omm_c_mat_from_bsr = h
   end function omm_c_mat_from_bsr
 
+  !> Complex device operator ('gzcsc') from 0-based BSR-4 block columns, 32 doubles per block.
+  integer(c_int) function omm_c_mat_from_bsr_z(nb, bcolptr, browidx, bval) bind(c, name='omm_c_mat_from_bsr_z')
+    integer(c_int), value :: nb
+    integer(c_int), intent(in) :: bcolptr(*), browidx(*)
+    real(c_double), intent(in) :: bval(*)
+    integer :: h, nbc, nnzb
+    h = new_handle()
+    if (h < 0) then
+      omm_c_mat_from_bsr_z = -1
+      return
+    end if
+    nbc = (nb + 3)/4
+    nnzb = bcolptr(nbc+1)
+    call m_register_gbsr_z(table(h), int(nb), bcolptr(1:nbc+1), browidx(1:max(nnzb,1)), bval(1:max(32*nnzb,1)))
+    omm_c_mat_from_bsr_z = h
+  end function omm_c_mat_from_bsr_z
+
+  !> Complex operator from a dense host complex matrix through m_copy (entries |a| > thr).
+  integer(c_int) function omm_c_mat_from_dense_z(nb, a, thr, label) bind(c, name='omm_c_mat_from_dense_z')
+    integer(c_int), value :: nb
+    type(c_ptr), value :: a
+    real(c_double), value :: thr
+    character(kind=c_char), intent(in) :: label(*)
+    complex(dp), pointer :: ap(:,:)
+    type(matrix) :: tmp
+    integer :: h
+    h = new_handle()
+    if (h < 0) then
+      omm_c_mat_from_dense_z = -1
+      return
+    end if
+    call c_f_pointer(a, ap, [nb, nb])
+    call m_allocate(tmp, int(nb), int(nb), 'szden')
+    tmp%zval = ap
+    call m_copy(table(h), tmp, cstr(label, 5), thr)
+    call m_deallocate(tmp)
+    omm_c_mat_from_dense_z = h
+  end function omm_c_mat_from_dense_z
+
+  integer(c_int) function omm_c_mat_upload_z(h, a) bind(c, name='omm_c_mat_upload_z')
+    integer(c_int), value :: h
+    type(c_ptr), value :: a
+    complex(dp), pointer :: ap(:,:)
+    omm_c_mat_upload_z = -1
+    if (.not. valid(h)) return
+    call c_f_pointer(a, ap, [table(h)%dim1, table(h)%dim2])
+    if (table(h)%is_device) then
+      call ms_upload(table(h), ap)
+    else
+      table(h)%zval = ap
+    end if
+    omm_c_mat_upload_z = 0
+  end function omm_c_mat_upload_z
+
+  integer(c_int) function omm_c_mat_download_z(h, a) bind(c, name='omm_c_mat_download_z')
+    integer(c_int), value :: h
+    type(c_ptr), value :: a
+    complex(dp), pointer :: ap(:,:)
+    omm_c_mat_download_z = -1
+    if (.not. valid(h)) return
+    call c_f_pointer(a, ap, [table(h)%dim1, table(h)%dim2])
+    if (table(h)%is_device) then
+      call ms_download(table(h), ap)
+    else
+      ap = table(h)%zval
+    end if
+    omm_c_mat_download_z = 0
+  end function omm_c_mat_download_z
+
+  integer(c_int) function omm_c_get_element_z(h, i, j, re, im) bind(c, name='omm_c_get_element_z')
+    integer(c_int), value :: h, i, j
+    real(c_double), intent(out) :: re, im
+    complex(dp) :: z
+    omm_c_get_element_z = -1
+    if (.not. valid(h)) return
+    call m_get_element(table(h), int(i), int(j), z)
+    re = real(z, dp)
+    im = aimag(z)
+    omm_c_get_element_z = 0
+  end function omm_c_get_element_z
+
   integer(c_int) function omm_c_mat_upload(h, a) bind(c, name='omm_c_mat_upload')
     integer(c_int), value :: h
     type(c_ptr), value :: a

@@ -230,6 +230,79 @@ contains
     stop 1
   end subroutine omm_die
 
+  !================================================================================================!
+  ! Dense factorisations of the Cholesky and preconditioned flavours (omm_ops.F90:225-1220).  The  !
+  ! device versions act on 'gdden' matrices; host storage needs LAPACK, which this build does not   !
+  ! link, and stops as the reference does when compiled without it.                                !
+  !================================================================================================!
+
+  !> C := U with C = U^T U, upper (m_factorize, omm_ops.F90:246-315, dpotrf 'u').
+  subroutine m_factorize(C, label)
+    type(matrix), intent(inout) :: C
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc, info
+    call dense_device(C, 'm_dfactorize')
+    rc = omx_potrf(ms_hip_ctx(), C%dtype, int(C%dim1, c_int), C%dptr, C%ld, info)
+    if (rc /= 0) call omm_die('m_dfactorize: '//omx_error_string())
+    if (info /= 0) call omm_die('m_dfactorize: error in dpotrf')
+  end subroutine m_factorize
+
+  !> C := U^-T C U^-1 with A = U from m_factorize, both triangles (m_reduce, omm_ops.F90:417-524).
+  subroutine m_reduce(A, C, label)
+    type(matrix), intent(in) :: A
+    type(matrix), intent(inout) :: C
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc
+    call dense_device(A, 'm_dreduce')
+    call dense_device(C, 'm_dreduce')
+    rc = omx_sygst(ms_hip_ctx(), C%dtype, int(C%dim1, c_int), C%dptr, C%ld, A%dptr, A%ld)
+    if (rc /= 0) call omm_die('m_dreduce: '//omx_error_string())
+  end subroutine m_reduce
+
+  !> C := C U^T (m_transform, omm_ops.F90:662-730, dtrmm('r','u','t','n')).
+  subroutine m_transform(A, C, label)
+    type(matrix), intent(in) :: A
+    type(matrix), intent(inout) :: C
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc
+    call dense_device(A, 'm_dtransform')
+    call dense_device(C, 'm_dtransform')
+    rc = omx_trmm(ms_hip_ctx(), C%dtype, 'r', 't', int(C%dim1, c_int), int(C%dim2, c_int), 1.0_dp, A%dptr, A%ld, &
+                  C%dptr, C%ld)
+    if (rc /= 0) call omm_die('m_dtransform: '//omx_error_string())
+  end subroutine m_transform
+
+  !> C := C U^-T (m_back_transform, omm_ops.F90:829-897, dtrsm('r','u','t','n')).
+  subroutine m_back_transform(A, C, label)
+    type(matrix), intent(in) :: A
+    type(matrix), intent(inout) :: C
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc
+    call dense_device(A, 'm_dback_transform')
+    call dense_device(C, 'm_dback_transform')
+    rc = omx_trsm(ms_hip_ctx(), C%dtype, 'r', 't', int(C%dim1, c_int), int(C%dim2, c_int), 1.0_dp, A%dptr, A%ld, &
+                  C%dptr, C%ld)
+    if (rc /= 0) call omm_die('m_dback_transform: '//omx_error_string())
+  end subroutine m_back_transform
+
+  !> C := C^-1 for the symmetric positive definite preconditioner (m_inverse, omm_ops.F90:992-1106).
+  subroutine m_inverse(C, label)
+    type(matrix), intent(inout) :: C
+    character(3), intent(in), optional :: label
+    integer(c_int) :: rc, info
+    call dense_device(C, 'm_dinverse')
+    rc = omx_inverse_spd(ms_hip_ctx(), C%dtype, int(C%dim1, c_int), C%dptr, C%ld, info)
+    if (rc /= 0) call omm_die('m_dinverse: '//omx_error_string())
+    if (info /= 0) call omm_die('m_dinverse: error in the factorisation')
+  end subroutine m_inverse
+
+  subroutine dense_device(A, where)
+    type(matrix), intent(in) :: A
+    character(*), intent(in) :: where
+    if (.not. A%is_device) call omm_die(where//': compile with LAPACK')
+    if (A%is_sparse .or. A%dtype /= OMX_F64) call omm_die(where//': invalid implementation')
+  end subroutine dense_device
+
   !> Adjoint of the converged energy for a Hamiltonian linear in parameters theta:
   !> grad(p) = sum_e J_p(e) * Dm(e), with Dm = dE/dH on H's pattern (the 'gdcsc' D_min of a
   !> converged omm() call, C^T (2I - SW) C) and the Jacobian J as device arrays seg (nparam+1),

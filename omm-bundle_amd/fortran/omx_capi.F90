@@ -77,6 +77,63 @@ module omx_capi
       integer(c_int), value :: nb
       type(c_ptr), intent(out) :: out
     end function
+    integer(c_int) function omx_bsr_from_csc_z(ctx, nb, colptr, rowidx, val, ptrbase, rowbase, out) &
+        bind(c, name='omx_bsr_from_csc_z')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, colptr, rowidx, val
+      integer(c_int), value :: nb, ptrbase, rowbase
+      type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function omx_bsr_create_z(ctx, nb, bcolptr, browidx, bval, out) bind(c, name='omx_bsr_create_z')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, bcolptr, browidx, bval
+      integer(c_int), value :: nb
+      type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function omx_set_element_z(ctx, dt, C, ldc, i, j, alpha_re, alpha_im, beta) &
+        bind(c, name='omx_set_element_z')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx, C
+      integer(c_int), value :: dt, i, j
+      integer(c_int64_t), value :: ldc
+      real(c_double), value :: alpha_re, alpha_im, beta
+    end function
+    integer(c_int) function omx_potrf(ctx, dt, n, A, lda, info) bind(c, name='omx_potrf')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, A
+      integer(c_int), value :: dt, n
+      integer(c_int64_t), value :: lda
+      integer(c_int), intent(out) :: info
+    end function
+    integer(c_int) function omx_trsm(ctx, dt, side, trans, m, n, alpha, U, ldu, B, ldb) bind(c, name='omx_trsm')
+      import :: c_int, c_ptr, c_int64_t, c_double, c_char
+      type(c_ptr), value :: ctx, U, B
+      integer(c_int), value :: dt, m, n
+      character(kind=c_char), value :: side, trans
+      real(c_double), value :: alpha
+      integer(c_int64_t), value :: ldu, ldb
+    end function
+    integer(c_int) function omx_trmm(ctx, dt, side, trans, m, n, alpha, U, ldu, B, ldb) bind(c, name='omx_trmm')
+      import :: c_int, c_ptr, c_int64_t, c_double, c_char
+      type(c_ptr), value :: ctx, U, B
+      integer(c_int), value :: dt, m, n
+      character(kind=c_char), value :: side, trans
+      real(c_double), value :: alpha
+      integer(c_int64_t), value :: ldu, ldb
+    end function
+    integer(c_int) function omx_sygst(ctx, dt, n, H, ldh, U, ldu) bind(c, name='omx_sygst')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, H, U
+      integer(c_int), value :: dt, n
+      integer(c_int64_t), value :: ldh, ldu
+    end function
+    integer(c_int) function omx_inverse_spd(ctx, dt, n, A, lda, info) bind(c, name='omx_inverse_spd')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, A
+      integer(c_int), value :: dt, n
+      integer(c_int64_t), value :: lda
+      integer(c_int), intent(out) :: info
+    end function
     integer(c_int) function omx_bsr_destroy(a) bind(c, name='omx_bsr_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: a
@@ -146,14 +203,14 @@ module omx_capi
       type(c_ptr), value :: ctx, A
       integer(c_int), value :: dt, n
       integer(c_int64_t), value :: lda
-      real(c_double), intent(out) :: out
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function omx_mm_trace(ctx, dt, m, n, A, lda, B, ldb, out) bind(c, name='omx_mm_trace')
       import :: c_int, c_ptr, c_int64_t, c_double
       type(c_ptr), value :: ctx, A, B
       integer(c_int), value :: dt, m, n
       integer(c_int64_t), value :: lda, ldb
-      real(c_double), intent(out) :: out
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function omx_set_element(ctx, dt, C, ldc, i, j, alpha, beta) bind(c, name='omx_set_element')
       import :: c_int, c_ptr, c_int64_t, c_double
@@ -167,7 +224,7 @@ module omx_capi
       type(c_ptr), value :: ctx, C
       integer(c_int), value :: dt, i, j
       integer(c_int64_t), value :: ldc
-      real(c_double), intent(out) :: out
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function omx_omm_coeff(ctx, dt, n, ld, HW, SW, HWd, SWd, HWdd, SWdd, coeff) &
         bind(c, name='omx_omm_coeff')

@@ -50,6 +50,15 @@ SIGNATURES = {
     "omx_bsr_get_pattern": (_i, [_vp, _vp, _vp]),
     "omx_sddmm_bsr": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp]),
     "omx_bsr_param_grad": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "omx_bsr_create_z": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "omx_potrf": (_i, [_vp, _i, _i, _vp, _i64, C.POINTER(_i)]),
+    "omx_trsm": (_i, [_vp, _i, C.c_char, C.c_char, _i, _i, _d, _vp, _i64, _vp, _i64]),
+    "omx_trmm": (_i, [_vp, _i, C.c_char, C.c_char, _i, _i, _d, _vp, _i64, _vp, _i64]),
+    "omx_sygst": (_i, [_vp, _i, _i, _vp, _i64, _vp, _i64]),
+    "omx_inverse_spd": (_i, [_vp, _i, _i, _vp, _i64, C.POINTER(_i)]),
+    "omx_bsr_from_csc_z": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, C.POINTER(_vp)]),
+    "omx_set_element_z": (_i, [_vp, _i, _vp, _i64, _i, _i, _d, _d, _d]),
+    "omx_lattice_bsr_k": (_i, [_i, _i, _i, C.c_uint64, _i, _vp, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_bsr_get_element": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
@@ -139,6 +148,20 @@ def lattice_bsr(L, which: int, seed: int = DEFAULT_SEED):
     return cp, ri, bv
 
 
+def lattice_bsr_k(L, which: int, phi, seed: int = DEFAULT_SEED):
+    """Complex Bloch operator H(k) / S(k): (bcolptr, browidx, bval[nnzb,16] complex128)."""
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    phi = np.ascontiguousarray(phi, np.float64)
+    nnzb = C.c_int64(0)
+    check(lib().omx_lattice_bsr_k(Lx, Ly, Lz, seed, which, _ptr(phi), C.byref(nnzb), None, None, None), "lattice k")
+    cp = np.zeros(Lx * Ly * Lz + 1, np.int32)
+    ri = np.zeros(nnzb.value, np.int32)
+    bv = np.zeros((nnzb.value, 16), np.complex128)
+    check(lib().omx_lattice_bsr_k(Lx, Ly, Lz, seed, which, _ptr(phi), C.byref(nnzb), _ptr(cp), _ptr(ri), _ptr(bv)),
+          "lattice k")
+    return cp, ri, bv
+
+
 def lattice_bsr_theta(L, theta):
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     theta = np.ascontiguousarray(theta, np.float64)
@@ -199,7 +222,7 @@ def bsr_get_element(Dp: "Bsr", i: int, j: int) -> float:
 def bsr_to_dense(nb: int, cp, ri, bv) -> np.ndarray:
     """Dense nb x nb matrix of a BSR-4 operator (host, for checks)."""
     nbp = 4 * (len(cp) - 1)
-    A = np.zeros((nbp, nbp))
+    A = np.zeros((nbp, nbp), dtype=np.result_type(np.asarray(bv).dtype, np.float64))
     for J in range(len(cp) - 1):
         for l in range(cp[J], cp[J + 1]):
             I = ri[l]
@@ -335,6 +358,20 @@ class Bsr:
         nnzb = C.c_int64(0)
         check(lib().omx_bsr_info(h, None, C.byref(nnzb), None), "bsr_info")
         self.nnzb = nnzb.value
+        return self
+
+    @classmethod
+    def complex(cls, ctx, nb, cp, ri, bv):
+        """Complex operator from BSR-4 arrays with complex128 values (nnzb, 16)."""
+        self = cls.__new__(cls)
+        self.ctx, self.nb = ctx, nb
+        cp = np.ascontiguousarray(cp, np.int32)
+        ri = np.ascontiguousarray(ri, np.int32)
+        bv = np.ascontiguousarray(bv, np.complex128)
+        h = C.c_void_p()
+        check(lib().omx_bsr_create_z(ctx.h, nb, _ptr(cp), _ptr(ri), _ptr(bv), C.byref(h)), "omx_bsr_create_z")
+        self.h = h
+        self.nnzb = len(ri)
         return self
 
     def values(self):

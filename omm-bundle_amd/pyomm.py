@@ -42,6 +42,11 @@ def lib():
         L.omm_c_tlog.argtypes = [_vp, _i]
         L.omm_c_ctx.restype = _vp
         L.omm_c_param_grad.argtypes = [_i, _i, _vp, _vp, _vp, _vp]
+        L.omm_c_mat_from_bsr_z.argtypes = [_i, _vp, _vp, _vp]
+        L.omm_c_mat_from_dense_z.argtypes = [_i, _vp, _d, C.c_char_p]
+        L.omm_c_mat_upload_z.argtypes = [_i, _vp]
+        L.omm_c_mat_download_z.argtypes = [_i, _vp]
+        L.omm_c_get_element_z.argtypes = [_i, _i, _i, C.POINTER(_d), C.POINTER(_d)]
         L.omm_c_get_element.argtypes = [_i, _i, _i, C.POINTER(_d)]
         _lib = L
     return _lib
@@ -89,13 +94,54 @@ class Mat:
             raise RuntimeError("omm_c_mat_from_bsr failed")
         return self
 
+    @property
+    def is_complex(self):
+        return self.label[1] in "zc"
+
+    @classmethod
+    def from_bsr_z(cls, nb, bcolptr, browidx, bval):
+        """Complex device operator ('gzcsc') from BSR-4 arrays with complex128 values (nnzb, 16)."""
+        self = cls.__new__(cls)
+        self.rows = self.cols = nb
+        self.label = "gzcsc"
+        cp = np.ascontiguousarray(bcolptr, np.int32)
+        ri = np.ascontiguousarray(browidx, np.int32)
+        v = np.ascontiguousarray(bval, np.complex128)
+        self.h = lib().omm_c_mat_from_bsr_z(nb, _p(cp), _p(ri), _p(v))
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_from_bsr_z failed")
+        return self
+
+    @classmethod
+    def from_dense_z(cls, a, thr=0.0, label="gzcsc"):
+        """Complex operator from a dense complex host matrix (m_copy, entries |a| > thr)."""
+        a = np.asfortranarray(a, np.complex128)
+        self = cls.__new__(cls)
+        self.rows = self.cols = a.shape[0]
+        self.label = label
+        self.h = lib().omm_c_mat_from_dense_z(a.shape[0], _p(a), thr, label.encode())
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_from_dense_z failed")
+        return self
+
     def upload(self, a):
+        if self.is_complex:
+            a = np.asfortranarray(a, np.complex128)
+            assert a.shape == (self.rows, self.cols)
+            if lib().omm_c_mat_upload_z(self.h, _p(a)) != 0:
+                raise RuntimeError("upload failed")
+            return
         a = np.asfortranarray(a, np.float64)
         assert a.shape == (self.rows, self.cols)
         if lib().omm_c_mat_upload(self.h, _p(a)) != 0:
             raise RuntimeError("upload failed")
 
     def download(self):
+        if self.is_complex:
+            out = np.zeros((self.rows, self.cols), np.complex128, order="F")
+            if lib().omm_c_mat_download_z(self.h, _p(out)) != 0:
+                raise RuntimeError("download failed")
+            return out
         out = np.zeros((self.rows, self.cols), order="F")
         if lib().omm_c_mat_download(self.h, _p(out)) != 0:
             raise RuntimeError("download failed")
@@ -105,6 +151,13 @@ class Mat:
         ld = C.c_int64(0)
         p = lib().omm_c_mat_dptr(self.h, C.byref(ld))
         return p, ld.value
+
+    def get_element_z(self, i, j):
+        """Complex element (i, j), 1-based (m_get_element with a complex result)."""
+        re, im = C.c_double(0), C.c_double(0)
+        if lib().omm_c_get_element_z(self.h, i, j, C.byref(re), C.byref(im)) != 0:
+            raise RuntimeError("get_element failed")
+        return complex(re.value, im.value)
 
     def get_element(self, i, j):
         """Element (i, j), 1-based (m_get_element; sparse device matrices included)."""

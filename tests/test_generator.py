@@ -58,3 +58,16 @@ def test_theta_generator_reproduces_shared_hopping():
     assert np.allclose(H[0:4, 4:8], T) and np.allclose(H[4:8, 0:4], T.T)
     assert np.allclose(H[0:4, 0:4], np.diag(th[16:20]))
     assert np.array_equal(H, H.T)
+
+
+def test_bloch_generator_phases():
+    # k = 0 gives the real operator; the Bloch operator is Hermitian and its spectrum over the
+    # phases of a 2-cell supercell is the spectrum of the doubled cell
+    L = 3
+    nb = 4 * L ** 3
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    cpk, rik, bvk = oh.lattice_bsr_k(L, 0, [0.0, 0.0, 0.0])
+    assert np.array_equal(cp, cpk) and np.array_equal(ri, rik)
+    assert np.array_equal(bvk.real, bv) and np.all(bvk.imag == 0)
+    Hk = oh.bsr_to_dense(nb, *oh.lattice_bsr_k(L, 0, [0.3, 0.0, 0.7]))
+    assert np.allclose(Hk, Hk.conj().T, atol=1e-15)
