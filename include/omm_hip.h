@@ -232,6 +232,30 @@ int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const
 /* W := W + x*(Wd^T + Wd) + x^2*Wdd in one pass (omm.F90:496-502 and 511-517). */
 int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const void *Wd, const void *Wdd, void *W);
 
+/* ==== batched k-point solver ============================================================== */
+
+/*
+ * Many complex128 systems of one shape whose operators share one BSR-4 block pattern (the
+ * k-points of one cell, H(k) and S(k) from omx_lattice_bsr_k) solved together by omm()'s
+ * flavour-0 CG loop (libOMM/src/omm.F90:350-552), all decisions (quartic line search,
+ * convergence, rescaling, Polak-Ribiere) taken on the device: one CG step of all systems is a
+ * fixed launch sequence, captured once in a hipGraph and replayed.  No counterpart in the
+ * reference, which loops over the k-points (np, ip) one omm() call at a time.
+ * hval, sval: nsys * nnzb * 32 doubles (system s at s*32*nnzb, H already shifted by -eta*S).
+ */
+typedef struct omx_zbatch_s *omx_zbatch;
+int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
+                      const double *hval, const double *sval, omx_zbatch *out);
+int omx_zbatch_destroy(omx_zbatch b);
+/* Coefficient matrices of all systems (host, complex128 (re, im), no x nb column-major each). */
+int omx_zbatch_set_c(omx_zbatch b, const double *C);
+int omx_zbatch_get_c(omx_zbatch b, double *C);
+/* Run up to max_steps CG steps (reading the convergence flags every check_every steps; use_graph
+ * replays one captured step).  Returns per system e_min + eta*TrQS, the step count icg and the
+ * converged flag (e_diff <= cg_tol, cg_tol < 0: 1e-9). */
+int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int check_every, int use_graph,
+                     double *e_min, int *icg, int *converged, int *steps_run);
+
 /* ==== dense factorisations (Cholesky and preconditioned flavours) ========================= */
 /* All FP64 column-major; U is upper triangular and only its upper triangle is read. */
 

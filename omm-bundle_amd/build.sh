@@ -9,7 +9,7 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 FC="${FC_OMM:-/opt/rocm/bin/amdflang}"
 OBJDIR="$HERE/_build"
 mkdir -p "$OBJDIR/f"
-SRC=("$HERE"/csrc/api.hip "$HERE"/csrc/gemm.hip "$HERE"/csrc/spmm.hip "$HERE"/csrc/elementwise.hip "$HERE"/csrc/adjoint.hip "$HERE"/csrc/zgemm.hip "$HERE"/csrc/dense_la.hip "$HERE"/csrc/lattice.cpp)
+SRC=("$HERE"/csrc/api.hip "$HERE"/csrc/gemm.hip "$HERE"/csrc/spmm.hip "$HERE"/csrc/elementwise.hip "$HERE"/csrc/adjoint.hip "$HERE"/csrc/zgemm.hip "$HERE"/csrc/dense_la.hip "$HERE"/csrc/batch.hip "$HERE"/csrc/lattice.cpp)
 FLAGS=(--offload-arch=gfx950 -O3 -fPIC -std=c++17 -I"$ROOT/include" -I"$HERE/csrc"
        -Xclang -target-feature -Xclang -packed-fp32-ops)
 pids=()

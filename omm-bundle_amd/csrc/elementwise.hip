@@ -42,6 +42,8 @@ __device__ inline void block_reduce_store(double (&v)[NV], double *__restrict__ 
 template <int NV>
 __global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const double *__restrict__ partial,
                                                            double *__restrict__ out) {
+    partial += (int64_t)blockIdx.x * nblocks * NV;  // one workgroup per system in batched use
+    out += blockIdx.x * NV;
     double v[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.0;
@@ -195,8 +197,12 @@ __global__ __launch_bounds__(EW_NT) void k_dot_partial(int m, int n, const T *__
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_pr_partial(int64_t total_m, int m, const T *__restrict__ G,
                                                       const T *__restrict__ Gp, int64_t ld,
-                                                      double *__restrict__ partial) {
+                                                      double *__restrict__ partial, int64_t sG = 0) {
     typedef El<T> E;
+    // batched use: blockIdx.y selects the system
+    G += blockIdx.y * sG;
+    Gp += blockIdx.y * sG;
+    partial += 2 * (int64_t)blockIdx.y * gridDim.x;
     double v[2] = {0.0, 0.0};
     for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total_m; q += (int64_t)gridDim.x * EW_NT) {
         int64_t i = q % m, j = q / m;
@@ -231,8 +237,20 @@ template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, const T *__restrict__ HW,
                                                          const T *__restrict__ SW, const T *__restrict__ HWd,
                                                          const T *__restrict__ SWd, const T *__restrict__ HWdd,
-                                                         const T *__restrict__ SWdd, double *__restrict__ partial) {
+                                                         const T *__restrict__ SWdd, double *__restrict__ partial,
+                                                         int64_t sW = 0) {
     typedef El<T> E;
+    // batched use: blockIdx.y selects the system (all six matrices offset by sW elements)
+    {
+        const int64_t o = blockIdx.y * sW;
+        HW += o;
+        SW += o;
+        HWd += o;
+        SWd += o;
+        HWdd += o;
+        SWdd += o;
+        partial += 13 * (int64_t)blockIdx.y * gridDim.x;
+    }
     __shared__ double swdT[TT][TT + 1];
     __shared__ double swdTi[El<T>::cplx ? TT : 1][TT + 1];
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
@@ -279,7 +297,10 @@ __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, con
 }
 
 __global__ void k_coeff_final(int nblocks, const double *__restrict__ partial, double *__restrict__ out) {
-    // One workgroup: deterministic per-term sums, then the closed-form coefficients.
+    // One workgroup (per system in batched use): deterministic per-term sums, then the closed-form
+    // coefficients.
+    partial += 13 * (int64_t)blockIdx.x * nblocks;
+    out += 18 * blockIdx.x;
     double v[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) v[q] = 0.0;
@@ -377,6 +398,17 @@ inline bool with_type(int dt, F &&f) {
         default: return false;
     }
 }
+
+// Batched complex128 reductions over nbatch systems (used by the batched solver): coefficients to
+// out[18*s ...] (5 coefficients then the 13 traces) and Polak-Ribiere dots to out2[2*s ...], on
+// the device, no readback.
+int launch_coeff_z_batched(omx_ctx ctx, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
+                           const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
+                           double *partial, double *out);
+int launch_pr_z_batched(omx_ctx ctx, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp, int nbatch,
+                        double *partial, double *out2);
+size_t coeff_partial_doubles(int n, int nbatch);
+size_t pr_partial_doubles(int n, int m, int nbatch);
 
 inline int grid_for(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + EW_NT - 1) / EW_NT, 65536)); }
 
@@ -627,3 +659,41 @@ int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const voi
 }
 
 }  // extern "C"
+
+namespace omx {
+
+size_t coeff_partial_doubles(int n, int nbatch) {
+    const int64_t tiles = (n + TT - 1) / TT;
+    return (size_t)13 * tiles * tiles * nbatch;
+}
+size_t pr_partial_doubles(int n, int m, int nbatch) {
+    return (size_t)2 * std::min(2048, grid_for((int64_t)n * m)) * nbatch;
+}
+
+int launch_coeff_z_batched(omx_ctx ctx, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
+                           const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
+                           double *partial, double *out) {
+    typedef Cx<double> T;
+    const int tiles = (n + TT - 1) / TT;
+    const int blocks = tiles * tiles;
+    hipLaunchKernelGGL(k_coeff_partial<T>, dim3((unsigned)blocks, nbatch), dim3(EW_NT), 0, ctx->stream, n, ldm,
+                       (const T *)HW, (const T *)SW, (const T *)HWd, (const T *)SWd, (const T *)HWdd, (const T *)SWdd,
+                       partial, sW);
+    hipLaunchKernelGGL(k_coeff_final, dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int launch_pr_z_batched(omx_ctx ctx, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp, int nbatch,
+                        double *partial, double *out2) {
+    typedef Cx<double> T;
+    const int64_t total = (int64_t)n * m;
+    const int blocks = std::min(2048, grid_for(total));
+    hipLaunchKernelGGL(k_pr_partial<T>, dim3(blocks, nbatch), dim3(EW_NT), 0, ctx->stream, total, n, (const T *)G,
+                       (const T *)Gp, ld, partial, sG);
+    hipLaunchKernelGGL((k_reduce_partials<2>), dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out2);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+}  // namespace omx

@@ -213,9 +213,17 @@ __global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4_z(int no, int nb, int nb
                                                         const R *__restrict__ X, int64_t ldx,
                                                         const int *__restrict__ colptr, const int *__restrict__ rowidx,
                                                         const double *__restrict__ val, double beta,
-                                                        R *__restrict__ Y, int64_t ldy, int nbands, int jgroups) {
+                                                        R *__restrict__ Y, int64_t ldy, int nbands, int jgroups,
+                                                        int64_t sX, int64_t sY, int64_t sV,
+                                                        const int *__restrict__ active) {
     constexpr int ROWS = 64 * VEC;
     constexpr int NB = 4;
+    // batched use: blockIdx.y selects the system (operator values, X and Y offset by its strides)
+    const int bsys = blockIdx.y;
+    if (active && !active[bsys]) return;
+    X += bsys * sX;
+    Y += bsys * sY;
+    val += bsys * sV;
     const int w = blockIdx.x;
     int band, jg;
     if ((nbands & 7) == 0) {
@@ -317,13 +325,31 @@ static int run_spmm_z(omx_ctx ctx, int no, int nb, double alpha, const R *X, int
               2.0 * sizeof(R) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (32.0 * 8 + 4) +
                   4.0 * (A->nbc + 1));
     hipLaunchKernelGGL((k_spmm_bsr4_z<R, VEC, WPG>), dim3((unsigned)total), dim3(64 * WPG), 0, ctx->stream, no, nb,
-                       A->nbc, alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nbands, jgroups);
+                       A->nbc, alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nbands, jgroups,
+                       (int64_t)0, (int64_t)0, (int64_t)0, (const int *)nullptr);
     prof_end(ctx, OMX_K_SPMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// Batched complex128 SpMM over nbatch systems sharing one block pattern: Y_s := X_s * A_s (beta 0),
+// A_s values at val + s*sV (doubles), X_s, Y_s at s*sX, s*sY (complex elements).
+int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colptr, const int *rowidx,
+                          const double *val, int64_t sV, const void *X, int64_t ldx, int64_t sX, void *Y, int64_t ldy,
+                          int64_t sY, int nbatch, const int *active) {
+    constexpr int VEC = 1, ROWS = 64 * VEC, WPG = 4;
+    const int nbands = (no + ROWS - 1) / ROWS;
+    const int jgroups = (nbc + WPG - 1) / WPG;
+    const int64_t total = (int64_t)nbands * jgroups;
+    OMX_REQUIRE(total < (1LL << 31) && nbatch < 65536, "batched spmm: grid too large");
+    hipLaunchKernelGGL((k_spmm_bsr4_z<double, VEC, WPG>), dim3((unsigned)total, nbatch), dim3(64 * WPG), 0,
+                       ctx->stream, no, nb, nbc, 1.0, (const double *)X, ldx, colptr, rowidx, val, 0.0, (double *)Y,
+                       ldy, nbands, jgroups, 2 * sX, 2 * sY, sV, active);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
 
 }  // namespace omx
 

@@ -46,6 +46,13 @@ struct ZPrec<float> {
 };
 
 constexpr int ZBM = 64, ZBN = 128, ZBK = 8, ZNT = 256, ZPAD = 4;
+
+// Batched products: blockIdx.y selects the system; operand and result offsets are the system index
+// times these strides (complex elements); systems with active[s] == 0 are skipped.
+struct ZBatch {
+    int64_t sA = 0, sB = 0, sC = 0;
+    const int *active = nullptr;
+};
 constexpr int ZWM = ZBM / 2, ZWN = ZBN / 2, ZFBM = ZWM / 16, ZFBN = ZWN / 16;
 
 // One BK slice of op(A) (BM x BK, "rows" r = i) or op(B)^T (BN x BK, r = j) into registers.
@@ -101,13 +108,21 @@ __global__ __launch_bounds__(ZNT) void k_zgemm(int m, int n, int k, double alpha
                                                const typename ZPrec<R>::cplx *__restrict__ B2, int64_t ldb,
                                                double beta0, const typename ZPrec<R>::cplx *__restrict__ C0,
                                                double beta, typename ZPrec<R>::cplx *__restrict__ C, int64_t ldc,
-                                               int k_per_split, R *__restrict__ partial) {
+                                               int k_per_split, R *__restrict__ partial, ZBatch bat) {
     typedef typename ZPrec<R>::accv AV;
     typedef typename ZPrec<R>::cplx C2;
     constexpr int LA = ZBK * (ZBM + ZPAD), LB = ZBK * (ZBN + ZPAD);
     constexpr int STAGE = 2 * (LA + LB);  // re and im planes of A and B
     __shared__ R smem[2 * STAGE];
 
+    const int bsys = blockIdx.y;
+    if (bat.active && !bat.active[bsys]) return;  // whole workgroup, before any barrier
+    A1 += bsys * bat.sA;
+    A2 += bsys * bat.sA;
+    B1 += bsys * bat.sB;
+    B2 += bsys * bat.sB;
+    C += bsys * bat.sC;
+    if (C0) C0 += bsys * bat.sC;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int tiles_m = (m + ZBM - 1) / ZBM, tiles_n = (n + ZBN - 1) / ZBN;
@@ -265,13 +280,13 @@ __global__ void k_zsplitk_reduce(int m, int n, int splits, const R *__restrict__
 template <typename R, bool TRA, bool CJA, bool TRB, bool CJB, int NPAIR>
 static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void *A1, const void *A2, int64_t lda,
                      const void *B1, const void *B2, int64_t ldb, double beta0, const void *C0, double beta, void *C,
-                     int64_t ldc) {
+                     int64_t ldc, int nbatch = 1, ZBatch bat = ZBatch()) {
     typedef typename ZPrec<R>::cplx C2;
     const int tiles = ((m + ZBM - 1) / ZBM) * ((n + ZBN - 1) / ZBN);
     const int ktot = NPAIR * k;
     int splits = 1;
     const int target = 2 * ctx->num_cus;
-    if (tiles < target) {
+    if (nbatch == 1 && tiles < target) {
         const int max_splits = std::max(1, ktot / (4 * ZBK));
         splits = std::max(1, std::min(std::min(max_splits, (target + tiles - 1) / tiles), 64));
     }
@@ -286,12 +301,12 @@ static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void 
     }
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_GEMM, &ev);
-    prof_work(ctx, OMX_K_GEMM, 8.0 * m * n * (double)ktot,
-              2.0 * sizeof(R) * ((double)NPAIR * ((double)m * k + (double)k * n) +
-                                 (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
-    hipLaunchKernelGGL((k_zgemm<R, TRA, CJA, TRB, CJB, NPAIR>), dim3(tiles, 1, splits), dim3(ZNT), 0, ctx->stream, m,
-                       n, k, alpha, (const C2 *)A1, (const C2 *)A2, lda, (const C2 *)B1, (const C2 *)B2, ldb, beta0,
-                       (const C2 *)C0, beta, (C2 *)C, ldc, kps, partial);
+    prof_work(ctx, OMX_K_GEMM, nbatch * 8.0 * m * n * (double)ktot,
+              nbatch * 2.0 * sizeof(R) * ((double)NPAIR * ((double)m * k + (double)k * n) +
+                                          (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
+    hipLaunchKernelGGL((k_zgemm<R, TRA, CJA, TRB, CJB, NPAIR>), dim3(tiles, nbatch, splits), dim3(ZNT), 0, ctx->stream,
+                       m, n, k, alpha, (const C2 *)A1, (const C2 *)A2, lda, (const C2 *)B1, (const C2 *)B2, ldb, beta0,
+                       (const C2 *)C0, beta, (C2 *)C, ldc, kps, partial, bat);
     if (splits > 1) {
         const int64_t total = (int64_t)m * n;
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -333,6 +348,42 @@ int launch_zgemm(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, d
     if (dt == OMX_C64) return zdispatch<float>(ctx, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     set_error("complex product: unsupported element type %d", dt);
     return OMX_ERR_UNSUPPORTED;
+}
+
+// Batched complex128 products over nbatch systems with element strides sA, sB, sC (result and C0).
+int launch_zgemm_batched(omx_ctx ctx, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+                         int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, double beta, void *C,
+                         int64_t ldc, int64_t sC, int nbatch, const int *active) {
+    ZBatch bat;
+    bat.sA = sA;
+    bat.sB = sB;
+    bat.sC = sC;
+    bat.active = active;
+    const bool ta = (opA != 'n' && opA != 'N'), ca = (opA == 'c' || opA == 'C');
+    const bool tb = (opB != 'n' && opB != 'N'), cb = (opB == 'c' || opB == 'C');
+    if (!ta && tb && cb)  // the Gram products X Y^H
+        return run_zgemm<double, false, false, true, true, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr,
+                                                              beta, C, ldc, nbatch, bat);
+    if (!ta && !tb)
+        return run_zgemm<double, false, false, false, false, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0,
+                                                                nullptr, beta, C, ldc, nbatch, bat);
+    (void)ca;
+    set_error("batched complex product: op (%c,%c) not available", opA, opB);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, const void *A1, const void *A2,
+                              int64_t lda, int64_t sA, const void *B1, const void *B2, int64_t ldb, int64_t sB,
+                              double beta0, const void *C0, double beta, void *C, int64_t ldc, int64_t sC, int nbatch,
+                              const int *active) {
+    if (k % ZBK != 0) return OMX_ERR_UNSUPPORTED;
+    ZBatch bat;
+    bat.sA = sA;
+    bat.sB = sB;
+    bat.sC = sC;
+    bat.active = active;
+    return run_zgemm<double, false, false, false, false, 2>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
+                                                            beta, C, ldc, nbatch, bat);
 }
 
 int launch_zgemm_pair(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,

@@ -52,6 +52,11 @@ SIGNATURES = {
     "omx_bsr_param_grad": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "omx_bsr_create_z": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
     "omx_potrf": (_i, [_vp, _i, _i, _vp, _i64, C.POINTER(_i)]),
+    "omx_zbatch_create": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "omx_zbatch_destroy": (_i, [_vp]),
+    "omx_zbatch_set_c": (_i, [_vp, _vp]),
+    "omx_zbatch_get_c": (_i, [_vp, _vp]),
+    "omx_zbatch_solve": (_i, [_vp, _d, _d, _i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
     "omx_trsm": (_i, [_vp, _i, C.c_char, C.c_char, _i, _i, _d, _vp, _i64, _vp, _i64]),
     "omx_trmm": (_i, [_vp, _i, C.c_char, C.c_char, _i, _i, _d, _vp, _i64, _vp, _i64]),
     "omx_sygst": (_i, [_vp, _i, _i, _vp, _i64, _vp, _i64]),
@@ -480,3 +485,48 @@ def get_element(ctx, M: DMat, i, j) -> float:
     out = C.c_double(0)
     check(lib().omx_get_element(ctx.h, M.dt, M.p, M.ld, i, j, C.byref(out)), "get_element")
     return out.value
+
+
+class ZBatch:
+    """Batched complex128 k-point solver (omx_zbatch_*): nsys systems sharing one block pattern."""
+
+    def __init__(self, ctx: Ctx, no: int, nb: int, cp, ri, hvals, svals):
+        """hvals, svals: lists (one per system) of complex (nnzb, 16) block values, H already shifted."""
+        self.ctx, self.no, self.nb, self.nsys = ctx, no, nb, len(hvals)
+        cp = np.ascontiguousarray(cp, np.int32)
+        ri = np.ascontiguousarray(ri, np.int32)
+        hv = np.ascontiguousarray(np.stack(hvals), np.complex128)
+        sv = np.ascontiguousarray(np.stack(svals), np.complex128)
+        h = C.c_void_p()
+        check(lib().omx_zbatch_create(ctx.h, self.nsys, no, nb, _ptr(cp), _ptr(ri), _ptr(hv), _ptr(sv), C.byref(h)),
+              "omx_zbatch_create")
+        self.h = h
+
+    def set_c(self, Cs):
+        a = np.ascontiguousarray(np.stack([np.asfortranarray(c, np.complex128).ravel(order="F") for c in Cs]))
+        check(lib().omx_zbatch_set_c(self.h, _ptr(a)), "zbatch_set_c")
+
+    def get_c(self):
+        a = np.zeros((self.nsys, self.no * self.nb), np.complex128)
+        check(lib().omx_zbatch_get_c(self.h, _ptr(a)), "zbatch_get_c")
+        return [a[s].reshape((self.no, self.nb), order="F") for s in range(self.nsys)]
+
+    def solve(self, eta=0.0, cg_tol=1e-9, max_steps=10000, check_every=10, use_graph=True):
+        e = np.zeros(self.nsys)
+        icg = np.zeros(self.nsys, np.int32)
+        conv = np.zeros(self.nsys, np.int32)
+        steps = C.c_int(0)
+        check(lib().omx_zbatch_solve(self.h, eta, cg_tol, max_steps, check_every, int(use_graph), _ptr(e), _ptr(icg),
+                                     _ptr(conv), C.byref(steps)), "zbatch_solve")
+        return {"e_min": e, "icg": icg, "converged": conv.astype(bool), "steps": steps.value}
+
+    def free(self):
+        if self.h:
+            lib().omx_zbatch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
