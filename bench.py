@@ -163,6 +163,9 @@ def main():
     ap.add_argument("--no-lowp", action="store_true", help="skip the BF16 / FP32 step-rate runs")
     ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
     ap.add_argument("--ttc-log", default=None, help="write the per-step energies of the --ttc solves (JSON) here")
+    ap.add_argument("--kpoints", action="store_true",
+                    help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
+    ap.add_argument("--kpoint-steps", type=int, default=4, help="CG steps timed per k-point in --kpoints mode")
     args = ap.parse_args()
 
     if args.compute_eta:
@@ -176,6 +179,10 @@ def main():
 
     import ommhip as oh
     import pyomm as F
+
+    if args.kpoints:
+        print(json.dumps(kpoint_bench(oh, F, args.kpoint_steps)), flush=True)
+        return 0
 
     L = args.L
     nb, no = 4 * L ** 3, 2 * L ** 3
@@ -306,6 +313,71 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
                        "mixed": {"e": rm["e_log"].tolist(), "t": (rm["t_log"] - rm["t0"]).tolist(),
                                  "icg_low_precision": rm["icg_lowp"]}}, f)
     return out
+
+
+def kpoint_bench(oh, F, steps, L=10, nk1=4, eta=0.05):
+    """CG steps of the 64 k-points of the 10x10x10 cell (n = 4000, m = 2000, complex128): one
+    batched solve (all systems per graph-captured step) against one omm() call per k-point, both
+    timed over the same number of CG steps per system, from the same resident inputs."""
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    g = [i / nk1 for i in range(nk1)]
+    phis = [(a, b, c) for a in g for b in g for c in g]
+    t = time.time()
+    hv, sv = [], []
+    for phi in phis:
+        cp, ri, h = oh.lattice_bsr_k(L, 0, phi)
+        _, _, sk = oh.lattice_bsr_k(L, 1, phi)
+        hv.append(h - eta * sk)
+        sv.append(sk)
+    C0 = oh.random_c0(no, nb).astype(np.complex128)
+    log(f"bench kpoints: {len(phis)} systems n={nb} m={no}, inputs in {time.time() - t:.1f}s")
+    F.setup(0)
+    ctx = oh.Ctx(0)
+    B = oh.ZBatch(ctx, no, nb, cp, ri, hv, sv)
+    res = {"systems": len(phis), "n": nb, "m": no, "dtype": "c128", "steps_per_system": steps}
+
+    def run_batch(nsteps):
+        B.set_c([C0] * len(phis))
+        ctx.sync()
+        t0 = time.perf_counter()
+        B.solve(eta=eta, cg_tol=1e-14, max_steps=nsteps, check_every=nsteps, use_graph=True)
+        return time.perf_counter() - t0
+
+    run_batch(1)  # graph capture and warm-up
+    t1 = run_batch(1)
+    tn = run_batch(1 + steps)
+    res["batched_s_per_step_all_systems"] = (tn - t1) / steps
+    log(f"bench kpoints: batched {res['batched_s_per_step_all_systems'] * 1e3:.1f} ms per CG step of all systems")
+    B.free()
+    # sequential: one omm() call per k-point (np = 64), the same number of steps each
+    Hm = [F.Mat.from_bsr_z(nb, cp, ri, h) for h in hv]
+    Sm = [F.Mat.from_bsr_z(nb, cp, ri, sk) for sk in sv]
+    Cm = F.Mat(no, nb, "gzden")
+
+    def run_seq(nsteps):
+        F.set_option("max_iter", nsteps)
+        tot = 0.0
+        for k in range(len(phis)):
+            Cm.upload(C0)
+            F.sync()
+            t0 = time.perf_counter()
+            F.omm(nb, no, Hm[k], Sm[k], Cm, init_C=True, eta=eta, cg_tol=1e-14, storage="gzden", np_=len(phis),
+                  ip=k + 1, dealloc=(k == len(phis) - 1))
+            F.sync()
+            tot += time.perf_counter() - t0
+        F.set_option("max_iter", 0)
+        return tot
+
+    run_seq(1)
+    s1 = run_seq(1)
+    sn = run_seq(1 + steps)
+    res["sequential_s_per_step_all_systems"] = (sn - s1) / steps
+    res["batched_speedup"] = res["sequential_s_per_step_all_systems"] / res["batched_s_per_step_all_systems"]
+    log(f"bench kpoints: sequential {res['sequential_s_per_step_all_systems'] * 1e3:.1f} ms per CG step of all "
+        f"systems; batched speedup {res['batched_speedup']:.2f}x")
+    for M in Hm + Sm + [Cm]:
+        M.free()
+    return {"kpoint_batch": res}
 
 
 if __name__ == "__main__":
