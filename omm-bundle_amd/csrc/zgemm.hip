@@ -280,13 +280,14 @@ __global__ void k_zsplitk_reduce(int m, int n, int splits, const R *__restrict__
 template <typename R, bool TRA, bool CJA, bool TRB, bool CJB, int NPAIR>
 static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void *A1, const void *A2, int64_t lda,
                      const void *B1, const void *B2, int64_t ldb, double beta0, const void *C0, double beta, void *C,
-                     int64_t ldc, int nbatch = 1, ZBatch bat = ZBatch()) {
+                     int64_t ldc, int nbatch = 1, ZBatch bat = ZBatch(), bool allow_split = true) {
     typedef typename ZPrec<R>::cplx C2;
     const int tiles = ((m + ZBM - 1) / ZBM) * ((n + ZBN - 1) / ZBN);
     const int ktot = NPAIR * k;
     int splits = 1;
     const int target = 2 * ctx->num_cus;
-    if (nbatch == 1 && tiles < target) {
+    // batched launches never split K: no workspace, so the launch can be graph-captured
+    if (allow_split && nbatch == 1 && tiles < target) {
         const int max_splits = std::max(1, ktot / (4 * ZBK));
         splits = std::max(1, std::min(std::min(max_splits, (target + tiles - 1) / tiles), 64));
     }
@@ -363,10 +364,10 @@ int launch_zgemm_batched(omx_ctx ctx, char opA, char opB, int m, int n, int k, d
     const bool tb = (opB != 'n' && opB != 'N'), cb = (opB == 'c' || opB == 'C');
     if (!ta && tb && cb)  // the Gram products X Y^H
         return run_zgemm<double, false, false, true, true, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr,
-                                                              beta, C, ldc, nbatch, bat);
+                                                              beta, C, ldc, nbatch, bat, false);
     if (!ta && !tb)
         return run_zgemm<double, false, false, false, false, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0,
-                                                                nullptr, beta, C, ldc, nbatch, bat);
+                                                                nullptr, beta, C, ldc, nbatch, bat, false);
     (void)ca;
     set_error("batched complex product: op (%c,%c) not available", opA, opB);
     return OMX_ERR_UNSUPPORTED;
@@ -383,7 +384,7 @@ int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, co
     bat.sC = sC;
     bat.active = active;
     return run_zgemm<double, false, false, false, false, 2>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
-                                                            beta, C, ldc, nbatch, bat);
+                                                            beta, C, ldc, nbatch, bat, false);
 }
 
 int launch_zgemm_pair(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,
