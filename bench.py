@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--kpoints", action="store_true",
                     help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
     ap.add_argument("--kpoint-steps", type=int, default=4, help="CG steps timed per k-point in --kpoints mode")
+    ap.add_argument("--kpoint-L", type=int, default=10, help="cell edge of the k-point systems (n = 4 L^3)")
     args = ap.parse_args()
 
     if args.compute_eta:
@@ -181,7 +182,7 @@ def main():
     import pyomm as F
 
     if args.kpoints:
-        print(json.dumps(kpoint_bench(oh, F, args.kpoint_steps)), flush=True)
+        print(json.dumps(kpoint_bench(oh, F, args.kpoint_steps, L=args.kpoint_L)), flush=True)
         return 0
 
     L = args.L
@@ -271,6 +272,20 @@ def main():
     return 0
 
 
+EIGS_FILE = "tests/golden/lattice_eigs.json"
+
+
+def exact_eigs(nb):
+    """Tabulated exact eigenvalue data of the seeded lattice with nb basis functions, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), EIGS_FILE)
+    if not os.path.exists(path):
+        return None
+    for v in json.load(open(path)).values():
+        if v.get("n") == nb:
+            return v
+    return None
+
+
 def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None):
     H, S, Cm = mats
     out = {"cg_tol": tol}
@@ -297,11 +312,21 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
     oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
     fro = ctypes.c_double(0)
     oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
+    fro_ref = ctypes.c_double(0)
+    oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p1, ld, p1, ld, ctypes.byref(fro_ref)), "fro")
     out["mixed"] = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
                     "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
                     "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
                     "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
+                    "density_rel_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0) / max(fro_ref.value, 1e-300))),
                     "speedup_vs_fp64": t64 / tmx}
+    # both results against the exact eigenvalue sum of this lattice when it is tabulated
+    exact = exact_eigs(nb)
+    if exact is not None:
+        ex = exact["sum_lowest_m"]
+        out["exact"] = {"sum_lowest_m": ex, "gap_at_m": exact["gap_at_m"], "source": EIGS_FILE,
+                        "fp64_energy_rel_err": abs(r["e_min"] - ex) / abs(ex),
+                        "mixed_energy_rel_err": abs(rm["e_min"] - ex) / abs(ex)}
     F.set_option("progress", 0)
     log(f"bench: ttc mixed {tmx:.2f}s icg={rm['icg']} (low precision {rm['icg_lowp']}) e={rm['e_min']:.12f} "
         f"speedup {t64 / tmx:.2f}x")
@@ -316,7 +341,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
 
 
 def kpoint_bench(oh, F, steps, L=10, nk1=4, eta=0.05):
-    """CG steps of the 64 k-points of the 10x10x10 cell (n = 4000, m = 2000, complex128): one
+    """CG steps of the 64 k-points of the L x L x L cell (L = 10: n = 4000, m = 2000, complex128): one
     batched solve (all systems per graph-captured step) against one omm() call per k-point, both
     timed over the same number of CG steps per system, from the same resident inputs."""
     nb, no = 4 * L ** 3, 2 * L ** 3
@@ -334,7 +359,7 @@ def kpoint_bench(oh, F, steps, L=10, nk1=4, eta=0.05):
     F.setup(0)
     ctx = oh.Ctx(0)
     B = oh.ZBatch(ctx, no, nb, cp, ri, hv, sv)
-    res = {"systems": len(phis), "n": nb, "m": no, "dtype": "c128", "steps_per_system": steps}
+    res = {"systems": len(phis), "L": L, "n": nb, "m": no, "dtype": "c128", "steps_per_system": steps}
 
     def run_batch(nsteps):
         B.set_c([C0] * len(phis))

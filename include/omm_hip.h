@@ -170,16 +170,31 @@ int omx_bsr_param_grad(omx_ctx ctx, omx_bsr Dp, int nparam, const int64_t *seg, 
  */
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
              int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
-/* SpMM launch shape of later products on this context: blocks whose loads are issued together
- * (1, 4 or 8; 0 = default 8) and block columns (one wave each) per workgroup (1, 2 or 4;
- * 0 = default 4).  Tuning hook; results do not depend on it. */
-int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper);
+/* SpMM launch shape of later real products on this context: blocks whose loads are issued
+ * together (batch 1, 4 or 8), waves stacked in rows per workgroup (wpg 1, 2 or 4), block columns
+ * walked per workgroup (jper >= 1) and XCD map (mode 0: each XCD owns a contiguous eighth of the
+ * rows; 1: each XCD walks one row chunk at a time).  batch = 0 returns to the default shape (or
+ * the autotuner's).  Tuning hook; results do not depend on it. */
+int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode);
+/* Attach to an operator the order in which SpMMs visit its block columns (a permutation of
+ * 0..nbc-1, host array copied to the device; NULL clears it).  A schedule hint only: every
+ * output column is computed the same way in any order (no counterpart in the reference). */
+int omx_bsr_set_walk(omx_bsr A, const int *order);
 /* Force the GEMM tile configuration of later products on this context (-1 restores the choice
  * by shape) for the Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
  * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256 (FP64 products); 4 = 256x256 with 8 waves
  * (FP32 and BF16 products on FP32 storage).  A configuration that does not apply to a product
  * falls back to 0.  Used by the tile autotuner; results do not depend on it. */
 int omx_gemm_set_tile(omx_ctx ctx, int cfg);
+/* Autotuner (project-written; no counterpart in the reference): with enable != 0, the first
+ * product or SpMM of each shape on this context times every applicable tile configuration / SpMM
+ * launch shape into a scratch result and keeps the fastest for that shape.  Tuning allocates and
+ * synchronises: do not enable it while a stream is being captured.  Forced shapes
+ * (omx_gemm_set_tile, omx_spmm_set_tile) take precedence. */
+int omx_autotune(omx_ctx ctx, int enable);
+/* The choices made so far, one line per shape, NUL-terminated into buf (len bytes); returns the
+ * full text length. */
+int omx_autotune_report(omx_ctx ctx, char *buf, int64_t len);
 /* C := alpha*op(A) + beta*C  (m_add, MatrixSwitch.F90:1731-1902; MatrixSwitch_m_add.F90:95-175). */
 int omx_add(omx_ctx ctx, int dt, char opA, int m, int n, double alpha, const void *A, int64_t lda, double beta,
             void *C, int64_t ldc);
@@ -323,6 +338,9 @@ int omx_lattice_theta_jacobian(int Lx, int Ly, int Lz, int64_t *nent, int64_t *s
 /* Initial parameters: hopping U(-1,1) from the seed, onsite levels (-6, -6, 6, 6) (half filling
  * then has a gap of about 3.8). */
 int omx_lattice_theta0(uint64_t seed, double *theta);
+/* Tiled walk order of the lattice sites (x-y tiles of tx x ty, each swept through z), a
+ * permutation of 0..Lx*Ly*Lz-1 for omx_bsr_set_walk on the lattice operators. */
+int omx_lattice_walk(int Lx, int Ly, int Lz, int tx, int ty, int *order);
 /* C0(i,k) = U(-0.5,0.5)/sqrt(nb), no x nb column-major. */
 int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc);
 /* The underlying counter-based uniform variate in [0,1). */

@@ -468,6 +468,7 @@ int omx_bsr_destroy(omx_bsr a) {
     if (a->d_colptr) (void)hipFree(a->d_colptr);
     if (a->d_rowidx) (void)hipFree(a->d_rowidx);
     if (a->d_val) (void)hipFree(a->d_val);
+    if (a->d_walk) (void)hipFree(a->d_walk);
     delete a;
     return OMX_OK;
 }
@@ -507,6 +508,34 @@ int omx_bsr_get_pattern(omx_bsr a, int *bcolptr, int *browidx) {
                                      a->ctx->stream));
     OMX_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
     return OMX_OK;
+}
+
+// ---- autotuner -------------------------------------------------------------------------------
+
+int omx_autotune(omx_ctx ctx, int enable) {
+    OMX_REQUIRE(ctx, "omx_autotune: null context");
+    ctx->autotune = (enable != 0);
+    return OMX_OK;
+}
+
+int omx_autotune_report(omx_ctx ctx, char *buf, int64_t len) {
+    OMX_REQUIRE(ctx && buf && len > 0, "omx_autotune_report: bad arguments");
+    std::string out;
+    char line[256];
+    for (auto &kv : ctx->tune_gemm) {
+        snprintf(line, sizeof line, "%s -> cfg %d (%.4f ms)\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        out += line;
+    }
+    for (auto &kv : ctx->tune_spmm) {
+        const omx_spmm_shape_s &sh = kv.second.first;
+        snprintf(line, sizeof line, "%s -> batch %d, waves %d, block columns %d, XCD map %d (%.4f ms)\n",
+                 kv.first.c_str(), sh.batch, sh.wpg, sh.jper, sh.mode, kv.second.second);
+        out += line;
+    }
+    const size_t n = std::min<size_t>(out.size(), (size_t)len - 1);
+    memcpy(buf, out.data(), n);
+    buf[n] = 0;
+    return (int)std::min<size_t>(out.size(), (size_t)INT32_MAX);
 }
 
 // ---- element access --------------------------------------------------------------------------

@@ -434,30 +434,26 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     return OMX_OK;
 }
 
-// Tile configuration: a forced one (omx_gemm_set_tile), else the tuned default.  Only the FP64
-// Gram and gradient products (the dominant ones) carry every configuration; other operand
-// layouts and precisions use configuration 0.
-template <int PREC>
-static int pick_cfg(omx_ctx ctx, int m, int n) {
-    if (ctx->gemm_cfg >= 0) return ctx->gemm_cfg;
-    (void)m;
-    (void)n;
-    return 0;
-}
+// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3,
+// the FP32 and BF16 ones on FP32 storage 0 and 4; every other product uses 0.
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+struct CfgSet {
+    static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
+    static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
+    static std::vector<int> ids() {
+        if (f64) return {0, 1, 2, 3};
+        if (lowp) return {0, 4};
+        return {0};
+    }
+};
 
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
-static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
-                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
-                    int64_t ldc) {
-    constexpr bool tunable = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
-    constexpr bool tunable_lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
-    if constexpr (tunable_lowp) {
-        if (pick_cfg<PREC>(ctx, m, n) == 4)
-            return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg4>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
-                                                            beta, C, ldc);
-    }
-    if constexpr (tunable) {
-        switch (pick_cfg<PREC>(ctx, m, n)) {
+static int run_cfg_id(int id, omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2,
+                      int64_t lda, const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta,
+                      TS *C, int64_t ldc) {
+    typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
+    if constexpr (CS::f64) {
+        switch (id) {
             case 1:
                 return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg1>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
                                                                 C0, beta, C, ldc);
@@ -471,8 +467,86 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                 break;
         }
     }
+    if constexpr (CS::lowp) {
+        if (id == 4)
+            return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg4>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
+                                                            beta, C, ldc);
+    }
     return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg0>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta,
                                                     C, ldc);
+}
+
+// Autotuner: the first time a product shape is seen with tuning on, every applicable tile
+// configuration is timed (one warm-up and one timed launch each, with HIP events on the library
+// stream, into a scratch result so the caller's C is untouched) and the fastest is remembered for
+// that shape on this context.  Tuning synchronises and allocates, so it must not run under graph
+// capture.
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+static int tuned_cfg(omx_ctx ctx, int m, int n, int k, const TS *A1, const TS *A2, int64_t lda, const TS *B1,
+                     const TS *B2, int64_t ldb, int &out) {
+    typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
+    char key[128];
+    snprintf(key, sizeof key, "gemm prec=%d es=%d op=%c%c pair=%d m=%d n=%d k=%d", PREC, (int)sizeof(TS),
+             TRA ? 't' : 'n', TRB ? 't' : 'n', NPAIR, m, n, k);
+    auto it = ctx->tune_gemm.find(key);
+    if (it != ctx->tune_gemm.end()) {
+        out = it->second.first;
+        return OMX_OK;
+    }
+    const std::vector<int> ids = CS::ids();
+    if (ids.size() == 1) {
+        ctx->tune_gemm[key] = {ids[0], 0.0};
+        out = ids[0];
+        return OMX_OK;
+    }
+    TS *tmp = nullptr;
+    OMX_CHECK_HIP(hipMalloc((void **)&tmp, sizeof(TS) * (size_t)std::max(m, 1) * std::max(n, 1)));
+    hipEvent_t e0, e1;
+    OMX_CHECK_HIP(hipEventCreate(&e0));
+    OMX_CHECK_HIP(hipEventCreate(&e1));
+    const bool prof = ctx->prof_on;
+    ctx->prof_on = false;
+    int best = ids[0], rc = OMX_OK;
+    float best_ms = 0.0f;
+    for (int id : ids) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && rc == OMX_OK; ++rep) {
+            (void)hipEventRecord(e0, ctx->stream);
+            rc = run_cfg_id<TS, PREC, TRA, TRB, NPAIR>(id, ctx, m, n, k, 1.0, A1, A2, lda, B1, B2, ldb, 0.0, nullptr,
+                                                       0.0, tmp, std::max(m, 1));
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (rc) break;
+        if (id == ids[0] || ms < best_ms) {
+            best = id;
+            best_ms = ms;
+        }
+    }
+    ctx->prof_on = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(tmp);
+    if (rc) return rc;
+    ctx->tune_gemm[key] = {best, (double)best_ms};
+    out = best;
+    return OMX_OK;
+}
+
+template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
+static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
+                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
+                    int64_t ldc) {
+    int id = 0;
+    if (ctx->gemm_cfg >= 0) {
+        id = ctx->gemm_cfg;  // forced (omx_gemm_set_tile); a configuration that does not apply falls back to 0
+    } else if (ctx->autotune) {
+        int rc = tuned_cfg<TS, PREC, TRA, TRB, NPAIR>(ctx, m, n, k, A1, A2, lda, B1, B2, ldb, id);
+        if (rc) return rc;
+    }
+    return run_cfg_id<TS, PREC, TRA, TRB, NPAIR>(id, ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta,
+                                                 C, ldc);
 }
 
 template <typename TS, int PREC>

@@ -284,4 +284,19 @@ int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc) {
 
 double omx_u01(uint64_t seed, uint64_t stream, uint64_t idx) { return u01(seed, stream, idx); }
 
+// Tiled walk over the sites: x-y tiles of tx x ty sites taken in row-major tile order, each
+// swept through all z (and within one z-plane row-major).  A site's +-x and +-y neighbours inside
+// the tile come within tx*ty positions of it and its +-z neighbours exactly tx*ty positions away,
+// instead of Lx*Ly in the natural order.
+int omx_lattice_walk(int Lx, int Ly, int Lz, int tx, int ty, int *order) {
+    if (Lx < 1 || Ly < 1 || Lz < 1 || tx < 1 || ty < 1 || !order) return OMX_ERR_ARG;
+    int64_t k = 0;
+    for (int y0 = 0; y0 < Ly; y0 += ty)
+        for (int x0 = 0; x0 < Lx; x0 += tx)
+            for (int z = 0; z < Lz; ++z)
+                for (int y = y0; y < std::min(Ly, y0 + ty); ++y)
+                    for (int x = x0; x < std::min(Lx, x0 + tx); ++x) order[k++] = x + Lx * (y + Ly * z);
+    return 0;
+}
+
 }  // extern "C"

@@ -6,7 +6,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "omm_hip.h"
@@ -43,6 +45,16 @@ struct ProfEvent {
 
 }  // namespace omx
 
+// Launch shape of a real SpMM: blocks whose loads are issued together (1, 4, 8), waves stacked in
+// rows per workgroup (1, 2, 4), block columns walked per workgroup, XCD map (0: each XCD owns
+// contiguous rows; 1: each XCD walks one row chunk at a time).
+struct omx_spmm_shape_s {
+    int batch = 4;
+    int wpg = 4;
+    int jper = 1;
+    int mode = 0;
+};
+
 struct omx_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -62,8 +74,12 @@ struct omx_ctx_s {
     double prof_bytes[OMX_K_COUNT] = {0};  // algorithmic (minimum) HBM bytes of the timed launches
     int num_cus = 256;
     int gemm_cfg = -1;    // forced GEMM tile configuration (-1: chosen by shape)
-    int spmm_batch = 0;   // SpMM blocks per load batch (0: default 8)
-    int spmm_jper = 0;    // SpMM block columns per workgroup (0: chosen by size)
+    omx_spmm_shape_s spmm_shape;  // forced SpMM launch shape (when spmm_forced)
+    bool spmm_forced = false;
+    // autotuner: per-shape choices (configuration or SpMM launch shape, best time in ms)
+    bool autotune = false;
+    std::map<std::string, std::pair<int, double>> tune_gemm;
+    std::map<std::string, std::pair<omx_spmm_shape_s, double>> tune_spmm;
 };
 
 struct omx_bsr_s {
@@ -76,6 +92,7 @@ struct omx_bsr_s {
     int *d_colptr = nullptr;  // nbc+1
     int *d_rowidx = nullptr;  // nnzb
     double *d_val = nullptr;  // 16*nnzb (x2 if complex)
+    int *d_walk = nullptr;    // optional order in which SpMMs visit the block columns (nbc)
 };
 
 namespace omx {

@@ -7,18 +7,21 @@
 //     Y(:,4J+c) = sum over blocks (I,J), r = 0..3 of X(:,4I+r) * A_IJ(r,c),
 // summed in ascending (I, r), the order of the reference's CSC loop.
 //
-// Mapping (band-major): the state index is cut into bands of ROWS = 64*VEC rows.  One wave
-// computes one (band, block column): each lane keeps 4*VEC accumulators, issues the loads of a
-// whole batch of blocks before using them (memory-level parallelism), and reads the 16 block
-// values as wave-uniform scalars.  A workgroup holds WPG waves on WPG consecutive block columns
-// of one band.  Workgroup ids are dealt round-robin over the 8 XCDs; each XCD owns every 8th band
-// and walks all block columns of one band before the next.  The workgroups resident on an XCD
-// therefore cover a window of consecutive block columns of a single band, and the ROWS-row
-// segments of the basis columns a block column gathers were loaded shortly before by the same
-// XCD: for a banded pattern (the lattice in natural order reaches +-Lx*Ly sites) they are
-// still in that XCD's L2, so HBM sees X about once.  A zero entry stored inside a block
-// contributes an exact zero, so the per-element sum equals the reference's CSC sum over its
-// stored entries.
+// Mapping: a workgroup owns a chunk of RC = 64*VEC*WPG consecutive states (its WPG waves are
+// stacked in rows, each lane loading one 8- or 16-byte vector per basis column) and a run of jper
+// block columns, taken in walk order (the identity, or an order attached to the operator that
+// keeps the basis columns a block column gathers close in time to their other uses, e.g. the
+// lattice in x-y tiles).  Each lane keeps 4*VEC accumulators and issues the loads of a whole
+// batch of NB blocks before using them (memory-level parallelism); the 16 block values are
+// wave-uniform and come through the scalar cache.  Workgroup ids are dealt round-robin over the 8
+// XCDs, and the id -> (row chunk, block-column run) map is XCD-aware:
+//   mode 0: XCD x owns a contiguous eighth of the row chunks and cycles through them for one
+//           block-column run before the next, so the chip reads whole basis columns at once;
+//   mode 1: XCD x owns chunks x, x+8, ... and walks every block column of one chunk before the
+//           next, so an XCD's L2 holds the neighbourhood of the current block columns of a
+//           single chunk.
+// A zero entry stored inside a block contributes an exact zero, so the per-element sum equals
+// the reference's CSC sum over its stored entries.
 #include <algorithm>
 
 #include "omx_internal.h"
@@ -71,110 +74,124 @@ __global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4(int no, int nb, int nbc,
                                                       const T *__restrict__ X, int64_t ldx,
                                                       const int *__restrict__ colptr, const int *__restrict__ rowidx,
                                                       const double *__restrict__ val, double beta,
-                                                      T *__restrict__ Y, int64_t ldy, int nbands, int jgroups) {
-    constexpr int ROWS = 64 * VEC;
+                                                      T *__restrict__ Y, int64_t ldy, int nrc, int njg, int jper,
+                                                      int mode, const int *__restrict__ walk) {
+    constexpr int RC = 64 * VEC * WPG;
     const int w = blockIdx.x;
-    int band, jg;
-    if ((nbands & 7) == 0) {
+    int rc, jg;
+    if ((nrc & 7) == 0) {
         const int xcd = w & 7, local = w >> 3;  // dispatch is round-robin over 8 XCDs
-        band = (local / jgroups) * 8 + xcd;
-        jg = local % jgroups;
+        if (mode == 0) {
+            const int per = nrc >> 3;
+            rc = xcd * per + local % per;
+            jg = local / per;
+        } else {
+            rc = (local / njg) * 8 + xcd;
+            jg = local % njg;
+        }
     } else {
-        band = w / jgroups;
-        jg = w % jgroups;
+        rc = w % nrc;
+        jg = w / nrc;
     }
-    const int J = jg * WPG + (threadIdx.x >> 6);
-    if (J >= nbc) return;  // whole wave; the kernel has no barriers
-    const int i0 = band * ROWS + (threadIdx.x & 63) * VEC;
+    const int i0 = rc * RC + threadIdx.x * VEC;
     const bool full = (i0 + VEC <= no);
     const bool any = (i0 < no);
+    const int jbeg = jg * jper, jend = min(nbc, jbeg + jper);
 
-    T acc[4][VEC];
+    for (int jj = jbeg; jj < jend; ++jj) {
+        const int J = walk ? walk[jj] : jj;
+        T acc[4][VEC];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int q = 0; q < VEC; ++q) acc[c][q] = (T)0;
-    const int lb = colptr[J], le = colptr[J + 1];
-    // Blocks are taken in batches of up to NB: all their basis columns are requested before any
-    // is used, so each lane keeps 4*NB vector loads in flight; the accumulation then runs in
-    // ascending (block, r) order.
-    for (int l0 = lb; l0 < le; l0 += NB) {
-        const int nblk = min(NB, le - l0);
-        T x[NB][4][VEC];
+            for (int q = 0; q < VEC; ++q) acc[c][q] = (T)0;
+        const int lb = colptr[J], le = colptr[J + 1];
+        // Blocks are taken in batches of up to NB: all their basis columns are requested before
+        // any is used, so each lane keeps 4*NB vector loads in flight; the accumulation then runs
+        // in ascending (block, r) order.
+        for (int l0 = lb; l0 < le; l0 += NB) {
+            const int nblk = min(NB, le - l0);
+            T x[NB][4][VEC];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (b < nblk) {
-                const int I = rowidx[l0 + b];
+            for (int b = 0; b < NB; ++b) {
+                if (b < nblk) {
+                    const int I = rowidx[l0 + b];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int kcol = 4 * I + r;
-                    const T *src = X + (int64_t)kcol * ldx + i0;
-                    if (kcol < nb && full) {
-                        vload<T, VEC>(src, x[b][r]);
-                    } else {
+                    for (int r = 0; r < 4; ++r) {
+                        const int kcol = 4 * I + r;
+                        const T *src = X + (int64_t)kcol * ldx + i0;
+                        if (kcol < nb && full) {
+                            vload<T, VEC>(src, x[b][r]);
+                        } else {
 #pragma unroll
-                        for (int q = 0; q < VEC; ++q) x[b][r][q] = (kcol < nb && any && i0 + q < no) ? src[q] : (T)0;
+                            for (int q = 0; q < VEC; ++q)
+                                x[b][r][q] = (kcol < nb && any && i0 + q < no) ? src[q] : (T)0;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (b < nblk) {
+                    const double *v = val + 16 * (int64_t)(l0 + b);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const T a = (T)v[r + 4 * c];
+#pragma unroll
+                            for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[b][r][q], a, acc[c][q]);
+                        }
                     }
                 }
             }
         }
+        if (!any) continue;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (b < nblk) {
-                const double *v = val + 16 * (int64_t)(l0 + b);
+        for (int c = 0; c < 4; ++c) {
+            const int jcol = 4 * J + c;
+            if (jcol >= nb) break;
+            T *dst = Y + (int64_t)jcol * ldy + i0;
+            T out[VEC];
+            if (full) {
+                if (beta != 0.0) {
+                    T old[VEC];
+                    vload<T, VEC>(dst, old);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
+                    for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q] + beta * (double)old[q]);
+                } else {
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const T a = (T)v[r + 4 * c];
-#pragma unroll
-                        for (int q = 0; q < VEC; ++q) acc[c][q] = fma(x[b][r][q], a, acc[c][q]);
-                    }
+                    for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q]);
                 }
-            }
-        }
-    }
-    if (!any) return;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int jcol = 4 * J + c;
-        if (jcol >= nb) break;
-        T *dst = Y + (int64_t)jcol * ldy + i0;
-        T out[VEC];
-        if (full) {
-            if (beta != 0.0) {
-                T old[VEC];
-                vload<T, VEC>(dst, old);
-#pragma unroll
-                for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q] + beta * (double)old[q]);
+                vstore<T, VEC>(dst, out);
             } else {
 #pragma unroll
-                for (int q = 0; q < VEC; ++q) out[q] = (T)(alpha * (double)acc[c][q]);
+                for (int q = 0; q < VEC; ++q)
+                    if (i0 + q < no) {
+                        double o = alpha * (double)acc[c][q];
+                        if (beta != 0.0) o += beta * (double)dst[q];
+                        dst[q] = (T)o;
+                    }
             }
-            vstore<T, VEC>(dst, out);
-        } else {
-#pragma unroll
-            for (int q = 0; q < VEC; ++q)
-                if (i0 + q < no) {
-                    double o = alpha * (double)acc[c][q];
-                    if (beta != 0.0) o += beta * (double)dst[q];
-                    dst[q] = (T)o;
-                }
         }
     }
 }
 
+typedef omx_spmm_shape_s SpmmShape;
+
 template <typename T, int VEC, int NB>
-static void launch_spmm(omx_ctx ctx, int wpg, int no, int nb, double alpha, const T *X,
-                        int64_t ldx, omx_bsr A, double beta, T *Y, int64_t ldy, int nbands) {
-    const int jgroups = (A->nbc + wpg - 1) / wpg;
-    const unsigned grid = (unsigned)((int64_t)nbands * jgroups);
-#define OMX_SPMM_LAUNCH(W)                                                                                           \
-    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NB, W>), dim3(grid), dim3(64 * W), 0, ctx->stream, no, nb, A->nbc, alpha, \
-                       X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nbands, jgroups)
-    if (wpg == 1)
+static void launch_spmm(omx_ctx ctx, const SpmmShape &sh, int no, int nb, double alpha, const T *X, int64_t ldx,
+                        omx_bsr A, double beta, T *Y, int64_t ldy) {
+    const int nrc = (no + 64 * VEC * sh.wpg - 1) / (64 * VEC * sh.wpg);
+    const int njg = (A->nbc + sh.jper - 1) / sh.jper;
+    const unsigned grid = (unsigned)((int64_t)nrc * njg);
+#define OMX_SPMM_LAUNCH(W)                                                                                       \
+    hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NB, W>), dim3(grid), dim3(64 * W), 0, ctx->stream, no, nb, A->nbc, \
+                       alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nrc, njg, sh.jper,   \
+                       sh.mode, A->d_walk)
+    if (sh.wpg == 1)
         OMX_SPMM_LAUNCH(1);
-    else if (wpg == 2)
+    else if (sh.wpg == 2)
         OMX_SPMM_LAUNCH(2);
     else
         OMX_SPMM_LAUNCH(4);
@@ -182,25 +199,93 @@ static void launch_spmm(omx_ctx ctx, int wpg, int no, int nb, double alpha, cons
 }
 
 template <typename T, int VEC>
+static void launch_spmm_shape(omx_ctx ctx, const SpmmShape &sh, int no, int nb, double alpha, const T *X,
+                              int64_t ldx, omx_bsr A, double beta, T *Y, int64_t ldy) {
+    if (sh.batch >= 8)
+        launch_spmm<T, VEC, 8>(ctx, sh, no, nb, alpha, X, ldx, A, beta, Y, ldy);
+    else if (sh.batch >= 4)
+        launch_spmm<T, VEC, 4>(ctx, sh, no, nb, alpha, X, ldx, A, beta, Y, ldy);
+    else
+        launch_spmm<T, VEC, 1>(ctx, sh, no, nb, alpha, X, ldx, A, beta, Y, ldy);
+}
+
+static int64_t spmm_grid(const SpmmShape &sh, int rows_per_wave, int no, int nbc) {
+    const int64_t nrc = (no + (int64_t)rows_per_wave * sh.wpg - 1) / ((int64_t)rows_per_wave * sh.wpg);
+    return nrc * ((nbc + sh.jper - 1) / sh.jper);
+}
+
+// Autotuner for the SpMM launch shape: the first time an operator shape is seen with tuning on,
+// the candidate shapes are timed into a scratch result (one warm-up and one timed launch each)
+// and the fastest is remembered.
+template <typename T, int VEC>
+static int tuned_spmm_shape(omx_ctx ctx, int no, int nb, const T *X, int64_t ldx, omx_bsr A, SpmmShape &out) {
+    char key[160];
+    snprintf(key, sizeof key, "spmm es=%d vec=%d no=%d nb=%d nnzb=%lld walk=%d", (int)sizeof(T), VEC, no, nb,
+             (long long)A->nnzb, A->d_walk ? 1 : 0);
+    auto it = ctx->tune_spmm.find(key);
+    if (it != ctx->tune_spmm.end()) {
+        out = it->second.first;
+        return OMX_OK;
+    }
+    std::vector<SpmmShape> cands;
+    for (int mode : {0, 1})
+        for (int w : {1, 2, 4})
+            for (int b : {4, 8})
+                for (int jp : {1, 4}) {
+                    SpmmShape sh;
+                    sh.batch = b;
+                    sh.wpg = w;
+                    sh.jper = jp;
+                    sh.mode = mode;
+                    if (spmm_grid(sh, 64 * VEC, no, A->nbc) < (1LL << 31)) cands.push_back(sh);
+                }
+    OMX_REQUIRE(!cands.empty(), "omx_spmm: grid too large");
+    T *tmp = nullptr;
+    OMX_CHECK_HIP(hipMalloc((void **)&tmp, sizeof(T) * (size_t)no * nb));
+    hipEvent_t e0, e1;
+    OMX_CHECK_HIP(hipEventCreate(&e0));
+    OMX_CHECK_HIP(hipEventCreate(&e1));
+    SpmmShape best = cands[0];
+    float best_ms = -1.0f;
+    for (const SpmmShape &sh : cands) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            (void)hipEventRecord(e0, ctx->stream);
+            launch_spmm_shape<T, VEC>(ctx, sh, no, nb, 1.0, X, ldx, A, 0.0, tmp, no);
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (best_ms < 0.0f || ms < best_ms) {
+            best = sh;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(tmp);
+    OMX_CHECK_HIP(hipGetLastError());
+    ctx->tune_spmm[key] = {best, (double)best_ms};
+    out = best;
+    return OMX_OK;
+}
+
+template <typename T, int VEC>
 static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64_t ldx, omx_bsr A, double beta, T *Y,
                     int64_t ldy) {
-    constexpr int ROWS = 64 * VEC;
-    const int nbands = (no + ROWS - 1) / ROWS;
-    int wpg = ctx->spmm_jper > 0 ? ctx->spmm_jper : 4;
-    wpg = wpg >= 4 ? 4 : (wpg >= 2 ? 2 : 1);
-    const int64_t total = (int64_t)nbands * ((A->nbc + wpg - 1) / wpg);
-    OMX_REQUIRE(total < (1LL << 31), "omx_spmm: grid too large");
-    const int batch = ctx->spmm_batch > 0 ? ctx->spmm_batch : 8;
+    SpmmShape sh;
+    if (ctx->spmm_forced) {
+        sh = ctx->spmm_shape;  // forced (omx_spmm_set_tile)
+    } else if (ctx->autotune) {
+        int rc = tuned_spmm_shape<T, VEC>(ctx, no, nb, X, ldx, A, sh);
+        if (rc) return rc;
+    }
+    OMX_REQUIRE(spmm_grid(sh, 64 * VEC, no, A->nbc) < (1LL << 31), "omx_spmm: grid too large");
     hipEvent_t ev;
     prof_begin(ctx, OMX_K_SPMM, &ev);
     prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * A->nnzb * no,
               sizeof(T) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (16.0 * 8 + 4) + 4.0 * (A->nbc + 1));
-    if (batch >= 8)
-        launch_spmm<T, VEC, 8>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
-    else if (batch >= 4)
-        launch_spmm<T, VEC, 4>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
-    else
-        launch_spmm<T, VEC, 1>(ctx, wpg, no, nb, alpha, X, ldx, A, beta, Y, ldy, nbands);
+    launch_spmm_shape<T, VEC>(ctx, sh, no, nb, alpha, X, ldx, A, beta, Y, ldy);
     prof_end(ctx, OMX_K_SPMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
@@ -355,10 +440,40 @@ int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colpt
 
 using namespace omx;
 
-extern "C" int omx_spmm_set_tile(omx_ctx ctx, int batch, int jper) {
-    OMX_REQUIRE(ctx && batch >= 0 && jper >= 0 && jper <= 4096, "omx_spmm_set_tile: bad arguments");
-    ctx->spmm_batch = batch;
-    ctx->spmm_jper = jper;
+extern "C" int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode) {
+    OMX_REQUIRE(ctx, "omx_spmm_set_tile: null context");
+    if (batch == 0) {
+        ctx->spmm_forced = false;
+        return OMX_OK;
+    }
+    OMX_REQUIRE((batch == 1 || batch == 4 || batch == 8) && (wpg == 1 || wpg == 2 || wpg == 4) && jper >= 1 &&
+                    jper <= 4096 && (mode == 0 || mode == 1),
+                "omx_spmm_set_tile: bad shape (batch %d, wpg %d, jper %d, mode %d)", batch, wpg, jper, mode);
+    ctx->spmm_shape.batch = batch;
+    ctx->spmm_shape.wpg = wpg;
+    ctx->spmm_shape.jper = jper;
+    ctx->spmm_shape.mode = mode;
+    ctx->spmm_forced = true;
+    return OMX_OK;
+}
+
+extern "C" int omx_bsr_set_walk(omx_bsr A, const int *order) {
+    OMX_REQUIRE(A, "omx_bsr_set_walk: null operator");
+    omx_ctx ctx = A->ctx;
+    if (!order) {
+        if (A->d_walk) OMX_CHECK_HIP(hipFree(A->d_walk));
+        A->d_walk = nullptr;
+        return OMX_OK;
+    }
+    std::vector<char> seen(A->nbc, 0);
+    for (int j = 0; j < A->nbc; ++j) {
+        OMX_REQUIRE(order[j] >= 0 && order[j] < A->nbc && !seen[order[j]],
+                    "omx_bsr_set_walk: order is not a permutation of the %d block columns (entry %d)", A->nbc, j);
+        seen[order[j]] = 1;
+    }
+    if (!A->d_walk) OMX_CHECK_HIP(hipMalloc((void **)&A->d_walk, sizeof(int) * std::max(A->nbc, 1)));
+    OMX_CHECK_HIP(hipMemcpyAsync(A->d_walk, order, sizeof(int) * A->nbc, hipMemcpyHostToDevice, ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return OMX_OK;
 }
 

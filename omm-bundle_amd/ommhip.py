@@ -68,7 +68,11 @@ SIGNATURES = {
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     "omx_gemm_set_tile": (_i, [_vp, _i]),
-    "omx_spmm_set_tile": (_i, [_vp, _i, _i]),
+    "omx_spmm_set_tile": (_i, [_vp, _i, _i, _i, _i]),
+    "omx_bsr_set_walk": (_i, [_vp, _vp]),
+    "omx_lattice_walk": (_i, [_i, _i, _i, _i, _i, _vp]),
+    "omx_autotune": (_i, [_vp, _i]),
+    "omx_autotune_report": (_i, [_vp, _vp, _i64]),
     "omx_add": (_i, [_vp, _i, _c, _i, _i, _d, _vp, _i64, _d, _vp, _i64]),
     "omx_scale": (_i, [_vp, _i, _i, _i, _d, _vp, _i64]),
     "omx_set": (_i, [_vp, _i, _c, _i, _i, _d, _d, _vp, _i64]),
@@ -195,6 +199,14 @@ def lattice_theta_jacobian(L):
     return seg, ent[:nent.value], coef[:nent.value]
 
 
+def lattice_walk(L, tx: int, ty: int) -> np.ndarray:
+    """Tiled visiting order of the lattice sites (x-y tiles of tx x ty swept through z)."""
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    order = np.zeros(Lx * Ly * Lz, np.int32)
+    check(lib().omx_lattice_walk(Lx, Ly, Lz, tx, ty, _ptr(order)), "lattice walk")
+    return order
+
+
 def lattice_theta0(seed: int = DEFAULT_SEED):
     th = np.zeros(20, np.float64)
     check(lib().omx_lattice_theta0(seed, _ptr(th)), "theta0")
@@ -276,6 +288,19 @@ class Ctx:
 
     def sync(self):
         check(lib().omx_sync(self.h), "omx_sync")
+
+    # -- tuning --
+    def autotune(self, enable=True):
+        check(lib().omx_autotune(self.h, int(enable)), "autotune")
+
+    def autotune_report(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        lib().omx_autotune_report(self.h, buf, len(buf))
+        return buf.value.decode()
+
+    def spmm_shape(self, batch=0, wpg=4, jper=1, mode=0):
+        """Force the real SpMM launch shape (batch 0: back to the default or tuned shape)."""
+        check(lib().omx_spmm_set_tile(self.h, batch, wpg, jper, mode), "spmm_set_tile")
 
     # -- profiling --
     def prof(self, enable=True):
@@ -392,6 +417,14 @@ class Bsr:
         check(lib().omx_bsr_get_pattern(self.h, _ptr(cp), _ptr(ri)), "bsr_get_pattern")
         return cp, ri[: self.nnzb]
 
+    def set_walk(self, order):
+        """Block-column visiting order of later SpMMs (None clears it); results do not change."""
+        if order is None:
+            check(lib().omx_bsr_set_walk(self.h, None), "bsr_set_walk")
+            return
+        o = np.ascontiguousarray(order, np.int32)
+        check(lib().omx_bsr_set_walk(self.h, _ptr(o)), "bsr_set_walk")
+
     def free(self):
         if self.h:
             lib().omx_bsr_destroy(self.h)
@@ -439,16 +472,19 @@ def mset(ctx, seC, Cm: DMat, alpha, beta):
     check(lib().omx_set(ctx.h, Cm.dt, seC.encode(), Cm.rows, Cm.cols, alpha, beta, Cm.p, Cm.ld), "omx_set")
 
 
-def trace(ctx, A: DMat) -> float:
-    out = C.c_double(0)
-    check(lib().omx_trace(ctx.h, A.dt, A.rows, A.p, A.ld, C.byref(out)), "omx_trace")
-    return out.value
+def trace(ctx, A: DMat):
+    """tr(A); complex matrices give a complex value (the C call writes (re, im))."""
+    out = np.zeros(2)
+    check(lib().omx_trace(ctx.h, A.dt, A.rows, A.p, A.ld, out.ctypes.data_as(_pd)), "omx_trace")
+    return complex(out[0], out[1]) if A.dt in (OMX_C128, OMX_C64) else float(out[0])
 
 
-def mm_trace(ctx, A: DMat, B: DMat) -> float:
-    out = C.c_double(0)
-    check(lib().omx_mm_trace(ctx.h, A.dt, A.rows, A.cols, A.p, A.ld, B.p, B.ld, C.byref(out)), "omx_mm_trace")
-    return out.value
+def mm_trace(ctx, A: DMat, B: DMat):
+    """sum_ij A_ij B_ij (complex: sum conj(A_ij) B_ij, a complex value)."""
+    out = np.zeros(2)
+    check(lib().omx_mm_trace(ctx.h, A.dt, A.rows, A.cols, A.p, A.ld, B.p, B.ld, out.ctypes.data_as(_pd)),
+          "omx_mm_trace")
+    return complex(out[0], out[1]) if A.dt in (OMX_C128, OMX_C64) else float(out[0])
 
 
 def omm_coeff(ctx, HW, SW, HWd, SWd, HWdd, SWdd) -> np.ndarray:

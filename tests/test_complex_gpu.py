@@ -92,11 +92,11 @@ def test_zelementwise(ctx, dt):
     W = crandn(n, n)
     dW = oh.DMat.from_host(ctx, W, dt)
     tr = np.zeros(2)
-    oh.check(oh.lib().omx_trace(ctx.h, dt, n, dW.p, dW.ld, tr.ctypes.data_as(oh.C.c_void_p)), "trace")
+    oh.check(oh.lib().omx_trace(ctx.h, dt, n, dW.p, dW.ld, tr.ctypes.data_as(oh._pd)), "trace")
     assert abs(complex(*tr) - np.trace(W)) < tol * 100
     B = crandn(n, n)
     dB = oh.DMat.from_host(ctx, B, dt)
-    oh.check(oh.lib().omx_mm_trace(ctx.h, dt, n, n, dW.p, dW.ld, dB.p, dB.ld, tr.ctypes.data_as(oh.C.c_void_p)),
+    oh.check(oh.lib().omx_mm_trace(ctx.h, dt, n, n, dW.p, dW.ld, dB.p, dB.ld, tr.ctypes.data_as(oh._pd)),
              "mm_trace")
     assert abs(complex(*tr) - np.sum(W.conj() * B)) < tol * 1000
     oh.mset(ctx, "a", dW, 0.25, 2.0)

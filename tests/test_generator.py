@@ -71,3 +71,21 @@ def test_bloch_generator_phases():
     assert np.array_equal(bvk.real, bv) and np.all(bvk.imag == 0)
     Hk = oh.bsr_to_dense(nb, *oh.lattice_bsr_k(L, 0, [0.3, 0.0, 0.7]))
     assert np.allclose(Hk, Hk.conj().T, atol=1e-15)
+
+
+def test_exact_eigenvalue_fixture_reproduces():
+    # the tabulated exact eigenvalue data (used to report the benchmark's energy error against the
+    # exact answer) regenerate from the numpy generator at L = 4
+    import json
+    import scipy.linalg as sl
+    tab = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lattice_eigs.json")))
+    e = tab["L4"]
+    H = G.dense_lattice(4, 4, 4, e["seed"], 0)
+    S = G.dense_lattice(4, 4, 4, e["seed"], 1)
+    w = sl.eigh(H, S, eigvals_only=True)
+    m = e["m"]
+    assert abs(np.sum(w[:m]) - e["sum_lowest_m"]) < 1e-9
+    assert abs((w[m] - w[m - 1]) - e["gap_at_m"]) < 1e-12
+    assert set(tab) >= {"L4", "L8", "L16"}
+    for v in tab.values():
+        assert v["gap_at_m"] > 0 and v["m"] == v["n"] // 2

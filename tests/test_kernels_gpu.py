@@ -119,6 +119,85 @@ def test_spmm_f32(ctx):
     assert np.max(np.abs(dY.download() - X.astype(np.float64) @ S)) < 1e-5
 
 
+@pytest.mark.parametrize("dt", [oh.OMX_F64, oh.OMX_F32])
+@pytest.mark.parametrize("no", [1030, 2048])
+def test_spmm_shapes_and_walks(ctx, dt, no):
+    # every launch shape and block-column walk computes each output element with the same
+    # ascending (block, r) sum, so all of them agree bit for bit
+    L = 6
+    _note("spmm_shapes", dt, no)
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    nb = 4 * L ** 3
+    npdt = np.float64 if dt == oh.OMX_F64 else np.float32
+    X = rng.standard_normal((no, nb)).astype(npdt)
+    Y0 = rng.standard_normal((no, nb)).astype(npdt)
+    A = oh.Bsr(ctx, nb, cp, ri, bv)
+    dX = oh.DMat.from_host(ctx, X, dt)
+    dY = oh.DMat.from_host(ctx, Y0, dt)
+    ctx.spmm_shape(0)
+    oh.spmm(ctx, dX, A, dY, alpha=0.75, beta=-1.5)
+    ref = dY.download()
+    H = oh.bsr_to_dense(nb, cp, ri, bv)
+    tol = 1e-12 if dt == oh.OMX_F64 else 1e-4
+    assert np.max(np.abs(ref - (0.75 * X.astype(np.float64) @ H - 1.5 * Y0))) < tol * 30
+    walks = [None, oh.lattice_walk(L, 2, 3), rng.permutation(nb // 4).astype(np.int32)]
+    try:
+        for walk in walks:
+            A.set_walk(walk)
+            for mode in (0, 1):
+                for wpg in (1, 2, 4):
+                    for batch in (1, 4, 8):
+                        for jper in (1, 3, 16):
+                            ctx.spmm_shape(batch, wpg, jper, mode)
+                            dY.upload(Y0)
+                            oh.spmm(ctx, dX, A, dY, alpha=0.75, beta=-1.5)
+                            got = dY.download()
+                            assert np.array_equal(got, ref), (walk is None, mode, wpg, batch, jper)
+    finally:
+        ctx.spmm_shape(0)
+        A.set_walk(None)
+
+
+def test_spmm_walk_rejects_non_permutation(ctx):
+    L = 3
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    A = oh.Bsr(ctx, 4 * L ** 3, cp, ri, bv)
+    bad = np.arange(L ** 3, dtype=np.int32)
+    bad[1] = 0
+    with pytest.raises(oh.OmxError, match="permutation"):
+        A.set_walk(bad)
+    with pytest.raises(oh.OmxError, match="bad shape"):
+        ctx.spmm_shape(3, 4, 1, 0)
+
+
+def test_autotune_matches_default():
+    # tuned shapes change only the schedule: SpMM results are bitwise those of the default shape,
+    # GEMM results agree to rounding (tile configurations may split K differently)
+    _note("autotune")
+    L, no = 4, 256
+    nb = 4 * L ** 3
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    X = rng.standard_normal((no, nb))
+    with oh.Ctx(0) as c:
+        A = oh.Bsr(c, nb, cp, ri, bv)
+        dX, dY = oh.DMat.from_host(c, X), oh.DMat(c, no, nb)
+        oh.spmm(c, dX, A, dY)
+        ref = dY.download()
+        G = oh.DMat(c, no, no)
+        oh.gemm(c, "n", "t", dX, dX, G)
+        gref = G.download()
+        c.autotune(True)
+        for _ in range(2):  # the first call tunes, the second reuses the choice
+            oh.spmm(c, dX, A, dY)
+            assert np.array_equal(dY.download(), ref)
+            oh.gemm(c, "n", "t", dX, dX, G)
+            assert np.max(np.abs(G.download() - gref)) < 1e-12 * np.max(np.abs(gref))
+        rep = c.autotune_report()
+        assert "spmm" in rep and "gemm" in rep, rep
+        for M in (dX, dY, G, A):
+            M.free()
+
+
 @pytest.mark.parametrize("op", ["n", "t"])
 def test_add_scale_set(ctx, op):
     m, n = 70, 45
