@@ -245,7 +245,10 @@ def main():
                    "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
         "roofline": {"bound": "mfma", "kernel": "k_gemm FP64 (v_mfma_f64_16x16x4f64)",
                      "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
+                     "traffic": profiled_traffic(GEMM_PMC_FILE) if L == 16 else None,
+                     "traffic_source": GEMM_PMC_FILE + " (FETCH_SIZE + WRITE_SIZE of the 8192 x 8192 x 16384 "
+                                       "FP64 Gram launch, scaled by a copy kernel)",
                      "launches": gemm_n, "flops_per_launch": gemm_fl / max(gemm_n, 1),
                      "avg_launch_ms": round(gemm_ms / max(gemm_n, 1), 4)},
         "spmm": {"achieved_GBs": round(spmm_gbs, 1), "frac_hbm": round(spmm_gbs / PEAK_HBM_GBS, 4),
@@ -273,6 +276,18 @@ def main():
 
 
 EIGS_FILE = "tests/golden/lattice_eigs.json"
+GEMM_PMC_FILE = "profiles/k_gemm_fp64_gram_n16384_pmc.md"
+
+
+def profiled_traffic(rel):
+    """HBM bytes per launch recorded in a committed counter summary ('HBM bytes per launch: N')."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), rel)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.strip().startswith("HBM bytes per launch:"):
+            return float(ln.split(":", 1)[1].split()[0])
+    return None
 
 
 def exact_eigs(nb):

@@ -52,7 +52,7 @@ struct omx_spmm_shape_s {
     int batch = 4;
     int wpg = 4;
     int jper = 1;
-    int mode = 0;
+    int mode = 1;
 };
 
 struct omx_ctx_s {
