@@ -60,8 +60,8 @@ def log(*a):
 def prof_totals(oh, ctxh):
     """Per kernel class: (summed event ms, launches, algorithmic flops, algorithmic bytes)."""
     out = {}
-    for k, name in ((oh.K_GEMM, "gemm"), (oh.K_SPMM, "spmm"), (oh.K_COEFF, "coeff"), (oh.K_ELEM, "elem"),
-                    (oh.K_DOT, "dot")):
+    for k, name in ((oh.K_GEMM, "gemm"), (oh.K_GEMM_PAIR, "gemm_pair"), (oh.K_SPMM, "spmm"), (oh.K_COEFF, "coeff"),
+                    (oh.K_ELEM, "elem"), (oh.K_DOT, "dot")):
         ms, n = ctypes.c_double(0), ctypes.c_int64(0)
         fl, by = ctypes.c_double(0), ctypes.c_double(0)
         oh.check(oh.lib().omx_prof_read(ctxh, k, ctypes.byref(ms), ctypes.byref(n)), "prof_read")
@@ -243,7 +243,8 @@ def main():
         "data": "synthetic (seeded lattice generator)",
         "config": {"workload": f"omm flavour 0, {L}x{L}x{L} lattice, n={nb}, m={no}, eta={eta:.6f} (mid-gap)",
                    "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm FP64 (v_mfma_f64_16x16x4f64)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_gemm FP64 Gram and state products (v_mfma_f64_16x16x4f64, 128x128 tiles)",
                      "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
                      "traffic": profiled_traffic(GEMM_PMC_FILE) if L == 16 else None,
@@ -253,6 +254,9 @@ def main():
                      "avg_launch_ms": round(gemm_ms / max(gemm_n, 1), 4)},
         "spmm": {"achieved_GBs": round(spmm_gbs, 1), "frac_hbm": round(spmm_gbs / PEAK_HBM_GBS, 4),
                  "avg_launch_ms": round(spmm_ms / max(spmm_n, 1), 4), "bytes_per_launch": spmm_by / max(spmm_n, 1)},
+        "gemm_pair": {"achieved_TFLOPs": round(prof["gemm_pair"][2] / max(prof["gemm_pair"][0], 1e-9) / 1e9, 3),
+                      "avg_launch_ms": round(prof["gemm_pair"][0] / max(prof["gemm_pair"][1], 1), 4),
+                      "launches": prof["gemm_pair"][1]},
         "coeff_reduction": {"achieved_GBs": round(coeff_gbs, 1), "avg_launch_ms": round(coeff_ms / max(coeff_n, 1), 4)},
         "kernel_ms_in_timed_call": {k: round(v[0], 3) for k, v in prof.items()},
     }
@@ -260,7 +264,9 @@ def main():
         if lab in res:
             line[f"{key}_products_iters_per_s"] = round(res[lab][0], 4)
             g = res[lab][2]["gemm"]
-            line[f"{key}_products_gemm_TFLOPs"] = round(g[2] / (g[0] * 1e-3) / 1e12, 2)
+            line[f"{key}_products_gram_TFLOPs"] = round(g[2] / max(g[0] * 1e-3, 1e-12) / 1e12, 2)
+            g = res[lab][2]["gemm_pair"]  # the gradient pair runs in FP32 in both low-precision labels
+            line[f"{key}_products_gradient_TFLOPs"] = round(g[2] / max(g[0] * 1e-3, 1e-12) / 1e12, 2)
 
     if args.ttc:
         line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log)

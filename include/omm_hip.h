@@ -60,7 +60,8 @@ extern "C" {
 #define OMX_K_COEFF 2
 #define OMX_K_ELEM 3
 #define OMX_K_DOT 4
-#define OMX_K_COUNT 5
+#define OMX_K_GEMM_PAIR 5 /* calc_G's fused pair products (OMX_K_GEMM: every other product) */
+#define OMX_K_COUNT 6
 
 typedef struct omx_ctx_s *omx_ctx;
 typedef struct omx_bsr_s *omx_bsr;
@@ -270,6 +271,25 @@ int omx_zbatch_get_c(omx_zbatch b, double *C);
  * converged flag (e_diff <= cg_tol, cg_tol < 0: 1e-9). */
 int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int check_every, int use_graph,
                      double *e_min, int *icg, int *converged, int *steps_run);
+
+/* ==== uneven-size group of real systems ===================================================== */
+/* Replaces the reference's sequence of independent omm() calls, one per system (np/ip state,
+ * libOMM/src/omm.F90:34-35, 96-106), for real flavour-0 systems of different sizes: system s has
+ * no[s] states and the operators H[s] (already shifted by -eta_s S as omm() expects) and S[s]
+ * (created with omx_bsr_create on ctx; the group keeps references, not copies).  Every CG
+ * decision is taken on the device; one step of all systems is captured as one graph with a
+ * parallel branch per system.  With omx_autotune on, each system's shapes are tuned in the eager
+ * initial pass of the first solve, before the capture. */
+typedef struct omx_group_s *omx_group;
+int omx_group_create(omx_ctx ctx, int nsys, const int *no, const omx_bsr *H, const omx_bsr *S, omx_group *out);
+int omx_group_destroy(omx_group g);
+/* C of system s (no[s] x nb[s] column-major, ld = no[s]), host memory. */
+int omx_group_set_c(omx_group g, int s, const double *C);
+int omx_group_get_c(omx_group g, int s, double *C);
+/* Solve all systems from their current C (init_C semantics): e_min[s] (with + eta[s]*TrQS),
+ * icg[s], converged[s]; the flags are read every check_every steps. */
+int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps, int check_every, int use_graph,
+                    double *e_min, int *icg, int *converged, int *steps_run);
 
 /* ==== dense factorisations (Cholesky and preconditioned flavours) ========================= */
 /* All FP64 column-major; U is upper triangular and only its upper triangle is read. */

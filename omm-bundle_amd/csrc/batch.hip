@@ -18,7 +18,14 @@
 // reference's initial pass computes (omm.F90:350-413).  A converged system is frozen: its
 // `active` flag masks every later launch, and its final state is the reference's (the HW, HC and
 // G updates after the convergence exit are skipped, omm.F90:506-509).
+//
+// Uneven-size group (omx_group_*): real systems of different sizes, each with its own H and S,
+// advanced by the same device-side step.  A system's step is its own sequence of library launches
+// (SpMM with its shape, GEMMs with its tuned tiles, fused reductions) masked by its flags; the
+// systems' sequences go to separate streams forked from the library stream, so the captured graph
+// holds one parallel branch per system and a single graph launch advances all of them.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -142,24 +149,28 @@ __global__ void k_batch_quartic(int nsys, const double *__restrict__ coeff, doub
     grad[s] = (!done && !fail) ? 1 : 0;
 }
 
+// The element-wise kernels below are written over El<T> (double for real systems, double2 for
+// complex128); system s = blockIdx.y (blockIdx.z for update_w) of a batch with contiguous
+// per-system blocks of len elements.
+
 // D := G + lambda*D; G_p := G (active systems)
-__global__ __launch_bounds__(BT_NT) void k_batch_direction(int64_t len, const double2 *__restrict__ G,
-                                                           double2 *__restrict__ D, double2 *__restrict__ Gp,
-                                                           const double *__restrict__ sc,
+template <typename T>
+__global__ __launch_bounds__(BT_NT) void k_batch_direction(int64_t len, const T *__restrict__ G, T *__restrict__ D,
+                                                           T *__restrict__ Gp, const double *__restrict__ sc,
                                                            const int *__restrict__ active) {
+    typedef El<T> E;
     const int s = blockIdx.y;
     if (!active[s]) return;
     const double lam = sc[4 * s + 1];
     const int64_t o = (int64_t)s * len;
     for (int64_t q = (int64_t)blockIdx.x * BT_NT + threadIdx.x; q < len; q += (int64_t)gridDim.x * BT_NT) {
-        const double2 g = G[o + q];
-        double2 d;
+        const T g = G[o + q];
+        T d;
         if (lam == 0.0) {
             d = g;
         } else {
-            const double2 dd = D[o + q];
-            d.x = lam * dd.x + g.x;
-            d.y = lam * dd.y + g.y;
+            const T dd = D[o + q];
+            d = E::make(lam * E::re(dd) + E::re(g), E::cplx ? lam * E::im(dd) + E::im(g) : 0.0);
         }
         D[o + q] = d;
         Gp[o + q] = g;
@@ -167,87 +178,89 @@ __global__ __launch_bounds__(BT_NT) void k_batch_direction(int64_t len, const do
 }
 
 // For systems with mask[s]: Y := Y + x*X, or Y := fail_scale*Y after a failed line search.
-__global__ __launch_bounds__(BT_NT) void k_batch_axpy(int64_t len, const double2 *__restrict__ X,
-                                                      double2 *__restrict__ Y, const double *__restrict__ sc,
-                                                      const int *__restrict__ mask, const int *__restrict__ failf,
-                                                      double fail_scale) {
+template <typename T>
+__global__ __launch_bounds__(BT_NT) void k_batch_axpy(int64_t len, const T *__restrict__ X, T *__restrict__ Y,
+                                                      const double *__restrict__ sc, const int *__restrict__ mask,
+                                                      const int *__restrict__ failf, double fail_scale) {
+    typedef El<T> E;
     const int s = blockIdx.y;
     if (!mask[s]) return;
     const bool failed = failf[s] != 0;
     const double x = sc[4 * s];
     const int64_t o = (int64_t)s * len;
     for (int64_t q = (int64_t)blockIdx.x * BT_NT + threadIdx.x; q < len; q += (int64_t)gridDim.x * BT_NT) {
-        double2 y = Y[o + q];
+        const T y = Y[o + q];
         if (failed) {
-            y.x *= fail_scale;
-            y.y *= fail_scale;
+            Y[o + q] = E::make(E::re(y) * fail_scale, E::im(y) * fail_scale);
         } else {
-            const double2 xx = X[o + q];
-            y.x = y.x + x * xx.x;
-            y.y = y.y + x * xx.y;
+            const T xx = X[o + q];
+            Y[o + q] = E::make(E::re(y) + x * E::re(xx), E::cplx ? E::im(y) + x * E::im(xx) : 0.0);
         }
-        Y[o + q] = y;
     }
 }
 
 // For systems with mask[s]: W := W + x*Wd^H + x*Wd + x^2*Wdd (the reference's order), or
 // W := 0.25*W after a failed line search.
-__global__ __launch_bounds__(BT_NT) void k_batch_update_w(int n, const double2 *__restrict__ Wd,
-                                                          const double2 *__restrict__ Wdd, double2 *__restrict__ W,
-                                                          const double *__restrict__ sc, const int *__restrict__ mask,
-                                                          const int *__restrict__ failf) {
-    constexpr int T = 32;
-    __shared__ double tr[T][T + 1], ti[T][T + 1];
+template <typename T>
+__global__ __launch_bounds__(BT_NT) void k_batch_update_w(int n, const T *__restrict__ Wd, const T *__restrict__ Wdd,
+                                                          T *__restrict__ W, const double *__restrict__ sc,
+                                                          const int *__restrict__ mask, const int *__restrict__ failf) {
+    typedef El<T> E;
+    constexpr int TS = 32;
+    __shared__ double tr[TS][TS + 1];
+    __shared__ double ti[E::cplx ? TS : 1][TS + 1];
     const int s = blockIdx.z;
     if (!mask[s]) return;  // whole workgroup, before the barrier
     const int64_t o = (int64_t)s * n * n;
     const bool failed = failf[s] != 0;
     const double x = sc[4 * s], x2 = x * x;
-    const int tx = threadIdx.x % T, ty = threadIdx.x / T;
-    const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
-    for (int b = ty; b < T; b += BT_NT / T) {
+    const int tx = threadIdx.x % TS, ty = threadIdx.x / TS;
+    const int i0 = blockIdx.x * TS, j0 = blockIdx.y * TS;
+    for (int b = ty; b < TS; b += BT_NT / TS) {
         const int r = j0 + tx, c = i0 + b;
-        double2 w = {0.0, 0.0};
-        if (r < n && c < n) w = Wd[o + r + (int64_t)c * n];
-        tr[tx][b] = w.x;
-        ti[tx][b] = -w.y;  // conjugate transpose
+        const T w = (r < n && c < n) ? Wd[o + r + (int64_t)c * n] : E::make(0.0, 0.0);
+        tr[tx][b] = E::re(w);
+        if constexpr (E::cplx) ti[tx][b] = -E::im(w);  // conjugate transpose
     }
     __syncthreads();
-    for (int c = ty; c < T; c += BT_NT / T) {
+    for (int c = ty; c < TS; c += BT_NT / TS) {
         const int i = i0 + tx, j = j0 + c;
         if (i >= n || j >= n) continue;
         const int64_t q = o + i + (int64_t)j * n;
-        double2 w = W[q];
+        const T w = W[q];
         if (failed) {
-            w.x *= 0.25;
-            w.y *= 0.25;
+            W[q] = E::make(E::re(w) * 0.25, E::im(w) * 0.25);
         } else {
-            const double2 d = Wd[q], dd = Wdd[q];
-            w.x = w.x + x * tr[c][tx];
-            w.x = w.x + x * d.x;
-            w.x = w.x + x2 * dd.x;
-            w.y = w.y + x * ti[c][tx];
-            w.y = w.y + x * d.y;
-            w.y = w.y + x2 * dd.y;
+            const T d = Wd[q], dd = Wdd[q];
+            double wr = E::re(w);
+            wr = wr + x * tr[c][tx];
+            wr = wr + x * E::re(d);
+            wr = wr + x2 * E::re(dd);
+            double wi = 0.0;
+            if constexpr (E::cplx) {
+                wi = E::im(w);
+                wi = wi + x * ti[c][tx];
+                wi = wi + x * E::im(d);
+                wi = wi + x2 * E::im(dd);
+            }
+            W[q] = E::make(wr, wi);
         }
-        W[q] = w;
     }
 }
 
 // After a rescaling (omm.F90:519-523): G_p := G; G := 1.5*HC + G (HC already halved).
-__global__ __launch_bounds__(BT_NT) void k_batch_fail_grad(int64_t len, const double2 *__restrict__ HC,
-                                                           double2 *__restrict__ G, double2 *__restrict__ Gp,
-                                                           const int *__restrict__ cont, const int *__restrict__ failf) {
+template <typename T>
+__global__ __launch_bounds__(BT_NT) void k_batch_fail_grad(int64_t len, const T *__restrict__ HC, T *__restrict__ G,
+                                                           T *__restrict__ Gp, const int *__restrict__ cont,
+                                                           const int *__restrict__ failf) {
+    typedef El<T> E;
     const int s = blockIdx.y;
     if (!cont[s] || !failf[s]) return;
     const int64_t o = (int64_t)s * len;
     for (int64_t q = (int64_t)blockIdx.x * BT_NT + threadIdx.x; q < len; q += (int64_t)gridDim.x * BT_NT) {
-        const double2 g = G[o + q], h = HC[o + q];
+        const T g = G[o + q], h = HC[o + q];
         Gp[o + q] = g;
-        double2 r;
-        r.x = 1.5 * h.x + g.x;
-        r.y = 1.5 * h.y + g.y;
-        G[o + q] = r;
+        G[o + q] = E::make(1.5 * E::re(h) + E::re(g), E::cplx ? 1.5 * E::im(h) + E::im(g) : 0.0);
     }
 }
 
@@ -280,18 +293,16 @@ __global__ void k_batch_init_flags(int nsys, int *__restrict__ active, int *__re
 }
 
 // G := 4*HC (systems with mask[s]); the first pass of calc_G when the pair launch does not apply.
-__global__ __launch_bounds__(BT_NT) void k_batch_scale_copy(int64_t len, const double2 *__restrict__ X,
-                                                            double2 *__restrict__ Y, double a,
-                                                            const int *__restrict__ mask) {
+template <typename T>
+__global__ __launch_bounds__(BT_NT) void k_batch_scale_copy(int64_t len, const T *__restrict__ X, T *__restrict__ Y,
+                                                            double a, const int *__restrict__ mask) {
+    typedef El<T> E;
     const int s = blockIdx.y;
     if (!mask[s]) return;
     const int64_t o = (int64_t)s * len;
     for (int64_t q = (int64_t)blockIdx.x * BT_NT + threadIdx.x; q < len; q += (int64_t)gridDim.x * BT_NT) {
-        const double2 x = X[o + q];
-        double2 y;
-        y.x = a * x.x;
-        y.y = a * x.y;
-        Y[o + q] = y;
+        const T x = X[o + q];
+        Y[o + q] = E::make(a * E::re(x), a * E::im(x));
     }
 }
 
@@ -316,7 +327,7 @@ static int batch_calc_G(omx_zbatch b, const int *mask) {
     int rc = launch_zgemm_pair_batched(ctx, no, nb, no, -2.0, b->SW, b->HW, no, lenW, b->HC, b->SC, no, lenC, 4.0,
                                        b->HC, 0.0, b->G, no, lenC, ns, mask);
     if (rc != OMX_ERR_UNSUPPORTED) return rc;
-    hipLaunchKernelGGL(k_batch_scale_copy, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, 4.0,
+    hipLaunchKernelGGL(k_batch_scale_copy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, 4.0,
                        mask);
     if ((rc = launch_zgemm_batched(ctx, 'n', 'n', no, nb, no, -2.0, b->SW, no, lenW, b->HC, no, lenC, 1.0, b->G, no,
                                    lenC, ns, mask)))
@@ -332,7 +343,7 @@ static int batch_step(omx_zbatch b) {
     const int64_t lenC = (int64_t)no * nb, lenW = (int64_t)no * no, sV = 32 * b->nnzb;
     int *act = flag(b, F_ACTIVE), *failf = flag(b, F_FAIL), *cont = flag(b, F_CONT), *grad = flag(b, F_GRAD);
     int rc;
-    hipLaunchKernelGGL(k_batch_direction, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->G, b->D, b->Gp,
+    hipLaunchKernelGGL(k_batch_direction<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->G, b->D, b->Gp,
                        b->sc, act);
     if ((rc = launch_spmm_z_batched(ctx, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->D, no, lenC, b->HG, no,
                                     lenC, ns, act)))
@@ -346,29 +357,29 @@ static int batch_step(omx_zbatch b) {
         if ((rc = launch_zgemm_batched(ctx, 'n', 'c', no, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC, 0.0, dst[q],
                                        no, lenW, ns, act)))
             return rc;
-    if ((rc = launch_coeff_z_batched(ctx, no, no, lenW, b->HW, b->SW, b->HWd, b->SWd, b->HWdd, b->SWdd, ns,
-                                     b->partial, b->coeff)))
+    if ((rc = launch_coeff_batched(ctx, OMX_C128, no, no, lenW, b->HW, b->SW, b->HWd, b->SWd, b->HWdd, b->SWdd, ns,
+                                     b->partial, b->coeff, nullptr)))
         return rc;
     hipLaunchKernelGGL(k_batch_quartic, dim3((ns + 63) / 64), dim3(64), 0, ctx->stream, ns, b->coeff, b->sc, act,
                        failf, cont, grad, flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->tol);
     // C and SW for every system that stepped (omm.F90:470-502)
-    hipLaunchKernelGGL(k_batch_axpy, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->D, b->C, b->sc, act,
+    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->D, b->C, b->sc, act,
                        failf, 0.5);
     const dim3 wgrid((no + 31) / 32, (no + 31) / 32, ns);
-    hipLaunchKernelGGL(k_batch_update_w, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->SWd, b->SWdd, b->SW, b->sc, act,
+    hipLaunchKernelGGL(k_batch_update_w<double2>, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->SWd, b->SWdd, b->SW, b->sc, act,
                        failf);
     // HW, HC, SC for the systems still running (omm.F90:511-527)
-    hipLaunchKernelGGL(k_batch_update_w, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->HWd, b->HWdd, b->HW, b->sc, cont,
+    hipLaunchKernelGGL(k_batch_update_w<double2>, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->HWd, b->HWdd, b->HW, b->sc, cont,
                        failf);
-    hipLaunchKernelGGL(k_batch_axpy, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HG, b->HC, b->sc, cont,
+    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HG, b->HC, b->sc, cont,
                        failf, 0.5);
-    hipLaunchKernelGGL(k_batch_axpy, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->SG, b->SC, b->sc, cont,
+    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->SG, b->SC, b->sc, cont,
                        failf, 0.5);
-    hipLaunchKernelGGL(k_batch_fail_grad, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, b->Gp,
+    hipLaunchKernelGGL(k_batch_fail_grad<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, b->Gp,
                        cont, failf);
     // G = 4HC - 2SW*HC - 2HW*SC where the line search succeeded, then Polak-Ribiere
     if ((rc = batch_calc_G(b, grad))) return rc;
-    if ((rc = launch_pr_z_batched(ctx, no, nb, no, lenC, b->G, b->Gp, ns, b->partial2, b->dots))) return rc;
+    if ((rc = launch_pr_batched(ctx, OMX_C128, no, nb, no, lenC, b->G, b->Gp, ns, b->partial2, b->dots, nullptr))) return rc;
     hipLaunchKernelGGL(k_batch_lambda, dim3((ns + 63) / 64), dim3(64), 0, ctx->stream, ns, b->dots, b->sc, grad, failf,
                        cont, act);
     OMX_CHECK_HIP(hipGetLastError());
@@ -532,6 +543,311 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
         if ((rc = omx_mm_trace(ctx, OMX_C128, no, no, W, no, W, no, dd))) return rc;
         const double TrQS = 2.0 * tr[0] - dd[0];
         e_min[s] = sc[4 * s + 2] + eta * TrQS;
+        icg[s] = h_icg[s];
+        converged[s] = h_conv[s];
+    }
+    return OMX_OK;
+}
+
+}  // extern "C"
+
+// ---- uneven-size group of real systems -------------------------------------------------------
+
+struct omx_group_s {
+    omx_ctx ctx = nullptr;
+    int nsys = 0;
+    std::vector<int> no, nb;
+    std::vector<omx_bsr> H, S;
+    // per system: C, HC, SC, G, Gp, D, HG, SG (no x nb), HW, SW, HWd, SWd, HWdd, SWdd (no x no), ld = no
+    std::vector<std::array<double *, 14>> m;
+    std::vector<double *> partial, partial2;
+    double *coeff = nullptr, *dots = nullptr, *sc = nullptr;
+    int *flags = nullptr;  // F_COUNT arrays of nsys
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    double tol = 1e-9;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+enum { M_C = 0, M_HC, M_SC, M_G, M_GP, M_D, M_HG, M_SG, M_HW, M_SW, M_HWD, M_SWD, M_HWDD, M_SWDD };
+
+static int *gflag(omx_group g, int which, int s) { return g->flags + which * g->nsys + s; }
+
+// G = 4HC - 2SW*HC - 2HW*SC for system s when *mask (calc_G, omm_ops.F90:17-39).
+static int group_calc_G(omx_group g, int s, const int *mask) {
+    omx_ctx ctx = g->ctx;
+    const int no = g->no[s], nb = g->nb[s];
+    auto &M = g->m[s];
+    ctx->launch_mask = mask;
+    int rc = launch_gemm_pair(ctx, OMX_PREC_F64, OMX_F64, no, nb, no, -2.0, M[M_SW], M[M_HW], no, M[M_HC], M[M_SC],
+                              no, 4.0, M[M_HC], 0.0, M[M_G], no);
+    if (rc == OMX_ERR_UNSUPPORTED) {
+        const int64_t lenC = (int64_t)no * nb;
+        hipLaunchKernelGGL(k_batch_scale_copy<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_HC],
+                           M[M_G], 4.0, mask);
+        rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, false, false, no, nb, no, -2.0, M[M_SW], no, M[M_HC], no, 1.0,
+                         M[M_G], no);
+        if (!rc)
+            rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, false, false, no, nb, no, -2.0, M[M_HW], no, M[M_SC], no,
+                             1.0, M[M_G], no);
+    }
+    ctx->launch_mask = nullptr;
+    return rc;
+}
+
+// Line-search products X*A^T (no x no) of system s, masked.
+static int group_gram(omx_group g, int s, const double *X, const double *Y, double *W) {
+    const int no = g->no[s], nb = g->nb[s];
+    return launch_gemm(g->ctx, OMX_PREC_F64, OMX_F64, false, true, no, no, nb, 1.0, X, no, Y, no, 0.0, W, no);
+}
+
+// One CG step of system s on the current library stream (the batch step, per system).
+static int group_sys_step(omx_group g, int s) {
+    omx_ctx ctx = g->ctx;
+    const int no = g->no[s], nb = g->nb[s];
+    const int64_t lenC = (int64_t)no * nb;
+    auto &M = g->m[s];
+    int *act = gflag(g, F_ACTIVE, s), *failf = gflag(g, F_FAIL, s), *cont = gflag(g, F_CONT, s),
+        *grad = gflag(g, F_GRAD, s);
+    double *scs = g->sc + 4 * s;
+    int rc;
+    hipLaunchKernelGGL(k_batch_direction<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_G],
+                       M[M_D], M[M_GP], scs, act);
+    ctx->launch_mask = act;
+    rc = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_D], no, g->H[s], 0.0, M[M_HG], no);
+    if (!rc) rc = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_D], no, g->S[s], 0.0, M[M_SG], no);
+    if (!rc) rc = group_gram(g, s, M[M_HC], M[M_D], M[M_HWD]);
+    if (!rc) rc = group_gram(g, s, M[M_SC], M[M_D], M[M_SWD]);
+    if (!rc) rc = group_gram(g, s, M[M_HG], M[M_D], M[M_HWDD]);
+    if (!rc) rc = group_gram(g, s, M[M_SG], M[M_D], M[M_SWDD]);
+    if (!rc)
+        rc = launch_coeff_batched(ctx, OMX_F64, no, no, 0, M[M_HW], M[M_SW], M[M_HWD], M[M_SWD], M[M_HWDD],
+                                  M[M_SWDD], 1, g->partial[s], g->coeff + 18 * s, act);
+    ctx->launch_mask = nullptr;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_batch_quartic, dim3(1), dim3(64), 0, ctx->stream, 1, g->coeff + 18 * s, scs, act, failf,
+                       cont, grad, gflag(g, F_ICG, s), gflag(g, F_CONV, s), gflag(g, F_NFAIL, s), g->tol);
+    // C and SW when the system stepped; HW, HC, SC when it is still running (omm.F90:470-527)
+    hipLaunchKernelGGL(k_batch_axpy<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_D], M[M_C],
+                       scs, act, failf, 0.5);
+    const dim3 wgrid((no + 31) / 32, (no + 31) / 32, 1);
+    hipLaunchKernelGGL(k_batch_update_w<double>, wgrid, dim3(BT_NT), 0, ctx->stream, no, M[M_SWD], M[M_SWDD], M[M_SW],
+                       scs, act, failf);
+    hipLaunchKernelGGL(k_batch_update_w<double>, wgrid, dim3(BT_NT), 0, ctx->stream, no, M[M_HWD], M[M_HWDD], M[M_HW],
+                       scs, cont, failf);
+    hipLaunchKernelGGL(k_batch_axpy<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_HG], M[M_HC],
+                       scs, cont, failf, 0.5);
+    hipLaunchKernelGGL(k_batch_axpy<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_SG], M[M_SC],
+                       scs, cont, failf, 0.5);
+    hipLaunchKernelGGL(k_batch_fail_grad<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_HC],
+                       M[M_G], M[M_GP], cont, failf);
+    if ((rc = group_calc_G(g, s, grad))) return rc;
+    if ((rc = launch_pr_batched(ctx, OMX_F64, no, nb, no, 0, M[M_G], M[M_GP], 1, g->partial2[s], g->dots + 2 * s,
+                                grad)))
+        return rc;
+    hipLaunchKernelGGL(k_batch_lambda, dim3(1), dim3(64), 0, ctx->stream, 1, g->dots + 2 * s, scs, grad, failf, cont,
+                       act);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+// Runs f(s) for every system on its own stream, forked from and joined back to the library stream.
+template <typename F>
+static int group_fanout(omx_group g, F &&f) {
+    omx_ctx ctx = g->ctx;
+    hipStream_t main = ctx->stream;
+    const bool prof = ctx->prof_on;
+    ctx->prof_on = false;
+    int rc = OMX_OK;
+    if (hipEventRecord(g->fork, main) != hipSuccess) rc = OMX_ERR_HIP;
+    for (int s = 0; s < g->nsys && !rc; ++s) {
+        if (hipStreamWaitEvent(g->streams[s], g->fork, 0) != hipSuccess) {
+            rc = OMX_ERR_HIP;
+            break;
+        }
+        ctx->stream = g->streams[s];
+        rc = f(s);
+        ctx->stream = main;
+        if (!rc && hipEventRecord(g->joins[s], g->streams[s]) != hipSuccess) rc = OMX_ERR_HIP;
+        if (!rc && hipStreamWaitEvent(main, g->joins[s], 0) != hipSuccess) rc = OMX_ERR_HIP;
+    }
+    ctx->stream = main;
+    ctx->launch_mask = nullptr;
+    ctx->prof_on = prof;
+    if (rc == OMX_ERR_HIP) set_error("omx_group: stream fork/join failed");
+    return rc;
+}
+
+static void group_free(omx_group g) {
+    if (!g) return;
+    if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (auto &M : g->m)
+        for (double *p : M)
+            if (p) (void)hipFree(p);
+    for (double *p : g->partial)
+        if (p) (void)hipFree(p);
+    for (double *p : g->partial2)
+        if (p) (void)hipFree(p);
+    void *ptrs[] = {g->coeff, g->dots, g->sc, g->flags};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipStream_t st : g->streams)
+        if (st) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : g->joins)
+        if (e) (void)hipEventDestroy(e);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    delete g;
+}
+
+extern "C" {
+
+int omx_group_create(omx_ctx ctx, int nsys, const int *no, const omx_bsr *H, const omx_bsr *S, omx_group *out) {
+    OMX_REQUIRE(ctx && out && nsys > 0 && no && H && S, "omx_group_create: bad arguments");
+    for (int s = 0; s < nsys; ++s) {
+        OMX_REQUIRE(H[s] && S[s] && H[s]->ctx == ctx && S[s]->ctx == ctx, "omx_group_create: system %d: operators "
+                    "missing or on another context", s);
+        OMX_REQUIRE(!H[s]->is_complex && !S[s]->is_complex, "omx_group_create: system %d: complex operator", s);
+        OMX_REQUIRE(H[s]->nb == S[s]->nb && no[s] > 0 && no[s] <= H[s]->nb,
+                    "omx_group_create: system %d: bad sizes (no %d, H %d, S %d)", s, no[s], H[s]->nb, S[s]->nb);
+    }
+    omx_group g = new omx_group_s();
+    g->ctx = ctx;
+    g->nsys = nsys;
+    g->no.assign(no, no + nsys);
+    g->H.assign(H, H + nsys);
+    g->S.assign(S, S + nsys);
+    g->m.resize(nsys);
+    g->partial.assign(nsys, nullptr);
+    g->partial2.assign(nsys, nullptr);
+    g->streams.assign(nsys, nullptr);
+    g->joins.assign(nsys, nullptr);
+    bool ok = true;
+    for (int s = 0; s < nsys && ok; ++s) {
+        const int n = H[s]->nb, o = no[s];
+        g->nb.push_back(n);
+        g->m[s].fill(nullptr);
+        for (int q = 0; q < 14 && ok; ++q) {
+            const size_t elems = (size_t)o * (q < M_HW ? n : o);
+            ok = hipMalloc(&g->m[s][q], sizeof(double) * elems) == hipSuccess;
+            if (ok) ok = hipMemsetAsync(g->m[s][q], 0, sizeof(double) * elems, ctx->stream) == hipSuccess;
+        }
+        ok = ok && hipMalloc(&g->partial[s], sizeof(double) * coeff_partial_doubles(o, 1)) == hipSuccess &&
+             hipMalloc(&g->partial2[s], sizeof(double) * pr_partial_doubles(o, n, 1)) == hipSuccess &&
+             hipStreamCreateWithFlags(&g->streams[s], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&g->joins[s], hipEventDisableTiming) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&g->coeff, sizeof(double) * 18 * nsys) == hipSuccess &&
+         hipMalloc(&g->dots, sizeof(double) * 2 * nsys) == hipSuccess &&
+         hipMalloc(&g->sc, sizeof(double) * 4 * nsys) == hipSuccess &&
+         hipMalloc(&g->flags, sizeof(int) * F_COUNT * nsys) == hipSuccess &&
+         hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        group_free(g);
+        set_error("omx_group_create: device allocation failed");
+        return OMX_ERR_NOMEM;
+    }
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    *out = g;
+    return OMX_OK;
+}
+
+int omx_group_destroy(omx_group g) {
+    group_free(g);
+    return OMX_OK;
+}
+
+int omx_group_set_c(omx_group g, int s, const double *C) {
+    OMX_REQUIRE(g && C && s >= 0 && s < g->nsys, "omx_group_set_c: bad arguments");
+    OMX_CHECK_HIP(hipMemcpyAsync(g->m[s][M_C], C, sizeof(double) * (size_t)g->no[s] * g->nb[s],
+                                 hipMemcpyHostToDevice, g->ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+    return OMX_OK;
+}
+
+int omx_group_get_c(omx_group g, int s, double *C) {
+    OMX_REQUIRE(g && C && s >= 0 && s < g->nsys, "omx_group_get_c: bad arguments");
+    OMX_CHECK_HIP(hipMemcpyAsync(C, g->m[s][M_C], sizeof(double) * (size_t)g->no[s] * g->nb[s],
+                                 hipMemcpyDeviceToHost, g->ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+    return OMX_OK;
+}
+
+int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps, int check_every, int use_graph,
+                    double *e_min, int *icg, int *converged, int *steps_run) {
+    OMX_REQUIRE(g && eta && e_min && icg && converged && max_steps > 0 && check_every > 0,
+                "omx_group_solve: bad arguments");
+    omx_ctx ctx = g->ctx;
+    hipStream_t st = ctx->stream;
+    const int ns = g->nsys;
+    g->tol = cg_tol < 0.0 ? 1.0e-9 : cg_tol;
+    int rc;
+    hipLaunchKernelGGL(k_batch_init_flags, dim3((ns + 63) / 64), dim3(64), 0, st, ns, g->flags + F_ACTIVE * ns,
+                       g->flags + F_ICG * ns, g->flags + F_CONV * ns, g->flags + F_NFAIL * ns, g->sc);
+    // initial state of every system (omm.F90:350-370), eager: HC = C H, SC = C S, HW = HC C^T,
+    // SW = SC C^T, G; with tuning on, each system's shapes are tuned here, before any capture
+    rc = group_fanout(g, [&](int s) {
+        auto &M = g->m[s];
+        const int no = g->no[s], nb = g->nb[s];
+        ctx->launch_mask = gflag(g, F_ACTIVE, s);
+        int r = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_C], no, g->H[s], 0.0, M[M_HC], no);
+        if (!r) r = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_C], no, g->S[s], 0.0, M[M_SC], no);
+        if (!r) r = group_gram(g, s, M[M_HC], M[M_C], M[M_HW]);
+        if (!r) r = group_gram(g, s, M[M_SC], M[M_C], M[M_SW]);
+        ctx->launch_mask = nullptr;
+        if (!r) r = group_calc_G(g, s, gflag(g, F_ACTIVE, s));
+        return r;
+    });
+    if (rc) return rc;
+    if (use_graph && !g->exec) {
+        OMX_CHECK_HIP(hipStreamSynchronize(st));  // tuning (if on) has finished before the capture
+        OMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); });
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        OMX_CHECK_HIP(e);
+        g->graph = gr;
+        OMX_CHECK_HIP(hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0));
+    }
+    std::vector<int> h_act(ns);
+    int done_steps = 0;
+    while (done_steps < max_steps) {
+        const int burst = std::min(check_every, max_steps - done_steps);
+        for (int q = 0; q < burst; ++q) {
+            if (use_graph)
+                OMX_CHECK_HIP(hipGraphLaunch(g->exec, st));
+            else if ((rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); })))
+                return rc;
+        }
+        done_steps += burst;
+        OMX_CHECK_HIP(hipMemcpyAsync(h_act.data(), g->flags + F_ACTIVE * ns, sizeof(int) * ns, hipMemcpyDeviceToHost,
+                                     st));
+        OMX_CHECK_HIP(hipStreamSynchronize(st));
+        bool any = false;
+        for (int s = 0; s < ns; ++s) any = any || h_act[s];
+        if (!any) break;
+    }
+    if (steps_run) *steps_run = done_steps;
+    // e_min + eta*TrQS, TrQS = tr[(2I - SW) SW] = 2 tr(SW) - <SW, SW> (omm.F90:576-593)
+    std::vector<double> sc(4 * ns);
+    std::vector<int> h_icg(ns), h_conv(ns);
+    OMX_CHECK_HIP(hipMemcpyAsync(sc.data(), g->sc, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, st));
+    OMX_CHECK_HIP(hipMemcpyAsync(h_icg.data(), g->flags + F_ICG * ns, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
+    OMX_CHECK_HIP(hipMemcpyAsync(h_conv.data(), g->flags + F_CONV * ns, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
+    OMX_CHECK_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < ns; ++s) {
+        double tr[2], dd[2];
+        const double *W = g->m[s][M_SW];
+        if ((rc = omx_trace(ctx, OMX_F64, g->no[s], W, g->no[s], tr))) return rc;
+        if ((rc = omx_mm_trace(ctx, OMX_F64, g->no[s], g->no[s], W, g->no[s], W, g->no[s], dd))) return rc;
+        e_min[s] = sc[4 * s + 2] + eta[s] * (2.0 * tr[0] - dd[0]);
         icg[s] = h_icg[s];
         converged[s] = h_conv[s];
     }

@@ -41,7 +41,9 @@ __device__ inline void block_reduce_store(double (&v)[NV], double *__restrict__ 
 // Final stage: out[q] = sum over blocks b (ascending) of partial[b*NV + q].
 template <int NV>
 __global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const double *__restrict__ partial,
-                                                           double *__restrict__ out) {
+                                                           double *__restrict__ out,
+                                                           const int *__restrict__ mask = nullptr) {
+    if (mask && !mask[blockIdx.x]) return;          // system masked off (whole workgroup)
     partial += (int64_t)blockIdx.x * nblocks * NV;  // one workgroup per system in batched use
     out += blockIdx.x * NV;
     double v[NV];
@@ -52,30 +54,6 @@ __global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const do
         for (int q = 0; q < NV; ++q) v[q] += partial[(int64_t)b * NV + q];
     block_reduce_store<NV>(v, out);
 }
-
-// ---- element types -----------------------------------------------------------------------------
-// Real (double, float) and complex (Cx<double>, Cx<float>: the Fortran (re, im) layout).  Every
-// kernel below is written once over El<T>; for real types the imaginary terms are compiled out,
-// so the real arithmetic (and its rounding) is exactly the real-only one.
-
-template <typename R_>
-struct Cx {
-    R_ x, y;
-};
-template <typename T>
-struct El {
-    static constexpr bool cplx = false;
-    __device__ static inline double re(const T &v) { return (double)v; }
-    __device__ static inline double im(const T &) { return 0.0; }
-    __device__ static inline T make(double r, double) { return (T)r; }
-};
-template <typename R_>
-struct El<Cx<R_>> {
-    static constexpr bool cplx = true;
-    __device__ static inline double re(const Cx<R_> &v) { return (double)v.x; }
-    __device__ static inline double im(const Cx<R_> &v) { return (double)v.y; }
-    __device__ static inline Cx<R_> make(double r, double i) { return Cx<R_>{(R_)r, (R_)i}; }
-};
 
 // ---- m_add / m_scale / m_set -----------------------------------------------------------------
 
@@ -197,8 +175,10 @@ __global__ __launch_bounds__(EW_NT) void k_dot_partial(int m, int n, const T *__
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_pr_partial(int64_t total_m, int m, const T *__restrict__ G,
                                                       const T *__restrict__ Gp, int64_t ld,
-                                                      double *__restrict__ partial, int64_t sG = 0) {
+                                                      double *__restrict__ partial, int64_t sG = 0,
+                                                      const int *__restrict__ mask = nullptr) {
     typedef El<T> E;
+    if (mask && !mask[blockIdx.y]) return;
     // batched use: blockIdx.y selects the system
     G += blockIdx.y * sG;
     Gp += blockIdx.y * sG;
@@ -238,8 +218,9 @@ __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, con
                                                          const T *__restrict__ SW, const T *__restrict__ HWd,
                                                          const T *__restrict__ SWd, const T *__restrict__ HWdd,
                                                          const T *__restrict__ SWdd, double *__restrict__ partial,
-                                                         int64_t sW = 0) {
+                                                         int64_t sW = 0, const int *__restrict__ mask = nullptr) {
     typedef El<T> E;
+    if (mask && !mask[blockIdx.y]) return;  // whole workgroup, before the barrier
     // batched use: blockIdx.y selects the system (all six matrices offset by sW elements)
     {
         const int64_t o = blockIdx.y * sW;
@@ -296,9 +277,11 @@ __global__ __launch_bounds__(EW_NT) void k_coeff_partial(int n, int64_t ldm, con
     block_reduce_store<13>(v, partial + 13 * (int64_t)blockIdx.x);
 }
 
-__global__ void k_coeff_final(int nblocks, const double *__restrict__ partial, double *__restrict__ out) {
+__global__ void k_coeff_final(int nblocks, const double *__restrict__ partial, double *__restrict__ out,
+                              const int *__restrict__ mask = nullptr) {
     // One workgroup (per system in batched use): deterministic per-term sums, then the closed-form
     // coefficients.
+    if (mask && !mask[blockIdx.x]) return;
     partial += 13 * (int64_t)blockIdx.x * nblocks;
     out += 18 * blockIdx.x;
     double v[13];
@@ -402,11 +385,11 @@ inline bool with_type(int dt, F &&f) {
 // Batched complex128 reductions over nbatch systems (used by the batched solver): coefficients to
 // out[18*s ...] (5 coefficients then the 13 traces) and Polak-Ribiere dots to out2[2*s ...], on
 // the device, no readback.
-int launch_coeff_z_batched(omx_ctx ctx, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
-                           const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
-                           double *partial, double *out);
-int launch_pr_z_batched(omx_ctx ctx, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp, int nbatch,
-                        double *partial, double *out2);
+int launch_coeff_batched(omx_ctx ctx, int dt, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
+                         const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
+                         double *partial, double *out, const int *mask);
+int launch_pr_batched(omx_ctx ctx, int dt, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp,
+                      int nbatch, double *partial, double *out2, const int *mask);
 size_t coeff_partial_doubles(int n, int nbatch);
 size_t pr_partial_doubles(int n, int m, int nbatch);
 
@@ -670,28 +653,34 @@ size_t pr_partial_doubles(int n, int m, int nbatch) {
     return (size_t)2 * std::min(2048, grid_for((int64_t)n * m)) * nbatch;
 }
 
-int launch_coeff_z_batched(omx_ctx ctx, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
-                           const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
-                           double *partial, double *out) {
-    typedef Cx<double> T;
+int launch_coeff_batched(omx_ctx ctx, int dt, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
+                         const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
+                         double *partial, double *out, const int *mask) {
     const int tiles = (n + TT - 1) / TT;
     const int blocks = tiles * tiles;
-    hipLaunchKernelGGL(k_coeff_partial<T>, dim3((unsigned)blocks, nbatch), dim3(EW_NT), 0, ctx->stream, n, ldm,
-                       (const T *)HW, (const T *)SW, (const T *)HWd, (const T *)SWd, (const T *)HWdd, (const T *)SWdd,
-                       partial, sW);
-    hipLaunchKernelGGL(k_coeff_final, dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out);
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_coeff_partial<T>, dim3((unsigned)blocks, nbatch), dim3(EW_NT), 0, ctx->stream, n, ldm,
+                           (const T *)HW, (const T *)SW, (const T *)HWd, (const T *)SWd, (const T *)HWdd,
+                           (const T *)SWdd, partial, sW, mask);
+    });
+    OMX_REQUIRE(ok, "batched coefficients: unsupported element type %d", dt);
+    hipLaunchKernelGGL(k_coeff_final, dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out, mask);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
 
-int launch_pr_z_batched(omx_ctx ctx, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp, int nbatch,
-                        double *partial, double *out2) {
-    typedef Cx<double> T;
+int launch_pr_batched(omx_ctx ctx, int dt, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp,
+                      int nbatch, double *partial, double *out2, const int *mask) {
     const int64_t total = (int64_t)n * m;
     const int blocks = std::min(2048, grid_for(total));
-    hipLaunchKernelGGL(k_pr_partial<T>, dim3(blocks, nbatch), dim3(EW_NT), 0, ctx->stream, total, n, (const T *)G,
-                       (const T *)Gp, ld, partial, sG);
-    hipLaunchKernelGGL((k_reduce_partials<2>), dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out2);
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_pr_partial<T>, dim3(blocks, nbatch), dim3(EW_NT), 0, ctx->stream, total, n,
+                           (const T *)G, (const T *)Gp, ld, partial, sG, mask);
+    });
+    OMX_REQUIRE(ok, "batched dots: unsupported element type %d", dt);
+    hipLaunchKernelGGL((k_reduce_partials<2>), dim3(nbatch), dim3(EW_NT), 0, ctx->stream, blocks, partial, out2, mask);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

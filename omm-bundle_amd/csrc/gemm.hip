@@ -276,7 +276,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_gemm(int m, int n, int k, 
                                                  const TS *__restrict__ B2, int64_t ldb, double beta0,
                                                  const TS *__restrict__ C0, double beta, TS *__restrict__ C,
                                                  int64_t ldc, int k_per_split, int vec_a, int vec_b,
-                                                 typename Prec<PREC>::acc_t *__restrict__ partial) {
+                                                 typename Prec<PREC>::acc_t *__restrict__ partial,
+                                                 const int *__restrict__ mask) {
     typedef typename Prec<PREC>::lds_t LT;
     typedef typename Prec<PREC>::acc_t AT;
     typedef typename Prec<PREC>::accv AV;
@@ -284,6 +285,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_gemm(int m, int n, int k, 
     typedef LdsT<PREC, CF::BN> LB;
     constexpr int BK = Prec<PREC>::BK;
     __shared__ LT smem[2 * (LA::ELEMS + LB::ELEMS)];
+    if (mask && !*mask) return;  // whole workgroup, before any barrier
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
@@ -399,7 +401,7 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     // Split K when the tile count cannot fill the chip (aim for >= 2 workgroups per CU).
     int splits = 1;
     const int target = 2 * ctx->num_cus;
-    if (tiles < target) {
+    if (tiles < target && !ctx->launch_mask) {
         int max_splits = std::max(1, ktot / (4 * BK));
         splits = std::min(max_splits, (target + tiles - 1) / tiles);
         splits = std::max(1, std::min(splits, 64));
@@ -417,19 +419,20 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     const int vec_a = (lda % VEC == 0) && aligned16(A1) && aligned16(A2);
     const int vec_b = (ldb % VEC == 0) && aligned16(B1) && aligned16(B2);
     hipEvent_t ev;
-    prof_begin(ctx, OMX_K_GEMM, &ev);
-    prof_work(ctx, OMX_K_GEMM, 2.0 * m * n * (double)ktot,
+    prof_begin(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
+    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)ktot,
               sizeof(TS) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                             (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
     hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR, CF>), dim3(tiles, 1, splits), dim3(CF::NT), 0, ctx->stream,
-                       m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, vec_a, vec_b, partial);
+                       m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, vec_a, vec_b, partial,
+                       ctx->launch_mask);
     if (splits > 1) {
         int64_t total = (int64_t)m * n;
         int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL((k_splitk_reduce<TS, AT>), dim3(grid), dim3(256), 0, ctx->stream, m, n, splits, partial,
                            alpha, beta0, C0, beta, C, ldc);
     }
-    prof_end(ctx, OMX_K_GEMM, ev);
+    prof_end(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

@@ -76,6 +76,9 @@ struct omx_ctx_s {
     int gemm_cfg = -1;    // forced GEMM tile configuration (-1: chosen by shape)
     omx_spmm_shape_s spmm_shape;  // forced SpMM launch shape (when spmm_forced)
     bool spmm_forced = false;
+    // Device flag for the grouped solver: while set, GEMM, SpMM and reduction launches do nothing
+    // when *launch_mask == 0, and GEMMs do not split K (no workspace, so launches can be captured).
+    const int *launch_mask = nullptr;
     // autotuner: per-shape choices (configuration or SpMM launch shape, best time in ms)
     bool autotune = false;
     std::map<std::string, std::pair<int, double>> tune_gemm;
@@ -98,6 +101,37 @@ struct omx_bsr_s {
 namespace omx {
 
 constexpr int kScalarSlots = 256;
+
+// ---- element types -----------------------------------------------------------------------------
+// Real (double, float) and complex (Cx<double>, Cx<float>: the Fortran (re, im) layout).  The
+// element-wise kernels are written once over El<T>; for real types the imaginary terms are
+// compiled out, so the real arithmetic (and its rounding) is exactly the real-only one.
+
+template <typename R_>
+struct Cx {
+    R_ x, y;
+};
+template <typename T>
+struct El {
+    static constexpr bool cplx = false;
+    __device__ static inline double re(const T &v) { return (double)v; }
+    __device__ static inline double im(const T &) { return 0.0; }
+    __device__ static inline T make(double r, double) { return (T)r; }
+};
+template <typename R_>
+struct El<Cx<R_>> {
+    static constexpr bool cplx = true;
+    __device__ static inline double re(const Cx<R_> &v) { return (double)v.x; }
+    __device__ static inline double im(const Cx<R_> &v) { return (double)v.y; }
+    __device__ static inline Cx<R_> make(double r, double i) { return Cx<R_>{(R_)r, (R_)i}; }
+};
+template <>
+struct El<double2> {
+    static constexpr bool cplx = true;
+    __device__ static inline double re(const double2 &v) { return v.x; }
+    __device__ static inline double im(const double2 &v) { return v.y; }
+    __device__ static inline double2 make(double r, double i) { return make_double2(r, i); }
+};
 
 // Ensure ctx->ws holds at least `bytes`.  Returns OMX_OK or an error.
 int ensure_ws(omx_ctx ctx, size_t bytes);
@@ -173,11 +207,14 @@ int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, co
 int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colptr, const int *rowidx,
                           const double *val, int64_t sV, const void *X, int64_t ldx, int64_t sX, void *Y, int64_t ldy,
                           int64_t sY, int nbatch, const int *active);
-int launch_coeff_z_batched(omx_ctx ctx, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
-                           const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
-                           double *partial, double *out);
-int launch_pr_z_batched(omx_ctx ctx, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp, int nbatch,
-                        double *partial, double *out2);
+// Line-search coefficients (to out[18*s ...]: the 5 coefficients, then the 13 traces) and
+// Polak-Ribiere dots (to out2[2*s ...]) of nbatch systems of element type dt, on the device; a
+// system s with mask && !mask[s] is skipped.
+int launch_coeff_batched(omx_ctx ctx, int dt, int n, int64_t ldm, int64_t sW, const void *HW, const void *SW,
+                         const void *HWd, const void *SWd, const void *HWdd, const void *SWdd, int nbatch,
+                         double *partial, double *out, const int *mask);
+int launch_pr_batched(omx_ctx ctx, int dt, int n, int m, int64_t ld, int64_t sG, const void *G, const void *Gp,
+                      int nbatch, double *partial, double *out2, const int *mask);
 size_t coeff_partial_doubles(int n, int nbatch);
 size_t pr_partial_doubles(int n, int m, int nbatch);
 

@@ -75,8 +75,10 @@ __global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4(int no, int nb, int nbc,
                                                       const int *__restrict__ colptr, const int *__restrict__ rowidx,
                                                       const double *__restrict__ val, double beta,
                                                       T *__restrict__ Y, int64_t ldy, int nrc, int njg, int jper,
-                                                      int mode, const int *__restrict__ walk) {
+                                                      int mode, const int *__restrict__ walk,
+                                                      const int *__restrict__ mask) {
     constexpr int RC = 64 * VEC * WPG;
+    if (mask && !*mask) return;
     const int w = blockIdx.x;
     int rc, jg;
     if ((nrc & 7) == 0) {
@@ -188,7 +190,7 @@ static void launch_spmm(omx_ctx ctx, const SpmmShape &sh, int no, int nb, double
 #define OMX_SPMM_LAUNCH(W)                                                                                       \
     hipLaunchKernelGGL((k_spmm_bsr4<T, VEC, NB, W>), dim3(grid), dim3(64 * W), 0, ctx->stream, no, nb, A->nbc, \
                        alpha, X, ldx, A->d_colptr, A->d_rowidx, A->d_val, beta, Y, ldy, nrc, njg, sh.jper,   \
-                       sh.mode, A->d_walk)
+                       sh.mode, A->d_walk, ctx->launch_mask)
     if (sh.wpg == 1)
         OMX_SPMM_LAUNCH(1);
     else if (sh.wpg == 2)

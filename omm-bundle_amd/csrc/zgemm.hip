@@ -301,8 +301,8 @@ static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void 
         partial = (R *)ctx->ws;
     }
     hipEvent_t ev;
-    prof_begin(ctx, OMX_K_GEMM, &ev);
-    prof_work(ctx, OMX_K_GEMM, nbatch * 8.0 * m * n * (double)ktot,
+    prof_begin(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
+    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, nbatch * 8.0 * m * n * (double)ktot,
               nbatch * 2.0 * sizeof(R) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                                           (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
     hipLaunchKernelGGL((k_zgemm<R, TRA, CJA, TRB, CJB, NPAIR>), dim3(tiles, nbatch, splits), dim3(ZNT), 0, ctx->stream,
@@ -314,7 +314,7 @@ static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void 
         hipLaunchKernelGGL((k_zsplitk_reduce<R>), dim3(grid), dim3(256), 0, ctx->stream, m, n, splits, partial, alpha,
                            beta0, (const C2 *)C0, beta, (C2 *)C, ldc);
     }
-    prof_end(ctx, OMX_K_GEMM, ev);
+    prof_end(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

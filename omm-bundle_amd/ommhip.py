@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libommhip.so")
 
 OMX_F64, OMX_F32, OMX_BF16, OMX_C128, OMX_C64 = 0, 1, 2, 3, 4
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
-K_SPMM, K_GEMM, K_COEFF, K_ELEM, K_DOT = 0, 1, 2, 3, 4
+K_SPMM, K_GEMM, K_COEFF, K_ELEM, K_DOT, K_GEMM_PAIR = 0, 1, 2, 3, 4, 5
 
 _NP = {OMX_F64: np.float64, OMX_F32: np.float32, OMX_C128: np.complex128, OMX_C64: np.complex64}
 _ESIZE = {OMX_F64: 8, OMX_F32: 4, OMX_BF16: 2, OMX_C128: 16, OMX_C64: 8}
@@ -72,6 +72,11 @@ SIGNATURES = {
     "omx_bsr_set_walk": (_i, [_vp, _vp]),
     "omx_lattice_walk": (_i, [_i, _i, _i, _i, _i, _vp]),
     "omx_autotune": (_i, [_vp, _i]),
+    "omx_group_create": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "omx_group_destroy": (_i, [_vp]),
+    "omx_group_set_c": (_i, [_vp, _i, _vp]),
+    "omx_group_get_c": (_i, [_vp, _i, _vp]),
+    "omx_group_solve": (_i, [_vp, _vp, _d, _i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
     "omx_autotune_report": (_i, [_vp, _vp, _i64]),
     "omx_add": (_i, [_vp, _i, _c, _i, _i, _d, _vp, _i64, _d, _vp, _i64]),
     "omx_scale": (_i, [_vp, _i, _i, _i, _d, _vp, _i64]),
@@ -564,5 +569,55 @@ class ZBatch:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Group:
+    """Uneven-size group of real systems (omx_group_*): system s has no[s] states and its own
+    operators H[s] (already shifted by -eta_s S) and S[s] (Bsr objects on ctx, kept alive here)."""
+
+    def __init__(self, ctx: Ctx, no, H, S):
+        self.ctx, self.nsys = ctx, len(no)
+        self.no = [int(x) for x in no]
+        self.nb = [h.nb for h in H]
+        self._ops = list(H) + list(S)
+        nos = np.ascontiguousarray(self.no, np.int32)
+        hp = (C.c_void_p * self.nsys)(*[h.h.value if isinstance(h.h, C.c_void_p) else h.h for h in H])
+        sp = (C.c_void_p * self.nsys)(*[x.h.value if isinstance(x.h, C.c_void_p) else x.h for x in S])
+        g = C.c_void_p()
+        check(lib().omx_group_create(ctx.h, self.nsys, _ptr(nos), C.cast(hp, C.c_void_p), C.cast(sp, C.c_void_p),
+                                     C.byref(g)), "omx_group_create")
+        self.h = g
+
+    def set_c(self, s, Cm):
+        a = np.asfortranarray(Cm, np.float64)
+        assert a.shape == (self.no[s], self.nb[s])
+        check(lib().omx_group_set_c(self.h, s, _ptr(a)), "group_set_c")
+
+    def get_c(self, s):
+        a = np.zeros((self.no[s], self.nb[s]), np.float64, order="F")
+        check(lib().omx_group_get_c(self.h, s, _ptr(a)), "group_get_c")
+        return a
+
+    def solve(self, eta, cg_tol=1e-9, max_steps=10000, check_every=10, use_graph=True):
+        eta = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, np.float64), (self.nsys,)))
+        e = np.zeros(self.nsys)
+        icg = np.zeros(self.nsys, np.int32)
+        conv = np.zeros(self.nsys, np.int32)
+        steps = C.c_int(0)
+        check(lib().omx_group_solve(self.h, _ptr(eta), cg_tol, max_steps, check_every, int(use_graph), _ptr(e),
+                                    _ptr(icg), _ptr(conv), C.byref(steps)), "group_solve")
+        return {"e_min": e, "icg": icg, "converged": conv.astype(bool), "steps": steps.value}
+
+    def free(self):
+        if self.h:
+            lib().omx_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
         except Exception:
             pass
