@@ -167,6 +167,14 @@ def main():
                     help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
     ap.add_argument("--kpoint-steps", type=int, default=4, help="CG steps timed per k-point in --kpoints mode")
     ap.add_argument("--kpoint-L", type=int, default=10, help="cell edge of the k-point systems (n = 4 L^3)")
+    ap.add_argument("--group", action="store_true",
+                    help="uneven-size mode: 24 real systems, edges L drawn from U{6..20}, one graph launch per step "
+                         "with autotuned tiles, against one omm() call per system")
+    ap.add_argument("--group-steps", type=int, default=3, help="CG steps timed in --group mode")
+    ap.add_argument("--group-count", type=int, default=24, help="systems in --group mode")
+    ap.add_argument("--dims", default=None, help="non-cubic lattice Lx,Ly,Lz (overrides --L), e.g. 32,32,16")
+    ap.add_argument("--mixed-steps", type=int, default=0,
+                    help="also time this many low-precision steps of the mixed (hmx) solve, FP32 storage")
     args = ap.parse_args()
 
     if args.compute_eta:
@@ -184,13 +192,23 @@ def main():
     if args.kpoints:
         print(json.dumps(kpoint_bench(oh, F, args.kpoint_steps, L=args.kpoint_L)), flush=True)
         return 0
+    if args.group:
+        print(json.dumps(group_bench(oh, F, args.group_steps, args.group_count)), flush=True)
+        return 0
 
-    L = args.L
-    nb, no = 4 * L ** 3, 2 * L ** 3
+    if args.dims:
+        dims = tuple(int(x) for x in args.dims.split(","))
+        L = dims if len(set(dims)) > 1 else dims[0]
+    else:
+        dims, L = (args.L,) * 3, args.L
+    nb = 4 * dims[0] * dims[1] * dims[2]
+    no = nb // 2
     eta = args.eta if args.eta is not None else ETA_MID_GAP.get(L)
-    if eta is None:
+    if eta is None and nb <= 16384:
         eta, _ = compute_eta(L)
-    log(f"bench: L={L} n={nb} m={no} eta={eta:.6f}")
+    if eta is None:
+        eta = GROUP_ETA  # no dense eigensolve at this size; the shift near half filling of this generator
+    log(f"bench: lattice {dims} n={nb} m={no} eta={eta:.6f}")
     t = time.time()
     Hb = oh.lattice_bsr(L, 0)
     Sb = oh.lattice_bsr(L, 1)
@@ -218,6 +236,22 @@ def main():
             log(f"bench: {lab} {i2:.3f} CG it/s ({d2:.3f}s)")
             res[lab] = (i2, d2, p2)
 
+    if args.mixed_steps > 0:
+        # the mixed solve's low-precision phase (FP32 state, BF16 Grams, FP32 gradient): K steps,
+        # then one FP64 refinement step; the rate comes from the per-step timestamps
+        K = args.mixed_steps
+        F.set_option("lowp_max_iter", K + 1)
+        F.set_option("max_iter", 1)
+        Cm.upload(C0)
+        F.sync()
+        rm = F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=1e-300, operation="hmx", dealloc=True)
+        F.set_option("lowp_max_iter", 0)
+        F.set_option("max_iter", 0)
+        assert rm["icg_lowp"] == K + 1, (rm["icg_lowp"], K)
+        tl = rm["t_log"]
+        res["mixed_lowp_ips"] = K / (tl[K] - tl[0])
+        log(f"bench: mixed low-precision phase {res['mixed_lowp_ips']:.3f} CG it/s ({K} steps)")
+
     # roofline of the FP64 GEMM (dominant kernel) from the timed FP64 call: algorithmic flops
     # (2 m n k per product, counted by the library per launch) over the summed event durations
     steps = args.steps
@@ -241,7 +275,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded lattice generator)",
-        "config": {"workload": f"omm flavour 0, {L}x{L}x{L} lattice, n={nb}, m={no}, eta={eta:.6f} (mid-gap)",
+        "config": {"workload": f"omm flavour 0, {'x'.join(map(str, dims))} lattice, n={nb}, m={no}, eta={eta:.6f}"
+                               + (" (mid-gap)" if L in ETA_MID_GAP else ""),
                    "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
         "roofline": {"bound": "mfma",
                      "kernel": "k_gemm FP64 Gram and state products (v_mfma_f64_16x16x4f64, 128x128 tiles)",
@@ -268,6 +303,8 @@ def main():
             g = res[lab][2]["gemm_pair"]  # the gradient pair runs in FP32 in both low-precision labels
             line[f"{key}_products_gradient_TFLOPs"] = round(g[2] / max(g[0] * 1e-3, 1e-12) / 1e12, 2)
 
+    if "mixed_lowp_ips" in res:
+        line["mixed_low_precision_iters_per_s"] = round(res["mixed_lowp_ips"], 4)
     if args.ttc:
         line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log)
 
@@ -359,6 +396,88 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
                        "mixed": {"e": rm["e_log"].tolist(), "t": (rm["t_log"] - rm["t0"]).tolist(),
                                  "icg_low_precision": rm["icg_lowp"]}}, f)
     return out
+
+
+GROUP_ETA = 0.0517
+
+
+def group_bench(oh, F, steps, count, seed=None, stream=6):
+    """Uneven-size batch: `count` real systems with cubic edges L drawn from U{6..20} by the
+    counter-based generator (stream 6), H shifted by a common eta near half filling.  One
+    omx_group solve (one graph launch per CG step of all systems, tiles autotuned per shape in the
+    eager first pass) against one omm() call per system, both over the same CG steps."""
+    seed = oh.DEFAULT_SEED if seed is None else seed
+    Ls = [6 + int(15 * oh.u01(seed, stream, s)) for s in range(count)]
+    eta = GROUP_ETA
+    mem = sum(44.0 * (4 * L ** 3) ** 2 for L in Ls)
+    log(f"bench group: {count} systems, L = {Ls}, state {mem / 1e9:.1f} GB")
+    res = {"systems": count, "L": Ls, "n": [4 * L ** 3 for L in Ls], "eta": eta, "state_GB": mem / 1e9,
+           "steps": steps, "dtype": "f64"}
+    ctx = oh.Ctx(0)
+    ctx.autotune(True)
+    mats = []
+    for L in Ls:
+        cp, ri, h = oh.lattice_bsr(L, 0)
+        _, _, sv = oh.lattice_bsr(L, 1)
+        nb = 4 * L ** 3
+        mats.append((oh.Bsr(ctx, nb, cp, ri, h - eta * sv), oh.Bsr(ctx, nb, cp, ri, sv)))
+    g = oh.Group(ctx, [2 * L ** 3 for L in Ls], [m[0] for m in mats], [m[1] for m in mats])
+
+    def run_group(nsteps):
+        for k, L in enumerate(Ls):
+            g.set_c(k, oh.random_c0(2 * L ** 3, 4 * L ** 3))
+        ctx.sync()
+        t0 = time.perf_counter()
+        g.solve([eta] * count, cg_tol=1e-14, max_steps=nsteps, check_every=nsteps, use_graph=True)
+        return time.perf_counter() - t0
+
+    t_first = run_group(1)  # tuning, capture and one step
+    t1 = run_group(1)
+    tn = run_group(1 + steps)
+    res["group_first_call_s"] = t_first
+    res["group_s_per_step_all_systems"] = (tn - t1) / steps
+    res["tuned_shapes"] = ctx.autotune_report().count("\n")
+    log(f"bench group: {res['group_s_per_step_all_systems']:.3f} s per CG step of all systems "
+        f"(first call with tuning {t_first:.1f} s)")
+    g.free()
+    for h, sv in mats:
+        h.free()
+        sv.free()
+    ctx.close()
+    # sequential: one omm() call per system through the Fortran drop-in, tuned tiles as well
+    F.setup(0)
+    oh.check(oh.lib().omx_autotune(F.ctx_handle(), 1), "autotune")
+
+    def run_seq(nsteps):
+        F.set_option("max_iter", nsteps)
+        tot = 0.0
+        for L in Ls:
+            nb, no = 4 * L ** 3, 2 * L ** 3
+            cp, ri, h = oh.lattice_bsr(L, 0)
+            _, _, sv = oh.lattice_bsr(L, 1)
+            Hm, Sm = F.Mat.from_bsr(nb, cp, ri, h - eta * sv), F.Mat.from_bsr(nb, cp, ri, sv)
+            Cm = F.Mat(no, nb)
+            Cm.upload(oh.random_c0(no, nb))
+            F.sync()
+            t0 = time.perf_counter()
+            F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-14, dealloc=True)
+            F.sync()
+            tot += time.perf_counter() - t0
+            for M in (Hm, Sm, Cm):
+                M.free()
+        F.set_option("max_iter", 0)
+        return tot
+
+    run_seq(1)  # tuning
+    s1 = run_seq(1)
+    sn = run_seq(1 + steps)
+    res["sequential_s_per_step_all_systems"] = (sn - s1) / steps
+    res["group_speedup"] = res["sequential_s_per_step_all_systems"] / res["group_s_per_step_all_systems"]
+    flops = sum(12.0 * (2 * L ** 3) ** 2 * (4 * L ** 3) for L in Ls)
+    res["group_TFLOPs"] = flops / res["group_s_per_step_all_systems"] / 1e12
+    log(f"bench group: sequential {res['sequential_s_per_step_all_systems']:.3f} s per step; "
+        f"group speedup {res['group_speedup']:.2f}x, {res['group_TFLOPs']:.1f} TF/s")
+    return {"group_batch": res}
 
 
 def kpoint_bench(oh, F, steps, L=10, nk1=4, eta=0.05):
