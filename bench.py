@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true", help="skip the BF16 / FP32 step-rate runs")
     ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
+    ap.add_argument("--ttc-bf16x3", action="store_true",
+                    help="with --ttc, also time the mixed solve with a BF16x3 gradient")
     ap.add_argument("--ttc-log", default=None, help="write the per-step energies of the --ttc solves (JSON) here")
     ap.add_argument("--kpoints", action="store_true",
                     help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
@@ -306,7 +308,9 @@ def main():
     if "mixed_lowp_ips" in res:
         line["mixed_low_precision_iters_per_s"] = round(res["mixed_lowp_ips"], 4)
     if args.ttc:
-        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log)
+        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log,
+                                                    variants=("mixed", "mixed_bf16x3") if args.ttc_bf16x3
+                                                    else ("mixed",))
 
     if not args.no_cpu:
         t = time.time()
@@ -344,7 +348,10 @@ def exact_eigs(nb):
     return None
 
 
-def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None):
+def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None, variants=("mixed",)):
+    """FP64 omm() and the mixed-precision variants ("mixed": FP32 gradient, "mixed_bf16x3": BF16x3
+    gradient) from the same C0 to the reference's e_diff <= tol; energies and density matrices
+    compared with the FP64 result and, when tabulated, with the exact eigenvalue sum."""
     H, S, Cm = mats
     out = {"cg_tol": tol}
     Dref = F.Mat(nb, nb)
@@ -357,44 +364,49 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
     t64 = time.perf_counter() - t
     out["fp64"] = {"seconds": t64, "icg": r["icg"], "e_min": r["e_min"], "converged": r["converged"]}
     log(f"bench: ttc FP64 {t64:.2f}s icg={r['icg']} e={r['e_min']:.12f}")
-    Dmx = F.Mat(nb, nb)
-    Cm.upload(C0)
-    F.sync()
-    t = time.perf_counter()
-    rm = F.omm(nb, no, H, S, Cm, D_min=Dmx, init_C=True, eta=eta, cg_tol=tol, operation="hmx", dealloc=True)
-    F.sync()
-    tmx = time.perf_counter() - t
-    # ||D_mx - D_64||_F on the device
     p1, ld = Dref.dptr()
-    p2, _ = Dmx.dptr()
-    oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
-    fro = ctypes.c_double(0)
-    oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
     fro_ref = ctypes.c_double(0)
     oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p1, ld, p1, ld, ctypes.byref(fro_ref)), "fro")
-    out["mixed"] = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
-                    "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
-                    "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
-                    "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
-                    "density_rel_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0) / max(fro_ref.value, 1e-300))),
-                    "speedup_vs_fp64": t64 / tmx}
-    # both results against the exact eigenvalue sum of this lattice when it is tabulated
     exact = exact_eigs(nb)
     if exact is not None:
-        ex = exact["sum_lowest_m"]
-        out["exact"] = {"sum_lowest_m": ex, "gap_at_m": exact["gap_at_m"], "source": EIGS_FILE,
-                        "fp64_energy_rel_err": abs(r["e_min"] - ex) / abs(ex),
-                        "mixed_energy_rel_err": abs(rm["e_min"] - ex) / abs(ex)}
+        out["exact"] = {"sum_lowest_m": exact["sum_lowest_m"], "gap_at_m": exact["gap_at_m"], "source": EIGS_FILE,
+                        "fp64_energy_rel_err": abs(r["e_min"] - exact["sum_lowest_m"]) / abs(exact["sum_lowest_m"])}
+    logs = {"fp64": {"e": r["e_log"].tolist(), "t": (r["t_log"] - r["t0"]).tolist()}}
+    for name in variants:
+        F.set_option("lowp_grad_bf16x3", 1 if name == "mixed_bf16x3" else 0)
+        Dmx = F.Mat(nb, nb)
+        Cm.upload(C0)
+        F.sync()
+        t = time.perf_counter()
+        rm = F.omm(nb, no, H, S, Cm, D_min=Dmx, init_C=True, eta=eta, cg_tol=tol, operation="hmx", dealloc=True)
+        F.sync()
+        tmx = time.perf_counter() - t
+        F.set_option("lowp_grad_bf16x3", 0)
+        # ||D_mx - D_64||_F on the device
+        p2, _ = Dmx.dptr()
+        oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
+        fro = ctypes.c_double(0)
+        oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
+        v = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
+             "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
+             "gradient": "bf16x3" if name == "mixed_bf16x3" else "fp32",
+             "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
+             "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
+             "density_rel_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0) / max(fro_ref.value, 1e-300))),
+             "speedup_vs_fp64": t64 / tmx}
+        if exact is not None:
+            v["energy_rel_err_vs_exact"] = abs(rm["e_min"] - exact["sum_lowest_m"]) / abs(exact["sum_lowest_m"])
+        out[name] = v
+        log(f"bench: ttc {name} {tmx:.2f}s icg={rm['icg']} (low precision {rm['icg_lowp']}) e={rm['e_min']:.12f} "
+            f"speedup {t64 / tmx:.2f}x")
+        logs[name] = {"e": rm["e_log"].tolist(), "t": (rm["t_log"] - rm["t0"]).tolist(),
+                      "icg_low_precision": rm["icg_lowp"]}
+        Dmx.free()
     F.set_option("progress", 0)
-    log(f"bench: ttc mixed {tmx:.2f}s icg={rm['icg']} (low precision {rm['icg_lowp']}) e={rm['e_min']:.12f} "
-        f"speedup {t64 / tmx:.2f}x")
     Dref.free()
-    Dmx.free()
     if log_path:
         with open(log_path, "w") as f:
-            json.dump({"fp64": {"e": r["e_log"].tolist(), "t": (r["t_log"] - r["t0"]).tolist()},
-                       "mixed": {"e": rm["e_log"].tolist(), "t": (rm["t_log"] - rm["t0"]).tolist(),
-                                 "icg_low_precision": rm["icg_lowp"]}}, f)
+            json.dump(logs, f)
     return out
 
 

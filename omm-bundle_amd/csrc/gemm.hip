@@ -552,6 +552,233 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                                                  C, ldc);
 }
 
+// ---------------------------------------------------------------------------------------------
+// BF16 products of FP32 matrices through k-contiguous BF16 copies.
+//
+// The operands are first rounded to BF16 (round to nearest even, the same rounding as the
+// staging conversion of k_gemm) into k-contiguous panels: op(A) as At[kk + i*K] and op(B) as
+// Bt[kk + j*K].  The GEMM then moves half the bytes of FP32 operands, loads 16-byte vectors of 8
+// consecutive k straight into the i-major LDS tiles, and skips the per-tile conversion.  With
+// split = true each operand is also written as a second BF16 term, lo = bf16(a - hi), and the
+// product runs over a K three times longer, [Ah Ah Al] x [Bh; Bl; Bh]: hi*hi + hi*lo + lo*hi,
+// which keeps about 16 significant bits of each operand (the BF16x3 product).
+// ---------------------------------------------------------------------------------------------
+
+constexpr int BG_T = 32;  // transpose tile
+
+// dst := bf16(src) (and dstlo := bf16(src - hi)) with src rows x cols (ld lds).  TRANS: element
+// (r, c) goes to dst[c + r*ldd], else to dst[r + c*ldd].
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const float *__restrict__ src, int64_t lds,
+                                                 __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, int64_t ldd) {
+    const int tx = threadIdx.x % BG_T, ty = threadIdx.x / BG_T;
+    const int r0 = blockIdx.x * BG_T, c0 = blockIdx.y * BG_T;
+    if constexpr (TRANS) {
+        __shared__ float tile[BG_T][BG_T + 1];
+        for (int c = ty; c < BG_T; c += 256 / BG_T) {
+            const int r = r0 + tx, cc = c0 + c;
+            tile[c][tx] = (r < rows && cc < cols) ? src[(int64_t)r + (int64_t)cc * lds] : 0.0f;
+        }
+        __syncthreads();
+        for (int r = ty; r < BG_T; r += 256 / BG_T) {
+            const int rr = r0 + r, cc = c0 + tx;  // write along c (contiguous in dst)
+            if (rr >= rows || cc >= cols) continue;
+            const float v = tile[tx][r];
+            const __bf16 h = (__bf16)v;
+            hi[(int64_t)cc + (int64_t)rr * ldd] = h;
+            if (lo) lo[(int64_t)cc + (int64_t)rr * ldd] = (__bf16)(v - (float)h);
+        }
+    } else {
+        for (int c = ty; c < BG_T; c += 256 / BG_T) {
+            const int r = r0 + tx, cc = c0 + c;
+            if (r >= rows || cc >= cols) continue;
+            const float v = src[(int64_t)r + (int64_t)cc * lds];
+            const __bf16 h = (__bf16)v;
+            hi[(int64_t)r + (int64_t)cc * ldd] = h;
+            if (lo) lo[(int64_t)r + (int64_t)cc * ldd] = (__bf16)(v - (float)h);
+        }
+    }
+}
+
+// C := alpha * sum_kk At[kk + i*lda] * Bt[kk + j*ldb] + beta0*C0 + beta*C over kk < K (K % 64 == 0,
+// lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation and result.
+template <class CF>
+__global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int K, double alpha,
+                                                     const __bf16 *__restrict__ At, int64_t lda,
+                                                     const __bf16 *__restrict__ Bt, int64_t ldb, double beta0,
+                                                     const float *__restrict__ C0, double beta,
+                                                     float *__restrict__ C, int64_t ldc) {
+    typedef LdsT<OMX_PREC_BF16, CF::BM> LA;
+    typedef LdsT<OMX_PREC_BF16, CF::BN> LB;
+    constexpr int BK = Prec<OMX_PREC_BF16>::BK;
+    constexpr int VPR = BK / 8;  // 16-byte vectors per row of a slice
+    constexpr int NVA = CF::BM * VPR / CF::NT, NVB = CF::BN * VPR / CF::NT;
+    static_assert(NVA * CF::NT == CF::BM * VPR && NVB * CF::NT == CF::BN * VPR, "slice not divisible");
+    __shared__ __bf16 smem[2 * (LA::ELEMS + LB::ELEMS)];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave / CF::WGN, wn = wave % CF::WGN;
+    const int tiles_m = (m + CF::BM - 1) / CF::BM, tiles_n = (n + CF::BN - 1) / CF::BN;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int i0 = tm * CF::BM, j0 = tn * CF::BN;
+
+    f4v acc[CF::FBM][CF::FBN];
+#pragma unroll
+    for (int a = 0; a < CF::FBM; ++a)
+#pragma unroll
+        for (int b = 0; b < CF::FBN; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+
+    bf8v ra[NVA], rb[NVB];
+    const bf8v zero = {};
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NVA; ++q) {
+            const int idx = t + CF::NT * q, r = idx / VPR, kv = (idx % VPR) * 8;
+            const int gi = i0 + r;
+            ra[q] = gi < m ? *reinterpret_cast<const bf8v *>(At + (int64_t)gi * lda + k0 + kv) : zero;
+        }
+#pragma unroll
+        for (int q = 0; q < NVB; ++q) {
+            const int idx = t + CF::NT * q, r = idx / VPR, kv = (idx % VPR) * 8;
+            const int gj = j0 + r;
+            rb[q] = gj < n ? *reinterpret_cast<const bf8v *>(Bt + (int64_t)gj * ldb + k0 + kv) : zero;
+        }
+    };
+    auto stash = [&](__bf16 *As, __bf16 *Bs) {
+#pragma unroll
+        for (int q = 0; q < NVA; ++q) {
+            const int idx = t + CF::NT * q;
+            *reinterpret_cast<bf8v *>(&As[LA::off(idx / VPR, (idx % VPR) * 8)]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NVB; ++q) {
+            const int idx = t + CF::NT * q;
+            *reinterpret_cast<bf8v *>(&Bs[LB::off(idx / VPR, (idx % VPR) * 8)]) = rb[q];
+        }
+    };
+
+    __bf16 *As0 = smem, *Bs0 = smem + LA::ELEMS;
+    __bf16 *As1 = smem + LA::ELEMS + LB::ELEMS, *Bs1 = As1 + LA::ELEMS;
+    const int nslices = K / BK;
+    if (nslices > 0) {
+        fetch(0);
+        stash(As0, Bs0);
+        __syncthreads();
+        for (int s = 0; s < nslices; ++s) {
+            __bf16 *Ac = (s & 1) ? As1 : As0, *Bc = (s & 1) ? Bs1 : Bs0;
+            __bf16 *An = (s & 1) ? As0 : As1, *Bn = (s & 1) ? Bs0 : Bs1;
+            const bool more = (s + 1 < nslices);
+            if (more) fetch((s + 1) * BK);
+            mma_slice<OMX_PREC_BF16, CF>(Ac, Bc, acc, wm, wn, lane);
+            if (more) stash(An, Bn);
+            __syncthreads();
+        }
+    }
+
+    const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < CF::FBN; ++fn) {
+            const int j = j0 + wn * CF::WN + fn * 16 + li;
+            if (j >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wm * CF::WM + fm * 16 + Prec<OMX_PREC_BF16>::out_row(lq, v);
+                if (i >= m) continue;
+                double r = alpha * (double)acc[fm][fn][v];
+                if (C0) r += beta0 * (double)C0[(int64_t)i + (int64_t)j * ldc];
+                if (beta != 0.0) r += beta * (double)C[(int64_t)i + (int64_t)j * ldc];
+                C[(int64_t)i + (int64_t)j * ldc] = (float)r;
+            }
+        }
+}
+
+// dst row r := src row r over k elements (rows of k-contiguous panels; k a multiple of 8, rows
+// 16-byte aligned).
+__global__ __launch_bounds__(256) void k_copy_cols_bf16(int rows, int k, const __bf16 *__restrict__ src, int64_t lds,
+                                                        __bf16 *__restrict__ dst, int64_t ldd) {
+    for (int r = blockIdx.x; r < rows; r += gridDim.x)
+        for (int q = threadIdx.x * 8; q < k; q += 256 * 8)
+            *reinterpret_cast<bf8v *>(dst + (int64_t)r * ldd + q) =
+                *reinterpret_cast<const bf8v *>(src + (int64_t)r * lds + q);
+}
+
+// One operand term of a BF16 panel product: op(X) with X rows x cols (ld ldx); as a k-contiguous
+// panel it is `extent` x K (extent = m for A, n for B).
+struct BgTerm {
+    const float *X;
+    int64_t ldx;
+    bool trans;  // true: X is stored K x extent (already k-contiguous)
+};
+
+static int bg_convert(omx_ctx ctx, const BgTerm &tm, int extent, int K, __bf16 *hi, __bf16 *lo, int64_t ldd) {
+    if (tm.trans) {
+        // X is K x extent with k contiguous: same layout, dst[kk + e*ldd]
+        dim3 grid((K + BG_T - 1) / BG_T, (extent + BG_T - 1) / BG_T);
+        hipLaunchKernelGGL(k_to_bf16<false>, grid, dim3(256), 0, ctx->stream, K, extent, tm.X, tm.ldx, hi, lo, ldd);
+    } else {
+        // X is extent x K: transpose so that dst[kk + e*ldd] = X(e, kk)
+        dim3 grid((extent + BG_T - 1) / BG_T, (K + BG_T - 1) / BG_T);
+        hipLaunchKernelGGL(k_to_bf16<true>, grid, dim3(256), 0, ctx->stream, extent, K, tm.X, tm.ldx, hi, lo, ldd);
+    }
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+// C := alpha * sum_t op(A_t) op(B_t) + beta0*C0 + beta*C for nterm FP32 products of inner size k,
+// in BF16 (split = false) or BF16x3 (split = true) through k-contiguous panels in the workspace.
+// Returns OMX_ERR_UNSUPPORTED (nothing launched) when the shape does not suit the panels.
+static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTerm *A, const BgTerm *B, bool split,
+                        double alpha, double beta0, const float *C0, double beta, float *C, int64_t ldc) {
+    constexpr int BK = Prec<OMX_PREC_BF16>::BK;
+    if (ctx->launch_mask || k % BK != 0 || m < 256 || n < 256 || k < 4 * BK) return OMX_ERR_UNSUPPORTED;
+    const int reps = split ? 3 : 1;
+    const int64_t K = (int64_t)k * nterm * reps;
+    OMX_REQUIRE(K < (1LL << 31), "bf16 panels: K too large");
+    const size_t bytes_a = sizeof(__bf16) * (size_t)K * m, bytes_b = sizeof(__bf16) * (size_t)K * n;
+    int rc = ensure_ws(ctx, bytes_a + bytes_b + 256);
+    if (rc) return rc;
+    __bf16 *At = (__bf16 *)ctx->ws;
+    __bf16 *Bt = (__bf16 *)((char *)ctx->ws + ((bytes_a + 255) / 256) * 256);
+    hipEvent_t ev;
+    prof_begin(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
+    prof_work(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)k * nterm,
+              4.0 * nterm * ((double)m * k + (double)k * n) + 4.0 * (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0)));
+    // Panel layout along K: for term t, reps blocks of k: split -> [hi, hi, lo] for A and
+    // [hi, lo, hi] for B.  Rows of At are m panels of length K (ld K), rows of Bt n of length K.
+    for (int tt = 0; tt < nterm && !rc; ++tt) {
+        const int64_t kb = (int64_t)tt * reps * k;
+        if (!split) {
+            rc = bg_convert(ctx, A[tt], m, k, At + kb, nullptr, K);
+            if (!rc) rc = bg_convert(ctx, B[tt], n, k, Bt + kb, nullptr, K);
+        } else {
+            // A: hi at kb and lo at kb + 2k, then hi copied to kb + k; B: hi at kb, lo at kb + k,
+            // hi copied to kb + 2k
+            rc = bg_convert(ctx, A[tt], m, k, At + kb, At + kb + 2 * k, K);
+            if (!rc) {
+                hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)m, 65535)), dim3(256), 0,
+                                   ctx->stream, m, k, At + kb, K, At + kb + k, K);
+                rc = bg_convert(ctx, B[tt], n, k, Bt + kb, Bt + kb + k, K);
+            }
+            if (!rc)
+                hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)n, 65535)), dim3(256), 0,
+                                   ctx->stream, n, k, Bt + kb, K, Bt + kb + 2 * k, K);
+        }
+    }
+    if (!rc) {
+        typedef Cfg4 CF;
+        const int tiles = ((m + CF::BM - 1) / CF::BM) * ((n + CF::BN - 1) / CF::BN);
+        hipLaunchKernelGGL(k_bgemm_tn<CF>, dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, (int)K, alpha, At, K, Bt,
+                           K, beta0, C0, beta, C, ldc);
+        rc = hipGetLastError() == hipSuccess ? OMX_OK : OMX_ERR_HIP;
+        if (rc) set_error("bf16 panel GEMM launch failed");
+    }
+    prof_end(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, ev);
+    return rc;
+}
+
 template <typename TS, int PREC>
 static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
                         int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
@@ -576,6 +803,14 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
         if (prec == OMX_PREC_BF16)
             return dispatch_ops<double, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     } else if (dt == OMX_F32) {
+        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3) {
+            // k-contiguous BF16 panels (A 't' and B 'n' are already k-contiguous)
+            const BgTerm a{(const float *)A, lda, trA}, b{(const float *)B, ldb, !trB};
+            int rc = bgemm_panels(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
+                                  (float *)C, ldc);
+            if (rc != OMX_ERR_UNSUPPORTED) return rc;
+            if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;  // the panels do not cover this shape
+        }
         if (prec == OMX_PREC_F32)
             return dispatch_ops<float, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
         if (prec == OMX_PREC_BF16)
@@ -589,6 +824,14 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc) {
+    if (dt == OMX_F32 && (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3)) {
+        const BgTerm a[2] = {{(const float *)A1, lda, false}, {(const float *)A2, lda, false}};
+        const BgTerm b[2] = {{(const float *)B1, ldb, true}, {(const float *)B2, ldb, true}};
+        int rc = bgemm_panels(ctx, m, n, k, 2, a, b, prec == OMX_PREC_BF16X3, alpha, beta0, (const float *)C0, beta,
+                              (float *)C, ldc);
+        if (rc != OMX_ERR_UNSUPPORTED) return rc;
+        if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;
+    }
     const int bk = (prec == OMX_PREC_F64)   ? Prec<OMX_PREC_F64>::BK
                    : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK
                                             : Prec<OMX_PREC_BF16>::BK;

@@ -475,12 +475,12 @@ contains
           call m_add(HC, 'n', G, 1.5_dp, 1.0_dp, scale_label())
         else
           if (rewin_now) then
-            ! HC = C H, HW = HC C^T, SC = C S, SW = SC C^T at the gradient's precision
-            call calc_AW(H, pC, pHW, HC, grad_label(lab))
+            ! HC = C H, HW = HC C^T, SC = C S, SW = SC C^T at the state's precision (FP32)
+            call calc_AW(H, pC, pHW, HC, state_label(lab))
             if (std_form) then
-              call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, grad_label(lab))
+              call mm_multiply(pC, 'n', pC, 'c', pSW, 1.0_dp, 0.0_dp, state_label(lab))
             else
-              call calc_AW(S, pC, pSW, pSC, grad_label(lab))
+              call calc_AW(S, pC, pSW, pSC, state_label(lab))
             end if
           else
             call m_add(HG, 'n', HC, x_min(ip), 1.0_dp, scale_label())

@@ -27,6 +27,7 @@ module omm_hip_ops
   logical, save :: opt_fused = .true.        ! use fused device kernels when all operands are on device
   integer, save :: opt_progress = 0          ! > 0: one line on stderr every opt_progress CG steps
   logical, save :: opt_grad_fp32 = .true.    ! BF16 phase: gradient and state products in FP32
+  logical, save :: opt_grad_b3 = .false.     ! BF16 phase: gradient in BF16x3 (state products in FP32)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
@@ -143,14 +144,24 @@ contains
     character(3) :: g
     g = lab
     if (lab == 'hbf' .and. opt_grad_fp32) g = 'h32'
+    if (lab == 'hbf' .and. opt_grad_b3) g = 'hb3'
   end function grad_label
+
+  !> Label of the periodic recomputation of HC, SC, HW, SW in the low-precision phase: FP32 unless
+  !> the gradient itself runs in plain BF16.
+  function state_label(lab) result(g)
+    character(3), intent(in) :: lab
+    character(3) :: g
+    g = lab
+    if (lab == 'hbf' .and. (opt_grad_fp32 .or. opt_grad_b3)) g = 'h32'
+  end function state_label
 
   !> Reductions always accumulate in FP64; low-precision product labels map to 'hip' for them.
   function trace_label(lab) result(t)
     character(3), intent(in) :: lab
     character(3) :: t
     t = lab
-    if (lab == 'h32' .or. lab == 'hbf') t = 'hip'
+    if (lab == 'h32' .or. lab == 'hbf' .or. lab == 'hb3') t = 'hip'
   end function trace_label
 
   !> Minimum of the quartic line-search polynomial (omm_quartic.F90:60-153, Numerical Recipes cubic).
@@ -346,6 +357,8 @@ contains
       opt_lowp_rewin = nint(value)
     case ('lowp_grad_fp32')
       opt_grad_fp32 = (value /= 0.0_dp)
+    case ('lowp_grad_bf16x3')
+      opt_grad_b3 = (value /= 0.0_dp)
     case default
       call omm_die('omm_set_option: unknown option '//name)
     end select

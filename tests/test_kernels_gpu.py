@@ -73,6 +73,71 @@ def test_gemm_f32_storage(ctx):
     assert np.max(np.abs(dC.download() - ref)) <= 1e-4 * np.sqrt(k)
 
 
+def _bf16_of_f32(x):
+    """Round-to-nearest-even of float32 values to bfloat16, as float64."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) & np.uint64(0xFFFF0000)
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _split3(x):
+    """The BF16x3 operand terms of float32 values: hi = bf16(x), lo = bf16(x - hi) (in FP32)."""
+    hi = _bf16_of_f32(x)
+    lo = _bf16_of_f32((np.asarray(x, np.float32) - hi.astype(np.float32)).astype(np.float32))
+    return hi, lo
+
+
+@pytest.mark.parametrize("opA,opB", OPS)
+@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
+def test_gemm_bf16_panels(ctx, opA, opB, prec):
+    # FP32 storage, shapes the k-contiguous BF16 panels cover (m, n >= 256, k % 64 == 0): BF16
+    # against the product of the rounded operands; BF16x3 against hi*hi + hi*lo + lo*hi and, much
+    # tighter than BF16, against the exact product
+    m, n, k = 300, 260, 512
+    _note("gemm_bf16_panels", prec, opA, opB)
+    A = rng.standard_normal((k, m) if opA == "t" else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if opB == "t" else (k, n)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    dA, dB = oh.DMat.from_host(ctx, A, oh.OMX_F32), oh.DMat.from_host(ctx, B, oh.OMX_F32)
+    dC = oh.DMat.from_host(ctx, C0, oh.OMX_F32)
+    oh.gemm(ctx, opA, opB, dA, dB, dC, alpha=0.5, beta=-1.0, prec=prec)
+    got = dC.download().astype(np.float64)
+    op = lambda X, o: X.T if o == "t" else X  # noqa: E731
+    exact = 0.5 * op(A.astype(np.float64), opA) @ op(B.astype(np.float64), opB) - C0
+    if prec == oh.PREC_BF16:
+        ref = 0.5 * op(_bf16_of_f32(A), opA) @ op(_bf16_of_f32(B), opB) - C0
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
+    else:
+        Ah, Al = _split3(A)
+        Bh, Bl = _split3(B)
+        ref = 0.5 * (op(Ah, opA) @ op(Bh, opB) + op(Ah, opA) @ op(Bl, opB) + op(Al, opA) @ op(Bh, opB)) - C0
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
+        bf = 0.5 * op(_bf16_of_f32(A), opA) @ op(_bf16_of_f32(B), opB) - C0
+        assert np.max(np.abs(got - exact)) < np.max(np.abs(bf - exact)) / 50
+
+
+@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
+@pytest.mark.parametrize("n,m", [(256, 512), (200, 300)])
+def test_calc_G_bf16_panels(ctx, prec, n, m):
+    # the gradient pair on FP32 storage: n % 64 == 0 runs on the panels, n = 200 falls back (BF16
+    # staging path, or FP32 for BF16x3)
+    _note("calc_G_bf16_panels", prec, n, m)
+    HW, SW = (rng.standard_normal((n, n)).astype(np.float32) for _ in range(2))
+    HC, SC = (rng.standard_normal((n, m)).astype(np.float32) for _ in range(2))
+    d = [oh.DMat.from_host(ctx, M, oh.OMX_F32) for M in (HW, SW, HC, SC)]
+    G = oh.DMat(ctx, n, m, oh.OMX_F32)
+    oh.omm_calc_G(ctx, d[0], d[1], d[2], d[3], G, prec=prec)
+    f = lambda X: X.astype(np.float64)  # noqa: E731
+    exact = 4 * f(HC) - 2 * f(SW) @ f(HC) - 2 * f(HW) @ f(SC)
+    got = G.download().astype(np.float64)
+    if prec == oh.PREC_BF16:
+        b = _bf16_of_f32
+        ref = 4 * f(HC) - 2 * b(SW) @ b(HC) - 2 * b(HW) @ b(SC)
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(n) * 8
+    else:
+        assert np.max(np.abs(got - exact)) <= 2e-4 * np.sqrt(n)
+
+
 @pytest.mark.parametrize("L,no", [(3, 54), (4, 128), (4, 127), (5, 250)])
 def test_spmm_lattice(ctx, L, no):
     _note("spmm", L, no)

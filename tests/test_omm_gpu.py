@@ -148,3 +148,34 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
     assert abs(r["e_min"] - ref["e_min"]) <= 1e-8 * abs(ref["e_min"])
     for M in (Hm, Sm, Cm):
         M.free()
+
+
+def test_mixed_precision_bf16x3_gradient():
+    """'hmx' with the BF16x3 gradient (hi*hi + hi*lo + lo*hi of the FP32 operands on BF16 MFMA) at
+    the 8x8x8 lattice, where the Gram and gradient shapes run on the k-contiguous BF16 panels:
+    the same bars as the FP32-gradient form."""
+    _note("mixed bf16x3")
+    L, eta = 8, 0.2
+    nb, Hb, Sb = lattice(L)
+    no = nb // 2
+    H, S = oh.bsr_to_dense(nb, *Hb), oh.bsr_to_dense(nb, *Sb)
+    w = np.linalg.eigvalsh(np.linalg.solve(np.linalg.cholesky(S), np.linalg.solve(np.linalg.cholesky(S), H).T))
+    exact = float(np.sum(w[:no][w[:no] < eta]))
+    Hs = H - eta * S
+    C0 = oh.random_c0(no, nb)
+    Hm, Sm = F.Mat.from_csc(nb, *csc_of(Hs)), F.Mat.from_csc(nb, *csc_of(S))
+    Cm = F.Mat(no, nb, data=C0)
+    ref = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True)
+    Cm.upload(C0)
+    F.set_option("lowp_grad_bf16x3", 1)
+    try:
+        r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True, operation="hmx")
+    finally:
+        F.set_option("lowp_grad_bf16x3", 0)
+    print("CASE mixed bf16x3 icg", r["icg"], "lowp", r["icg_lowp"], "fp64 icg", ref["icg"], "err",
+          abs(r["e_min"] - exact) / abs(exact), "fp64 err", abs(ref["e_min"] - exact) / abs(exact), flush=True)
+    assert r["converged"] and r["icg_lowp"] > 0
+    assert abs(r["e_min"] - exact) <= max(2.0 * abs(ref["e_min"] - exact), 1e-10 * abs(exact))
+    assert abs(r["e_min"] - ref["e_min"]) <= 1e-8 * abs(ref["e_min"])
+    for M in (Hm, Sm, Cm):
+        M.free()
