@@ -265,7 +265,7 @@ def main():
     coeff_gbs = coeff_by / (coeff_ms * 1e-3) / 1e9
 
     line = {
-        "metric": "OMM CG iterations/s (FP64, n=16384, m=8192); time-to-converge |dE|/E<1e-10 with --ttc",
+        "metric": f"OMM CG iterations/s (FP64, n={nb}, m={no}); time-to-converge |dE|/E<1e-10 with --ttc",
         "value": round(ips, 4),
         "unit": "CG iters/s",
         "n_gpus": 1,
