@@ -266,10 +266,16 @@ int omx_omm_update_w(omx_ctx ctx, int dt, int n, int64_t ld, double x, const voi
 typedef struct omx_zbatch_s *omx_zbatch;
 int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
                       const double *hval, const double *sval, omx_zbatch *out);
+/* The same with the state in dt = OMX_C128 or OMX_C64 (complex64: FP32 MFMA products, operator
+ * values rounded to FP32 in the SpMM; traces, coefficients and the line search stay FP64).
+ * omx_zbatch_create is omx_zbatch_create_dt with OMX_C128. */
+int omx_zbatch_create_dt(omx_ctx ctx, int dt, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
+                         const double *hval, const double *sval, omx_zbatch *out);
 int omx_zbatch_destroy(omx_zbatch b);
-/* Coefficient matrices of all systems (host, complex128 (re, im), no x nb column-major each). */
-int omx_zbatch_set_c(omx_zbatch b, const double *C);
-int omx_zbatch_get_c(omx_zbatch b, double *C);
+/* Coefficient matrices of all systems (host, in the batch's element type, (re, im), no x nb
+ * column-major each). */
+int omx_zbatch_set_c(omx_zbatch b, const void *C);
+int omx_zbatch_get_c(omx_zbatch b, void *C);
 /* Run up to max_steps CG steps (reading the convergence flags every check_every steps; use_graph
  * replays one captured step).  Returns per system e_min + eta*TrQS, the step count icg and the
  * converged flag (e_diff <= cg_tol, cg_tol < 0: 1e-9). */

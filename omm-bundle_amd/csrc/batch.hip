@@ -37,10 +37,11 @@ struct omx_zbatch_s {
     int64_t nnzb = 0;
     int *colptr = nullptr, *rowidx = nullptr;
     double *hval = nullptr, *sval = nullptr;  // nsys * nnzb * 32
+    int dt = OMX_C128;  // element type of the state: OMX_C128 or OMX_C64
     // state, system s at s * (no*nb) or s * (no*no) complex elements, ld = no
-    double2 *C = nullptr, *HC = nullptr, *SC = nullptr, *G = nullptr, *Gp = nullptr, *D = nullptr, *HG = nullptr,
-            *SG = nullptr;
-    double2 *HW = nullptr, *SW = nullptr, *HWd = nullptr, *SWd = nullptr, *HWdd = nullptr, *SWdd = nullptr;
+    void *C = nullptr, *HC = nullptr, *SC = nullptr, *G = nullptr, *Gp = nullptr, *D = nullptr, *HG = nullptr,
+         *SG = nullptr;
+    void *HW = nullptr, *SW = nullptr, *HWd = nullptr, *SWd = nullptr, *HWdd = nullptr, *SWdd = nullptr;
     // per-system scalars
     double *coeff = nullptr;  // 18 per system
     double *dots = nullptr;   // 2 per system
@@ -319,72 +320,83 @@ static int *flag(omx_zbatch b, int which) { return b->active + which * b->nsys; 
 enum { F_ACTIVE = 0, F_FAIL, F_CONT, F_GRAD, F_ICG, F_CONV, F_NFAIL, F_COUNT };
 
 // G = 4HC - 2SW*HC - 2HW*SC for the systems with mask[s] (calc_G, omm_ops.F90:17-39): one pair
-// launch, or three passes when no is not a multiple of the K slice.
+// launch, or three passes when no is not a multiple of the K slice.  T: double2 or Cx<float>.
+template <typename T>
 static int batch_calc_G(omx_zbatch b, const int *mask) {
     omx_ctx ctx = b->ctx;
     const int ns = b->nsys, no = b->no, nb = b->nb;
     const int64_t lenC = (int64_t)no * nb, lenW = (int64_t)no * no;
-    int rc = launch_zgemm_pair_batched(ctx, no, nb, no, -2.0, b->SW, b->HW, no, lenW, b->HC, b->SC, no, lenC, 4.0,
-                                       b->HC, 0.0, b->G, no, lenC, ns, mask);
+    int rc = launch_zgemm_pair_batched(ctx, b->dt, no, nb, no, -2.0, b->SW, b->HW, no, lenW, b->HC, b->SC, no, lenC,
+                                       4.0, b->HC, 0.0, b->G, no, lenC, ns, mask);
     if (rc != OMX_ERR_UNSUPPORTED) return rc;
-    hipLaunchKernelGGL(k_batch_scale_copy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, 4.0,
-                       mask);
-    if ((rc = launch_zgemm_batched(ctx, 'n', 'n', no, nb, no, -2.0, b->SW, no, lenW, b->HC, no, lenC, 1.0, b->G, no,
-                                   lenC, ns, mask)))
+    hipLaunchKernelGGL(k_batch_scale_copy<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->HC,
+                       (T *)b->G, 4.0, mask);
+    if ((rc = launch_zgemm_batched(ctx, b->dt, 'n', 'n', no, nb, no, -2.0, b->SW, no, lenW, b->HC, no, lenC, 1.0, b->G,
+                                   no, lenC, ns, mask)))
         return rc;
-    return launch_zgemm_batched(ctx, 'n', 'n', no, nb, no, -2.0, b->HW, no, lenW, b->SC, no, lenC, 1.0, b->G, no, lenC,
-                                ns, mask);
+    return launch_zgemm_batched(ctx, b->dt, 'n', 'n', no, nb, no, -2.0, b->HW, no, lenW, b->SC, no, lenC, 1.0, b->G,
+                                no, lenC, ns, mask);
 }
 
 // One CG step of every system (the sequence that is captured).
+template <typename T>
 static int batch_step(omx_zbatch b) {
     omx_ctx ctx = b->ctx;
-    const int ns = b->nsys, no = b->no, nb = b->nb;
+    const int ns = b->nsys, no = b->no, nb = b->nb, dt = b->dt;
     const int64_t lenC = (int64_t)no * nb, lenW = (int64_t)no * no, sV = 32 * b->nnzb;
     int *act = flag(b, F_ACTIVE), *failf = flag(b, F_FAIL), *cont = flag(b, F_CONT), *grad = flag(b, F_GRAD);
     int rc;
-    hipLaunchKernelGGL(k_batch_direction<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->G, b->D, b->Gp,
-                       b->sc, act);
-    if ((rc = launch_spmm_z_batched(ctx, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->D, no, lenC, b->HG, no,
-                                    lenC, ns, act)))
+    hipLaunchKernelGGL(k_batch_direction<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->G,
+                       (T *)b->D, (T *)b->Gp, b->sc, act);
+    if ((rc = launch_spmm_z_batched(ctx, dt, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->D, no, lenC, b->HG,
+                                    no, lenC, ns, act)))
         return rc;
-    if ((rc = launch_spmm_z_batched(ctx, no, nb, b->nbc, b->colptr, b->rowidx, b->sval, sV, b->D, no, lenC, b->SG, no,
-                                    lenC, ns, act)))
+    if ((rc = launch_spmm_z_batched(ctx, dt, no, nb, b->nbc, b->colptr, b->rowidx, b->sval, sV, b->D, no, lenC, b->SG,
+                                    no, lenC, ns, act)))
         return rc;
-    const double2 *src[4] = {b->HC, b->SC, b->HG, b->SG};
-    double2 *dst[4] = {b->HWd, b->SWd, b->HWdd, b->SWdd};
+    const void *src[4] = {b->HC, b->SC, b->HG, b->SG};
+    void *dst[4] = {b->HWd, b->SWd, b->HWdd, b->SWdd};
     for (int q = 0; q < 4; ++q)
-        if ((rc = launch_zgemm_batched(ctx, 'n', 'c', no, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC, 0.0, dst[q],
-                                       no, lenW, ns, act)))
+        if ((rc = launch_zgemm_batched(ctx, dt, 'n', 'c', no, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC, 0.0,
+                                       dst[q], no, lenW, ns, act)))
             return rc;
-    if ((rc = launch_coeff_batched(ctx, OMX_C128, no, no, lenW, b->HW, b->SW, b->HWd, b->SWd, b->HWdd, b->SWdd, ns,
-                                     b->partial, b->coeff, nullptr)))
+    if ((rc = launch_coeff_batched(ctx, dt, no, no, lenW, b->HW, b->SW, b->HWd, b->SWd, b->HWdd, b->SWdd, ns,
+                                   b->partial, b->coeff, nullptr)))
         return rc;
     hipLaunchKernelGGL(k_batch_quartic, dim3((ns + 63) / 64), dim3(64), 0, ctx->stream, ns, b->coeff, b->sc, act,
                        failf, cont, grad, flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->tol);
     // C and SW for every system that stepped (omm.F90:470-502)
-    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->D, b->C, b->sc, act,
-                       failf, 0.5);
+    hipLaunchKernelGGL(k_batch_axpy<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->D,
+                       (T *)b->C, b->sc, act, failf, 0.5);
     const dim3 wgrid((no + 31) / 32, (no + 31) / 32, ns);
-    hipLaunchKernelGGL(k_batch_update_w<double2>, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->SWd, b->SWdd, b->SW, b->sc, act,
-                       failf);
+    hipLaunchKernelGGL(k_batch_update_w<T>, wgrid, dim3(BT_NT), 0, ctx->stream, no, (const T *)b->SWd,
+                       (const T *)b->SWdd, (T *)b->SW, b->sc, act, failf);
     // HW, HC, SC for the systems still running (omm.F90:511-527)
-    hipLaunchKernelGGL(k_batch_update_w<double2>, wgrid, dim3(BT_NT), 0, ctx->stream, no, b->HWd, b->HWdd, b->HW, b->sc, cont,
-                       failf);
-    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HG, b->HC, b->sc, cont,
-                       failf, 0.5);
-    hipLaunchKernelGGL(k_batch_axpy<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->SG, b->SC, b->sc, cont,
-                       failf, 0.5);
-    hipLaunchKernelGGL(k_batch_fail_grad<double2>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, b->HC, b->G, b->Gp,
-                       cont, failf);
+    hipLaunchKernelGGL(k_batch_update_w<T>, wgrid, dim3(BT_NT), 0, ctx->stream, no, (const T *)b->HWd,
+                       (const T *)b->HWdd, (T *)b->HW, b->sc, cont, failf);
+    hipLaunchKernelGGL(k_batch_axpy<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->HG,
+                       (T *)b->HC, b->sc, cont, failf, 0.5);
+    hipLaunchKernelGGL(k_batch_axpy<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->SG,
+                       (T *)b->SC, b->sc, cont, failf, 0.5);
+    hipLaunchKernelGGL(k_batch_fail_grad<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->HC,
+                       (T *)b->G, (T *)b->Gp, cont, failf);
     // G = 4HC - 2SW*HC - 2HW*SC where the line search succeeded, then Polak-Ribiere
-    if ((rc = batch_calc_G(b, grad))) return rc;
-    if ((rc = launch_pr_batched(ctx, OMX_C128, no, nb, no, lenC, b->G, b->Gp, ns, b->partial2, b->dots, nullptr))) return rc;
+    if ((rc = batch_calc_G<T>(b, grad))) return rc;
+    if ((rc = launch_pr_batched(ctx, dt, no, nb, no, lenC, b->G, b->Gp, ns, b->partial2, b->dots, nullptr)))
+        return rc;
     hipLaunchKernelGGL(k_batch_lambda, dim3((ns + 63) / 64), dim3(64), 0, ctx->stream, ns, b->dots, b->sc, grad, failf,
                        cont, act);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
+
+static int batch_step_any(omx_zbatch b) {
+    return b->dt == OMX_C64 ? batch_step<Cx<float>>(b) : batch_step<double2>(b);
+}
+static int batch_calc_G_any(omx_zbatch b, const int *mask) {
+    return b->dt == OMX_C64 ? batch_calc_G<Cx<float>>(b, mask) : batch_calc_G<double2>(b, mask);
+}
+static size_t batch_esize(omx_zbatch b) { return b->dt == OMX_C64 ? 8 : 16; }
 
 static void batch_free(omx_zbatch b) {
     if (!b) return;
@@ -401,10 +413,11 @@ static void batch_free(omx_zbatch b) {
 
 extern "C" {
 
-int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
-                      const double *hval, const double *sval, omx_zbatch *out) {
+int omx_zbatch_create_dt(omx_ctx ctx, int dt, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
+                         const double *hval, const double *sval, omx_zbatch *out) {
     OMX_REQUIRE(ctx && out && nsys > 0 && no > 0 && nb > 0 && bcolptr && browidx && hval && sval,
                 "omx_zbatch_create: bad arguments");
+    OMX_REQUIRE(dt == OMX_C128 || dt == OMX_C64, "omx_zbatch_create: element type %d is not complex", dt);
     OMX_REQUIRE(nsys < 65536, "omx_zbatch_create: too many systems");
     const int nbc = (nb + 3) / 4;
     OMX_REQUIRE(bcolptr[0] == 0, "omx_zbatch_create: bcolptr must be 0-based");
@@ -416,6 +429,7 @@ int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr,
     }
     omx_zbatch b = new omx_zbatch_s();
     b->ctx = ctx;
+    b->dt = dt;
     b->nsys = nsys;
     b->no = no;
     b->nb = nb;
@@ -426,10 +440,11 @@ int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr,
               hipMalloc(&b->rowidx, sizeof(int) * std::max<int64_t>(b->nnzb, 1)) == hipSuccess &&
               hipMalloc(&b->hval, sizeof(double) * 32 * std::max<int64_t>(b->nnzb, 1) * nsys) == hipSuccess &&
               hipMalloc(&b->sval, sizeof(double) * 32 * std::max<int64_t>(b->nnzb, 1) * nsys) == hipSuccess;
-    double2 **nm[] = {&b->C, &b->HC, &b->SC, &b->G, &b->Gp, &b->D, &b->HG, &b->SG};
-    for (auto p : nm) ok = ok && hipMalloc(p, sizeof(double2) * lenC) == hipSuccess;
-    double2 **nn[] = {&b->HW, &b->SW, &b->HWd, &b->SWd, &b->HWdd, &b->SWdd};
-    for (auto p : nn) ok = ok && hipMalloc(p, sizeof(double2) * lenW) == hipSuccess;
+    const size_t es = batch_esize(b);
+    void **nm[] = {&b->C, &b->HC, &b->SC, &b->G, &b->Gp, &b->D, &b->HG, &b->SG};
+    for (auto p : nm) ok = ok && hipMalloc(p, es * lenC) == hipSuccess;
+    void **nn[] = {&b->HW, &b->SW, &b->HWd, &b->SWd, &b->HWdd, &b->SWdd};
+    for (auto p : nn) ok = ok && hipMalloc(p, es * lenW) == hipSuccess;
     ok = ok && hipMalloc(&b->coeff, sizeof(double) * 18 * nsys) == hipSuccess &&
          hipMalloc(&b->dots, sizeof(double) * 2 * nsys) == hipSuccess &&
          hipMalloc(&b->sc, sizeof(double) * 4 * nsys) == hipSuccess &&
@@ -453,22 +468,27 @@ int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr,
     return OMX_OK;
 }
 
+int omx_zbatch_create(omx_ctx ctx, int nsys, int no, int nb, const int *bcolptr, const int *browidx,
+                      const double *hval, const double *sval, omx_zbatch *out) {
+    return omx_zbatch_create_dt(ctx, OMX_C128, nsys, no, nb, bcolptr, browidx, hval, sval, out);
+}
+
 int omx_zbatch_destroy(omx_zbatch b) {
     batch_free(b);
     return OMX_OK;
 }
 
-int omx_zbatch_set_c(omx_zbatch b, const double *C) {
+int omx_zbatch_set_c(omx_zbatch b, const void *C) {
     OMX_REQUIRE(b && C, "omx_zbatch_set_c: bad arguments");
-    OMX_CHECK_HIP(hipMemcpyAsync(b->C, C, sizeof(double2) * (size_t)b->no * b->nb * b->nsys, hipMemcpyHostToDevice,
+    OMX_CHECK_HIP(hipMemcpyAsync(b->C, C, batch_esize(b) * (size_t)b->no * b->nb * b->nsys, hipMemcpyHostToDevice,
                                  b->ctx->stream));
     OMX_CHECK_HIP(hipStreamSynchronize(b->ctx->stream));
     return OMX_OK;
 }
 
-int omx_zbatch_get_c(omx_zbatch b, double *C) {
+int omx_zbatch_get_c(omx_zbatch b, void *C) {
     OMX_REQUIRE(b && C, "omx_zbatch_get_c: bad arguments");
-    OMX_CHECK_HIP(hipMemcpyAsync(C, b->C, sizeof(double2) * (size_t)b->no * b->nb * b->nsys, hipMemcpyDeviceToHost,
+    OMX_CHECK_HIP(hipMemcpyAsync(C, b->C, batch_esize(b) * (size_t)b->no * b->nb * b->nsys, hipMemcpyDeviceToHost,
                                  b->ctx->stream));
     OMX_CHECK_HIP(hipStreamSynchronize(b->ctx->stream));
     return OMX_OK;
@@ -487,23 +507,23 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
     hipLaunchKernelGGL(k_batch_init_flags, dim3((ns + 63) / 64), dim3(64), 0, st, ns, flag(b, F_ACTIVE),
                        flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->sc);
     int *act = flag(b, F_ACTIVE);
-    if ((rc = launch_spmm_z_batched(ctx, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->C, no, lenC, b->HC, no,
+    if ((rc = launch_spmm_z_batched(ctx, b->dt, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->C, no, lenC, b->HC, no,
                                     lenC, ns, act)))
         return rc;
-    if ((rc = launch_spmm_z_batched(ctx, no, nb, b->nbc, b->colptr, b->rowidx, b->sval, sV, b->C, no, lenC, b->SC, no,
+    if ((rc = launch_spmm_z_batched(ctx, b->dt, no, nb, b->nbc, b->colptr, b->rowidx, b->sval, sV, b->C, no, lenC, b->SC, no,
                                     lenC, ns, act)))
         return rc;
-    if ((rc = launch_zgemm_batched(ctx, 'n', 'c', no, no, nb, 1.0, b->HC, no, lenC, b->C, no, lenC, 0.0, b->HW, no,
+    if ((rc = launch_zgemm_batched(ctx, b->dt, 'n', 'c', no, no, nb, 1.0, b->HC, no, lenC, b->C, no, lenC, 0.0, b->HW, no,
                                    lenW, ns, act)))
         return rc;
-    if ((rc = launch_zgemm_batched(ctx, 'n', 'c', no, no, nb, 1.0, b->SC, no, lenC, b->C, no, lenC, 0.0, b->SW, no,
+    if ((rc = launch_zgemm_batched(ctx, b->dt, 'n', 'c', no, no, nb, 1.0, b->SC, no, lenC, b->C, no, lenC, 0.0, b->SW, no,
                                    lenW, ns, act)))
         return rc;
-    if ((rc = batch_calc_G(b, act))) return rc;
+    if ((rc = batch_calc_G_any(b, act))) return rc;
     // the CG steps: captured once, replayed; flags read every check_every steps
     if (use_graph && !b->exec) {
         OMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = batch_step(b);
+        rc = batch_step_any(b);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
         if (rc) return rc;
@@ -518,7 +538,7 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
         for (int q = 0; q < burst; ++q) {
             if (use_graph)
                 OMX_CHECK_HIP(hipGraphLaunch(b->exec, st));
-            else if ((rc = batch_step(b)))
+            else if ((rc = batch_step_any(b)))
                 return rc;
         }
         done_steps += burst;
@@ -538,9 +558,9 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
     OMX_CHECK_HIP(hipStreamSynchronize(st));
     for (int s = 0; s < ns; ++s) {
         double tr[2], dd[2];
-        const double2 *W = b->SW + (size_t)s * lenW;
-        if ((rc = omx_trace(ctx, OMX_C128, no, W, no, tr))) return rc;
-        if ((rc = omx_mm_trace(ctx, OMX_C128, no, no, W, no, W, no, dd))) return rc;
+        const void *W = (const char *)b->SW + batch_esize(b) * (size_t)s * lenW;
+        if ((rc = omx_trace(ctx, b->dt, no, W, no, tr))) return rc;
+        if ((rc = omx_mm_trace(ctx, b->dt, no, no, W, no, W, no, dd))) return rc;
         const double TrQS = 2.0 * tr[0] - dd[0];
         e_min[s] = sc[4 * s + 2] + eta * TrQS;
         icg[s] = h_icg[s];

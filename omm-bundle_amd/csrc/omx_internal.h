@@ -195,16 +195,16 @@ int launch_zgemm_pair(omx_ctx ctx, int dt, int m, int n, int k, double alpha, co
                       double beta, void *C, int64_t ldc);
 // Batched complex128 products (blockIdx.y = system; strides in complex elements; systems with
 // active[s] == 0 skipped; no split-K).  Ops: ('n','c') and ('n','n').
-int launch_zgemm_batched(omx_ctx ctx, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+int launch_zgemm_batched(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
                          int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, double beta, void *C,
                          int64_t ldc, int64_t sC, int nbatch, const int *active);
-int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, const void *A1, const void *A2,
+int launch_zgemm_pair_batched(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,
                               int64_t lda, int64_t sA, const void *B1, const void *B2, int64_t ldb, int64_t sB,
                               double beta0, const void *C0, double beta, void *C, int64_t ldc, int64_t sC, int nbatch,
                               const int *active);
 
-// Batched pieces of the k-point solver (complex128, systems sharing one block pattern).
-int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colptr, const int *rowidx,
+// Batched pieces of the k-point solver (complex128 or complex64, systems sharing one block pattern).
+int launch_spmm_z_batched(omx_ctx ctx, int dt, int no, int nb, int nbc, const int *colptr, const int *rowidx,
                           const double *val, int64_t sV, const void *X, int64_t ldx, int64_t sX, void *Y, int64_t ldy,
                           int64_t sY, int nbatch, const int *active);
 // Line-search coefficients (to out[18*s ...]: the 5 coefficients, then the 13 traces) and

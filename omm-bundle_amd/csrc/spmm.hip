@@ -421,9 +421,10 @@ static int run_spmm_z(omx_ctx ctx, int no, int nb, double alpha, const R *X, int
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
-// Batched complex128 SpMM over nbatch systems sharing one block pattern: Y_s := X_s * A_s (beta 0),
-// A_s values at val + s*sV (doubles), X_s, Y_s at s*sX, s*sY (complex elements).
-int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colptr, const int *rowidx,
+// Batched complex SpMM (complex128, or complex64 with the FP64 operator values rounded to FP32)
+// over nbatch systems sharing one block pattern: Y_s := X_s * A_s (beta 0), A_s values at
+// val + s*sV (doubles), X_s, Y_s at s*sX, s*sY (complex elements).
+int launch_spmm_z_batched(omx_ctx ctx, int dt, int no, int nb, int nbc, const int *colptr, const int *rowidx,
                           const double *val, int64_t sV, const void *X, int64_t ldx, int64_t sX, void *Y, int64_t ldy,
                           int64_t sY, int nbatch, const int *active) {
     constexpr int VEC = 1, ROWS = 64 * VEC, WPG = 4;
@@ -431,9 +432,14 @@ int launch_spmm_z_batched(omx_ctx ctx, int no, int nb, int nbc, const int *colpt
     const int jgroups = (nbc + WPG - 1) / WPG;
     const int64_t total = (int64_t)nbands * jgroups;
     OMX_REQUIRE(total < (1LL << 31) && nbatch < 65536, "batched spmm: grid too large");
-    hipLaunchKernelGGL((k_spmm_bsr4_z<double, VEC, WPG>), dim3((unsigned)total, nbatch), dim3(64 * WPG), 0,
-                       ctx->stream, no, nb, nbc, 1.0, (const double *)X, ldx, colptr, rowidx, val, 0.0, (double *)Y,
-                       ldy, nbands, jgroups, 2 * sX, 2 * sY, sV, active);
+    if (dt == OMX_C64)
+        hipLaunchKernelGGL((k_spmm_bsr4_z<float, VEC, WPG>), dim3((unsigned)total, nbatch), dim3(64 * WPG), 0,
+                           ctx->stream, no, nb, nbc, 1.0, (const float *)X, ldx, colptr, rowidx, val, 0.0, (float *)Y,
+                           ldy, nbands, jgroups, 2 * sX, 2 * sY, sV, active);
+    else
+        hipLaunchKernelGGL((k_spmm_bsr4_z<double, VEC, WPG>), dim3((unsigned)total, nbatch), dim3(64 * WPG), 0,
+                           ctx->stream, no, nb, nbc, 1.0, (const double *)X, ldx, colptr, rowidx, val, 0.0,
+                           (double *)Y, ldy, nbands, jgroups, 2 * sX, 2 * sY, sV, active);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

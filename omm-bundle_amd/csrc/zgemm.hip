@@ -351,8 +351,25 @@ int launch_zgemm(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, d
     return OMX_ERR_UNSUPPORTED;
 }
 
-// Batched complex128 products over nbatch systems with element strides sA, sB, sC (result and C0).
-int launch_zgemm_batched(omx_ctx ctx, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+// Batched complex products (complex128 or complex64) over nbatch systems with element strides sA, sB,
+// sC (result and C0).
+template <typename R>
+static int zgemm_batched_r(omx_ctx ctx, char opA, char opB, int m, int n, int k, double alpha, const void *A,
+                           int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc, int nbatch,
+                           const ZBatch &bat) {
+    const bool ta = (opA != 'n' && opA != 'N');
+    const bool tb = (opB != 'n' && opB != 'N'), cb = (opB == 'c' || opB == 'C');
+    if (!ta && tb && cb)  // the Gram products X Y^H
+        return run_zgemm<R, false, false, true, true, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr,
+                                                         beta, C, ldc, nbatch, bat, false);
+    if (!ta && !tb)
+        return run_zgemm<R, false, false, false, false, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr,
+                                                           beta, C, ldc, nbatch, bat, false);
+    set_error("batched complex product: op (%c,%c) not available", opA, opB);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+int launch_zgemm_batched(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
                          int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, double beta, void *C,
                          int64_t ldc, int64_t sC, int nbatch, const int *active) {
     ZBatch bat;
@@ -360,20 +377,15 @@ int launch_zgemm_batched(omx_ctx ctx, char opA, char opB, int m, int n, int k, d
     bat.sB = sB;
     bat.sC = sC;
     bat.active = active;
-    const bool ta = (opA != 'n' && opA != 'N'), ca = (opA == 'c' || opA == 'C');
-    const bool tb = (opB != 'n' && opB != 'N'), cb = (opB == 'c' || opB == 'C');
-    if (!ta && tb && cb)  // the Gram products X Y^H
-        return run_zgemm<double, false, false, true, true, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr,
-                                                              beta, C, ldc, nbatch, bat, false);
-    if (!ta && !tb)
-        return run_zgemm<double, false, false, false, false, 1>(ctx, m, n, k, alpha, A, A, lda, B, B, ldb, 0.0,
-                                                                nullptr, beta, C, ldc, nbatch, bat, false);
-    (void)ca;
-    set_error("batched complex product: op (%c,%c) not available", opA, opB);
+    if (dt == OMX_C128)
+        return zgemm_batched_r<double>(ctx, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nbatch, bat);
+    if (dt == OMX_C64)
+        return zgemm_batched_r<float>(ctx, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nbatch, bat);
+    set_error("batched complex product: unsupported element type %d", dt);
     return OMX_ERR_UNSUPPORTED;
 }
 
-int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, const void *A1, const void *A2,
+int launch_zgemm_pair_batched(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,
                               int64_t lda, int64_t sA, const void *B1, const void *B2, int64_t ldb, int64_t sB,
                               double beta0, const void *C0, double beta, void *C, int64_t ldc, int64_t sC, int nbatch,
                               const int *active) {
@@ -383,6 +395,9 @@ int launch_zgemm_pair_batched(omx_ctx ctx, int m, int n, int k, double alpha, co
     bat.sB = sB;
     bat.sC = sC;
     bat.active = active;
+    if (dt == OMX_C64)
+        return run_zgemm<float, false, false, false, false, 2>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                               C0, beta, C, ldc, nbatch, bat, false);
     return run_zgemm<double, false, false, false, false, 2>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0,
                                                             beta, C, ldc, nbatch, bat, false);
 }

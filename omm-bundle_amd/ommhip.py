@@ -54,6 +54,7 @@ SIGNATURES = {
     "omx_potrf": (_i, [_vp, _i, _i, _vp, _i64, C.POINTER(_i)]),
     "omx_zbatch_create": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "omx_zbatch_destroy": (_i, [_vp]),
+    "omx_zbatch_create_dt": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "omx_zbatch_set_c": (_i, [_vp, _vp]),
     "omx_zbatch_get_c": (_i, [_vp, _vp]),
     "omx_zbatch_solve": (_i, [_vp, _d, _d, _i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
@@ -531,24 +532,26 @@ def get_element(ctx, M: DMat, i, j) -> float:
 class ZBatch:
     """Batched complex128 k-point solver (omx_zbatch_*): nsys systems sharing one block pattern."""
 
-    def __init__(self, ctx: Ctx, no: int, nb: int, cp, ri, hvals, svals):
-        """hvals, svals: lists (one per system) of complex (nnzb, 16) block values, H already shifted."""
-        self.ctx, self.no, self.nb, self.nsys = ctx, no, nb, len(hvals)
+    def __init__(self, ctx: Ctx, no: int, nb: int, cp, ri, hvals, svals, dt: int = OMX_C128):
+        """hvals, svals: lists (one per system) of complex (nnzb, 16) block values, H already shifted;
+        dt: element type of the state (OMX_C128 or OMX_C64)."""
+        self.ctx, self.no, self.nb, self.nsys, self.dt = ctx, no, nb, len(hvals), dt
         cp = np.ascontiguousarray(cp, np.int32)
         ri = np.ascontiguousarray(ri, np.int32)
         hv = np.ascontiguousarray(np.stack(hvals), np.complex128)
         sv = np.ascontiguousarray(np.stack(svals), np.complex128)
         h = C.c_void_p()
-        check(lib().omx_zbatch_create(ctx.h, self.nsys, no, nb, _ptr(cp), _ptr(ri), _ptr(hv), _ptr(sv), C.byref(h)),
-              "omx_zbatch_create")
+        check(lib().omx_zbatch_create_dt(ctx.h, dt, self.nsys, no, nb, _ptr(cp), _ptr(ri), _ptr(hv), _ptr(sv),
+                                         C.byref(h)), "omx_zbatch_create")
         self.h = h
 
     def set_c(self, Cs):
-        a = np.ascontiguousarray(np.stack([np.asfortranarray(c, np.complex128).ravel(order="F") for c in Cs]))
+        t = _NP[self.dt]
+        a = np.ascontiguousarray(np.stack([np.asfortranarray(c, t).ravel(order="F") for c in Cs]))
         check(lib().omx_zbatch_set_c(self.h, _ptr(a)), "zbatch_set_c")
 
     def get_c(self):
-        a = np.zeros((self.nsys, self.no * self.nb), np.complex128)
+        a = np.zeros((self.nsys, self.no * self.nb), _NP[self.dt])
         check(lib().omx_zbatch_get_c(self.h, _ptr(a)), "zbatch_get_c")
         return [a[s].reshape((self.no, self.nb), order="F") for s in range(self.nsys)]
 

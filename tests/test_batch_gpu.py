@@ -89,3 +89,36 @@ def test_batch_graph_replay_equals_eager(ctx):
     assert np.array_equal(r0["e_min"], r1["e_min"]) and np.array_equal(r0["icg"], r1["icg"])
     for a, b in zip(c0, c1):
         assert np.array_equal(a, b)
+
+
+def test_batch_complex64_matches_closed_form(ctx):
+    # the k-point batch with complex64 state (FP32 MFMA products, FP64 traces and line search):
+    # every system converged to cg_tol = 1e-7 and within 1e-5 relative of the closed form, as a
+    # single complex64 omm() solve
+    L = 3
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    phis = [(0.0, 0.0, 0.0), (0.25, 0.5, 0.75), (0.0, 0.5, 0.25)]  # a common gap (0.051, 0.134)
+    _note("zbatch complex64")
+    hv, sv, ws = [], [], []
+    for phi in phis:
+        cp, ri, h = oh.lattice_bsr_k(L, 0, phi)
+        _, _, s = oh.lattice_bsr_k(L, 1, phi)
+        ws.append(sla.eigh(oh.bsr_to_dense(nb, cp, ri, h), oh.bsr_to_dense(nb, cp, ri, s), eigvals_only=True))
+        hv.append(h)
+        sv.append(s)
+    # one shift for the batch, inside every system's gap
+    lo, hi = max(w[no - 1] for w in ws), min(w[no] for w in ws)
+    assert lo < hi
+    eta = 0.5 * (lo + hi)
+    exact = [float(np.sum(w[:no])) for w in ws]
+    hv = [h - eta * s for h, s in zip(hv, sv)]
+    C0 = oh.random_c0(no, nb).astype(np.complex64)
+    B = oh.ZBatch(ctx, no, nb, cp, ri, hv, sv, dt=oh.OMX_C64)
+    B.set_c([C0] * len(phis))
+    r = B.solve(eta=eta, cg_tol=1e-7, max_steps=20000, check_every=16, use_graph=True)
+    _note("zbatch c64 icg", r["icg"].tolist(), "e", r["e_min"].tolist(), "exact", exact)
+    assert r["converged"].all()
+    for k in range(len(phis)):
+        assert abs(r["e_min"][k] - exact[k]) < 1e-5 * abs(exact[k])
+    assert B.get_c()[0].dtype == np.complex64
+    B.free()
