@@ -367,13 +367,15 @@ contains
     real(dp), intent(in) :: tol
     integer, intent(in) :: max_steps
     logical, intent(in) :: lowp
-    integer :: outer, inner, steps, n_below
+    integer :: outer, inner, steps, n_below, n_stall
     logical :: ls_conv, capped, rewin_now
-    real(dp) :: e_pred
+    real(dp) :: e_pred, e_pred_min
     conv = .false.
     capped = .false.
     steps = 0
     n_below = 0
+    n_stall = 0
+    e_pred_min = huge(1.0_dp)
     stat_t0 = wall_seconds()  ! the coefficient readback that precedes this point left the stream idle
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
@@ -452,6 +454,15 @@ contains
               n_below = 0
             end if
             if (n_below >= opt_lowp_consec) conv = .true.
+            ! precision floor of the phase: no new smallest predicted decrease for opt_lowp_stall
+            ! steps; the FP64 refinement takes over from there
+            if (e_pred < e_pred_min) then
+              e_pred_min = e_pred
+              n_stall = 0
+            else
+              n_stall = n_stall + 1
+            end if
+            if (opt_lowp_stall > 0 .and. n_stall >= opt_lowp_stall) conv = .true.
           end if
         else if (e_diff <= tol) then
           conv = .true.

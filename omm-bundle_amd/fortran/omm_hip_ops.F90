@@ -28,6 +28,8 @@ module omm_hip_ops
   integer, save :: opt_progress = 0          ! > 0: one line on stderr every opt_progress CG steps
   logical, save :: opt_grad_fp32 = .true.    ! BF16 phase: gradient and state products in FP32
   logical, save :: opt_grad_b3 = .false.     ! BF16 phase: gradient in BF16x3 (state products in FP32)
+  integer, save :: opt_lowp_stall = 50       ! low-precision phase ends after this many steps without a
+                                             ! new smallest predicted decrease (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
@@ -359,6 +361,8 @@ contains
       opt_grad_fp32 = (value /= 0.0_dp)
     case ('lowp_grad_bf16x3')
       opt_grad_b3 = (value /= 0.0_dp)
+    case ('lowp_stall')
+      opt_lowp_stall = max(0, nint(value))
     case default
       call omm_die('omm_set_option: unknown option '//name)
     end select
