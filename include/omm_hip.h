@@ -46,9 +46,9 @@ extern "C" {
 #define OMX_PREC_F64 0  /* FP64 MFMA, FP64 accumulate (the oracle-grade path) */
 #define OMX_PREC_F32 1  /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
 #define OMX_PREC_BF16 2 /* operands rounded to BF16, BF16 MFMA, FP32 accumulate */
-/* FP32 operands split into two BF16 terms (hi = bf16(a), lo = bf16(a - hi)); hi*hi + hi*lo +
- * lo*hi on BF16 MFMA with FP32 accumulation, about 16 significant bits per operand (FP32 storage
- * only; shapes the BF16 panels do not cover run as OMX_PREC_F32) */
+/* Operands split into two BF16 terms (hi = bf16(a), lo = bf16(a - hi)); hi*hi + hi*lo + lo*hi on
+ * BF16 MFMA with FP32 accumulation, about 16 significant bits per operand (real FP32 or FP64
+ * storage; shapes the BF16 panels do not cover run as OMX_PREC_F32) */
 #define OMX_PREC_BF16X3 3
 
 /* ---- counter-based generator streams (lattice inputs) ------------------------------------- */

@@ -566,48 +566,50 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
 
 constexpr int BG_T = 32;  // transpose tile
 
-// dst := bf16(src) (and dstlo := bf16(src - hi)) with src rows x cols (ld lds).  TRANS: element
-// (r, c) goes to dst[c + r*ldd], else to dst[r + c*ldd].
-template <bool TRANS>
-__global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const float *__restrict__ src, int64_t lds,
+// dst := bf16(src) (and dstlo := bf16(src - hi), in the source precision) with src rows x cols
+// (ld lds), FP32 or FP64, each rounded once from the stored value.  TRANS: element (r, c) goes to
+// dst[c + r*ldd], else to dst[r + c*ldd].
+template <bool TRANS, typename TS>
+__global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *__restrict__ src, int64_t lds,
                                                  __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, int64_t ldd) {
     const int tx = threadIdx.x % BG_T, ty = threadIdx.x / BG_T;
     const int r0 = blockIdx.x * BG_T, c0 = blockIdx.y * BG_T;
     if constexpr (TRANS) {
-        __shared__ float tile[BG_T][BG_T + 1];
+        __shared__ TS tile[BG_T][BG_T + 1];
         for (int c = ty; c < BG_T; c += 256 / BG_T) {
             const int r = r0 + tx, cc = c0 + c;
-            tile[c][tx] = (r < rows && cc < cols) ? src[(int64_t)r + (int64_t)cc * lds] : 0.0f;
+            tile[c][tx] = (r < rows && cc < cols) ? src[(int64_t)r + (int64_t)cc * lds] : (TS)0;
         }
         __syncthreads();
         for (int r = ty; r < BG_T; r += 256 / BG_T) {
             const int rr = r0 + r, cc = c0 + tx;  // write along c (contiguous in dst)
             if (rr >= rows || cc >= cols) continue;
-            const float v = tile[tx][r];
+            const TS v = tile[tx][r];
             const __bf16 h = (__bf16)v;
             hi[(int64_t)cc + (int64_t)rr * ldd] = h;
-            if (lo) lo[(int64_t)cc + (int64_t)rr * ldd] = (__bf16)(v - (float)h);
+            if (lo) lo[(int64_t)cc + (int64_t)rr * ldd] = (__bf16)(v - (TS)h);
         }
     } else {
         for (int c = ty; c < BG_T; c += 256 / BG_T) {
             const int r = r0 + tx, cc = c0 + c;
             if (r >= rows || cc >= cols) continue;
-            const float v = src[(int64_t)r + (int64_t)cc * lds];
+            const TS v = src[(int64_t)r + (int64_t)cc * lds];
             const __bf16 h = (__bf16)v;
             hi[(int64_t)r + (int64_t)cc * ldd] = h;
-            if (lo) lo[(int64_t)r + (int64_t)cc * ldd] = (__bf16)(v - (float)h);
+            if (lo) lo[(int64_t)r + (int64_t)cc * ldd] = (__bf16)(v - (TS)h);
         }
     }
 }
 
 // C := alpha * sum_kk At[kk + i*lda] * Bt[kk + j*ldb] + beta0*C0 + beta*C over kk < K (K % 64 == 0,
-// lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation and result.
-template <class CF>
+// lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation; C, C0 of type TO (the
+// storage type, FP32 or FP64).
+template <class CF, typename TO>
 __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int K, double alpha,
                                                      const __bf16 *__restrict__ At, int64_t lda,
                                                      const __bf16 *__restrict__ Bt, int64_t ldb, double beta0,
-                                                     const float *__restrict__ C0, double beta,
-                                                     float *__restrict__ C, int64_t ldc) {
+                                                     const TO *__restrict__ C0, double beta, TO *__restrict__ C,
+                                                     int64_t ldc) {
     typedef LdsT<OMX_PREC_BF16, CF::BM> LA;
     typedef LdsT<OMX_PREC_BF16, CF::BN> LB;
     constexpr int BK = Prec<OMX_PREC_BF16>::BK;
@@ -690,7 +692,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
                 double r = alpha * (double)acc[fm][fn][v];
                 if (C0) r += beta0 * (double)C0[(int64_t)i + (int64_t)j * ldc];
                 if (beta != 0.0) r += beta * (double)C[(int64_t)i + (int64_t)j * ldc];
-                C[(int64_t)i + (int64_t)j * ldc] = (float)r;
+                C[(int64_t)i + (int64_t)j * ldc] = (TO)r;
             }
         }
 }
@@ -708,30 +710,34 @@ __global__ __launch_bounds__(256) void k_copy_cols_bf16(int rows, int k, const _
 // One operand term of a BF16 panel product: op(X) with X rows x cols (ld ldx); as a k-contiguous
 // panel it is `extent` x K (extent = m for A, n for B).
 struct BgTerm {
-    const float *X;
+    const void *X;  // FP32 or FP64 (the product's storage type)
     int64_t ldx;
     bool trans;  // true: X is stored K x extent (already k-contiguous)
 };
 
+template <typename TS>
 static int bg_convert(omx_ctx ctx, const BgTerm &tm, int extent, int K, __bf16 *hi, __bf16 *lo, int64_t ldd) {
+    const TS *X = (const TS *)tm.X;
     if (tm.trans) {
         // X is K x extent with k contiguous: same layout, dst[kk + e*ldd]
         dim3 grid((K + BG_T - 1) / BG_T, (extent + BG_T - 1) / BG_T);
-        hipLaunchKernelGGL(k_to_bf16<false>, grid, dim3(256), 0, ctx->stream, K, extent, tm.X, tm.ldx, hi, lo, ldd);
+        hipLaunchKernelGGL((k_to_bf16<false, TS>), grid, dim3(256), 0, ctx->stream, K, extent, X, tm.ldx, hi, lo, ldd);
     } else {
         // X is extent x K: transpose so that dst[kk + e*ldd] = X(e, kk)
         dim3 grid((extent + BG_T - 1) / BG_T, (K + BG_T - 1) / BG_T);
-        hipLaunchKernelGGL(k_to_bf16<true>, grid, dim3(256), 0, ctx->stream, extent, K, tm.X, tm.ldx, hi, lo, ldd);
+        hipLaunchKernelGGL((k_to_bf16<true, TS>), grid, dim3(256), 0, ctx->stream, extent, K, X, tm.ldx, hi, lo, ldd);
     }
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
 
-// C := alpha * sum_t op(A_t) op(B_t) + beta0*C0 + beta*C for nterm FP32 products of inner size k,
-// in BF16 (split = false) or BF16x3 (split = true) through k-contiguous panels in the workspace.
-// Returns OMX_ERR_UNSUPPORTED (nothing launched) when the shape does not suit the panels.
+// C := alpha * sum_t op(A_t) op(B_t) + beta0*C0 + beta*C for nterm products of inner size k on
+// TS storage (FP32 or FP64), in BF16 (split = false) or BF16x3 (split = true) through
+// k-contiguous panels in the workspace.  Returns OMX_ERR_UNSUPPORTED (nothing launched) when the
+// shape does not suit the panels.
+template <typename TS>
 static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTerm *A, const BgTerm *B, bool split,
-                        double alpha, double beta0, const float *C0, double beta, float *C, int64_t ldc) {
+                        double alpha, double beta0, const TS *C0, double beta, TS *C, int64_t ldc) {
     constexpr int BK = Prec<OMX_PREC_BF16>::BK;
     if (ctx->launch_mask || k % BK != 0 || m < 256 || n < 256 || k < 4 * BK) return OMX_ERR_UNSUPPORTED;
     const int reps = split ? 3 : 1;
@@ -745,22 +751,22 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     hipEvent_t ev;
     prof_begin(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
     prof_work(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)k * nterm,
-              4.0 * nterm * ((double)m * k + (double)k * n) + 4.0 * (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0)));
+              sizeof(TS) * (nterm * ((double)m * k + (double)k * n) + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0))));
     // Panel layout along K: for term t, reps blocks of k: split -> [hi, hi, lo] for A and
     // [hi, lo, hi] for B.  Rows of At are m panels of length K (ld K), rows of Bt n of length K.
     for (int tt = 0; tt < nterm && !rc; ++tt) {
         const int64_t kb = (int64_t)tt * reps * k;
         if (!split) {
-            rc = bg_convert(ctx, A[tt], m, k, At + kb, nullptr, K);
-            if (!rc) rc = bg_convert(ctx, B[tt], n, k, Bt + kb, nullptr, K);
+            rc = bg_convert<TS>(ctx, A[tt], m, k, At + kb, nullptr, K);
+            if (!rc) rc = bg_convert<TS>(ctx, B[tt], n, k, Bt + kb, nullptr, K);
         } else {
             // A: hi at kb and lo at kb + 2k, then hi copied to kb + k; B: hi at kb, lo at kb + k,
             // hi copied to kb + 2k
-            rc = bg_convert(ctx, A[tt], m, k, At + kb, At + kb + 2 * k, K);
+            rc = bg_convert<TS>(ctx, A[tt], m, k, At + kb, At + kb + 2 * k, K);
             if (!rc) {
                 hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)m, 65535)), dim3(256), 0,
                                    ctx->stream, m, k, At + kb, K, At + kb + k, K);
-                rc = bg_convert(ctx, B[tt], n, k, Bt + kb, Bt + kb + k, K);
+                rc = bg_convert<TS>(ctx, B[tt], n, k, Bt + kb, Bt + kb + k, K);
             }
             if (!rc)
                 hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)n, 65535)), dim3(256), 0,
@@ -770,8 +776,8 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     if (!rc) {
         typedef Cfg4 CF;
         const int tiles = ((m + CF::BM - 1) / CF::BM) * ((n + CF::BN - 1) / CF::BN);
-        hipLaunchKernelGGL(k_bgemm_tn<CF>, dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, (int)K, alpha, At, K, Bt,
-                           K, beta0, C0, beta, C, ldc);
+        hipLaunchKernelGGL((k_bgemm_tn<CF, TS>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, (int)K, alpha, At, K,
+                           Bt, K, beta0, C0, beta, C, ldc);
         rc = hipGetLastError() == hipSuccess ? OMX_OK : OMX_ERR_HIP;
         if (rc) set_error("bf16 panel GEMM launch failed");
     }
@@ -796,6 +802,13 @@ static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, do
 int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
                 int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
     if (dt == OMX_F64) {
+        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3) {
+            const BgTerm a{A, lda, trA}, b{B, ldb, !trB};
+            int rc = bgemm_panels<double>(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
+                                          (double *)C, ldc);
+            if (rc != OMX_ERR_UNSUPPORTED) return rc;
+            if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;  // the panels do not cover this shape
+        }
         if (prec == OMX_PREC_F64)
             return dispatch_ops<double, OMX_PREC_F64>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
         if (prec == OMX_PREC_F32)
@@ -805,9 +818,9 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
     } else if (dt == OMX_F32) {
         if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3) {
             // k-contiguous BF16 panels (A 't' and B 'n' are already k-contiguous)
-            const BgTerm a{(const float *)A, lda, trA}, b{(const float *)B, ldb, !trB};
-            int rc = bgemm_panels(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
-                                  (float *)C, ldc);
+            const BgTerm a{A, lda, trA}, b{B, ldb, !trB};
+            int rc = bgemm_panels<float>(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
+                                         (float *)C, ldc);
             if (rc != OMX_ERR_UNSUPPORTED) return rc;
             if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;  // the panels do not cover this shape
         }
@@ -824,13 +837,16 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc) {
-    if (dt == OMX_F32 && (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3)) {
-        const BgTerm a[2] = {{(const float *)A1, lda, false}, {(const float *)A2, lda, false}};
-        const BgTerm b[2] = {{(const float *)B1, ldb, true}, {(const float *)B2, ldb, true}};
-        int rc = bgemm_panels(ctx, m, n, k, 2, a, b, prec == OMX_PREC_BF16X3, alpha, beta0, (const float *)C0, beta,
-                              (float *)C, ldc);
+    if ((dt == OMX_F32 || dt == OMX_F64) && (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3)) {
+        const BgTerm a[2] = {{A1, lda, false}, {A2, lda, false}};
+        const BgTerm b[2] = {{B1, ldb, true}, {B2, ldb, true}};
+        const bool split = prec == OMX_PREC_BF16X3;
+        int rc = dt == OMX_F32 ? bgemm_panels<float>(ctx, m, n, k, 2, a, b, split, alpha, beta0, (const float *)C0,
+                                                     beta, (float *)C, ldc)
+                               : bgemm_panels<double>(ctx, m, n, k, 2, a, b, split, alpha, beta0, (const double *)C0,
+                                                      beta, (double *)C, ldc);
         if (rc != OMX_ERR_UNSUPPORTED) return rc;
-        if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;
+        if (split) prec = OMX_PREC_F32;
     }
     const int bk = (prec == OMX_PREC_F64)   ? Prec<OMX_PREC_F64>::BK
                    : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK

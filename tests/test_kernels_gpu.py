@@ -89,30 +89,37 @@ def _split3(x):
 
 @pytest.mark.parametrize("opA,opB", OPS)
 @pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
-def test_gemm_bf16_panels(ctx, opA, opB, prec):
-    # FP32 storage, shapes the k-contiguous BF16 panels cover (m, n >= 256, k % 64 == 0): BF16
-    # against the product of the rounded operands; BF16x3 against hi*hi + hi*lo + lo*hi and, much
-    # tighter than BF16, against the exact product
+@pytest.mark.parametrize("dt", [oh.OMX_F32, oh.OMX_F64])
+def test_gemm_bf16_panels(ctx, opA, opB, prec, dt):
+    # shapes the k-contiguous BF16 panels cover (m, n >= 256, k % 64 == 0), FP32 or FP64 storage:
+    # BF16 against the product of the operands rounded once from the stored value; BF16x3 against
+    # hi*hi + hi*lo + lo*hi and, much tighter than BF16, against the exact product
     m, n, k = 300, 260, 512
-    _note("gemm_bf16_panels", prec, opA, opB)
-    A = rng.standard_normal((k, m) if opA == "t" else (m, k)).astype(np.float32)
-    B = rng.standard_normal((n, k) if opB == "t" else (k, n)).astype(np.float32)
-    C0 = rng.standard_normal((m, n)).astype(np.float32)
-    dA, dB = oh.DMat.from_host(ctx, A, oh.OMX_F32), oh.DMat.from_host(ctx, B, oh.OMX_F32)
-    dC = oh.DMat.from_host(ctx, C0, oh.OMX_F32)
+    _note("gemm_bf16_panels", dt, prec, opA, opB)
+    npdt = np.float32 if dt == oh.OMX_F32 else np.float64
+    A = rng.standard_normal((k, m) if opA == "t" else (m, k)).astype(npdt)
+    B = rng.standard_normal((n, k) if opB == "t" else (k, n)).astype(npdt)
+    C0 = rng.standard_normal((m, n)).astype(npdt)
+    dA, dB = oh.DMat.from_host(ctx, A, dt), oh.DMat.from_host(ctx, B, dt)
+    dC = oh.DMat.from_host(ctx, C0, dt)
     oh.gemm(ctx, opA, opB, dA, dB, dC, alpha=0.5, beta=-1.0, prec=prec)
     got = dC.download().astype(np.float64)
     op = lambda X, o: X.T if o == "t" else X  # noqa: E731
+    rnd = _bf16_of_f32 if dt == oh.OMX_F32 else _bf16
     exact = 0.5 * op(A.astype(np.float64), opA) @ op(B.astype(np.float64), opB) - C0
     if prec == oh.PREC_BF16:
-        ref = 0.5 * op(_bf16_of_f32(A), opA) @ op(_bf16_of_f32(B), opB) - C0
+        ref = 0.5 * op(rnd(A), opA) @ op(rnd(B), opB) - C0
         assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
     else:
-        Ah, Al = _split3(A)
-        Bh, Bl = _split3(B)
+        if dt == oh.OMX_F32:
+            Ah, Al = _split3(A)
+            Bh, Bl = _split3(B)
+        else:
+            Ah, Bh = _bf16(A), _bf16(B)
+            Al, Bl = _bf16(A - Ah), _bf16(B - Bh)
         ref = 0.5 * (op(Ah, opA) @ op(Bh, opB) + op(Ah, opA) @ op(Bl, opB) + op(Al, opA) @ op(Bh, opB)) - C0
         assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
-        bf = 0.5 * op(_bf16_of_f32(A), opA) @ op(_bf16_of_f32(B), opB) - C0
+        bf = 0.5 * op(rnd(A), opA) @ op(rnd(B), opB) - C0
         assert np.max(np.abs(got - exact)) < np.max(np.abs(bf - exact)) / 50
 
 

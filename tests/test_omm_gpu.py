@@ -152,15 +152,18 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
 
 def test_mixed_precision_bf16x3_gradient():
     """'hmx' with the BF16x3 gradient (hi*hi + hi*lo + lo*hi of the FP32 operands on BF16 MFMA) at
-    the 8x8x8 lattice, where the Gram and gradient shapes run on the k-contiguous BF16 panels:
-    the same bars as the FP32-gradient form."""
+    the 8x8x8 lattice, where the Gram and gradient shapes run on the k-contiguous BF16 panels.  The
+    gapped parameterised lattice (onsite levels -6, -6, 6, 6) is used, so that both solves can
+    reach the exact minimum: the same bars as the FP32-gradient form."""
     _note("mixed bf16x3")
-    L, eta = 8, 0.04129763547490687  # mid-gap shift of this lattice (tests/golden/lattice_eigs.json)
-    nb, Hb, Sb = lattice(L)
-    no = nb // 2
-    H, S = oh.bsr_to_dense(nb, *Hb), oh.bsr_to_dense(nb, *Sb)
+    L = 8
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    cp, ri, h = oh.lattice_bsr_theta(L, oh.lattice_theta0())
+    Sb = oh.lattice_bsr(L, 1)
+    H, S = oh.bsr_to_dense(nb, cp, ri, h), oh.bsr_to_dense(nb, *Sb)
     w = np.linalg.eigvalsh(np.linalg.solve(np.linalg.cholesky(S), np.linalg.solve(np.linalg.cholesky(S), H).T))
-    exact = float(np.sum(w[:no][w[:no] < eta]))
+    eta = 0.5 * (w[no - 1] + w[no])
+    exact = float(np.sum(w[:no]))
     Hs = H - eta * S
     C0 = oh.random_c0(no, nb)
     Hm, Sm = F.Mat.from_csc(nb, *csc_of(Hs)), F.Mat.from_csc(nb, *csc_of(S))
