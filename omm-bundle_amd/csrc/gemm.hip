@@ -85,7 +85,9 @@ typedef Cfg<128, 128, 2, 2, 1> Cfg1;  // 4 waves, 64x64 per wave, one workgroup 
 typedef Cfg<256, 128, 2, 2, 1> Cfg2;  // 4 waves, 128x64 per wave
 typedef Cfg<128, 256, 2, 2, 1> Cfg3;  // 4 waves, 64x128 per wave
 typedef Cfg<256, 256, 2, 4, 1> Cfg4;  // 8 waves, 128x64 per wave (BF16 / FP32 products)
-constexpr int kNumCfg = 5;
+typedef Cfg<128, 128, 4, 2, 2> Cfg5;  // 8 waves, 32x64 per wave, two workgroups per CU (FP64)
+typedef Cfg<128, 128, 2, 4, 2> Cfg6;  // 8 waves, 64x32 per wave, two workgroups per CU (FP64)
+constexpr int kNumCfg = 7;
 
 // LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
 // lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
@@ -437,14 +439,14 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     return OMX_OK;
 }
 
-// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3,
+// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3, 5, 6,
 // the FP32 and BF16 ones on FP32 storage 0 and 4; every other product uses 0.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
 struct CfgSet {
     static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
     static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
     static std::vector<int> ids() {
-        if (f64) return {0, 1, 2, 3};
+        if (f64) return {0, 1, 2, 3, 5, 6};
         if (lowp) return {0, 4};
         return {0};
     }
@@ -465,6 +467,12 @@ static int run_cfg_id(int id, omx_ctx ctx, int m, int n, int k, double alpha, co
                                                                 C0, beta, C, ldc);
             case 3:
                 return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg3>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                                C0, beta, C, ldc);
+            case 5:
+                return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg5>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
+                                                                C0, beta, C, ldc);
+            case 6:
+                return run_cfg<TS, PREC, TRA, TRB, NPAIR, Cfg6>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0,
                                                                 C0, beta, C, ldc);
             default:
                 break;

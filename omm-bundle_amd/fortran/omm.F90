@@ -369,13 +369,13 @@ contains
     logical, intent(in) :: lowp
     integer :: outer, inner, steps, n_below, n_stall
     logical :: ls_conv, capped, rewin_now
-    real(dp) :: e_pred, e_pred_min
+    real(dp) :: e_pred, e_best
     conv = .false.
     capped = .false.
     steps = 0
     n_below = 0
     n_stall = 0
-    e_pred_min = huge(1.0_dp)
+    e_best = huge(1.0_dp)
     stat_t0 = wall_seconds()  ! the coefficient readback that precedes this point left the stream idle
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
@@ -454,10 +454,10 @@ contains
               n_below = 0
             end if
             if (n_below >= opt_lowp_consec) conv = .true.
-            ! precision floor of the phase: no new smallest predicted decrease for opt_lowp_stall
-            ! steps; the FP64 refinement takes over from there
-            if (e_pred < e_pred_min) then
-              e_pred_min = e_pred
+            ! precision floor of the phase: no new lowest energy for opt_lowp_stall steps (while
+            ! CG progresses nearly every step lowers it); the FP64 refinement takes over there
+            if (e_min < e_best) then
+              e_best = e_min
               n_stall = 0
             else
               n_stall = n_stall + 1
