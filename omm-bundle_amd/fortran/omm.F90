@@ -12,12 +12,14 @@
 ! m_operation: 'hip' FP64, 'h32' FP32 products, 'hbf' BF16 products (FP32 accumulate), and 'hmx': !
 ! mixed precision -- a low-precision CG phase on an FP32 working set, then FP64 CG from the        !
 ! resulting coefficients (all quantities recomputed exactly) to cg_tol.  In the low-precision      !
-! phase the line-search Grams (HWd, SWd, HWdd, SWdd) are BF16 products and the gradient is an FP32 !
-! product; HW and SW are updated incrementally from the BF16 Grams and recomputed from C in FP32   !
-! every 'lowp_rewin' steps, which bounds their drift by a multiple of the step length.  The phase  !
-! ends when the energy decrease predicted by a line search (its quartic, independent of the drift  !
-! of HW/SW) stays below 'lowp_tol' relative for 'lowp_consec' consecutive steps.  Traces, the line !
-! search and the energy are FP64 in every mode.                                                    !
+! phase the line-search Grams (HWd, SWd, HWdd, SWdd) are BF16 products and the gradient a BF16x3  !
+! product (hi*hi + hi*lo + lo*hi of the FP32 operands; FP32 with lowp_grad_bf16x3 = 0); HW and SW  !
+! are updated incrementally from the BF16 Grams and recomputed from C in FP32 every 'lowp_rewin'   !
+! steps, which bounds their drift by a multiple of the step length.  The phase ends when the       !
+! energy decrease predicted by a line search (its quartic, independent of the drift of HW/SW)      !
+! stays below 'lowp_tol' relative for 'lowp_consec' consecutive steps, or when 'lowp_stall' steps  !
+! bring no new lowest energy (the phase's precision floor).  Traces, the line search and the      !
+! energy are FP64 in every mode.                                                                   !
 !                                                                                                  !
 ! Flavours 0 (basic), 1 and 2 (Cholesky reduction to standard form, S given or already factored)  !
 ! and 3 (preconditioner (S + T/scale_T)^-1) follow the reference's branches; their factorisations !

@@ -27,7 +27,7 @@ module omm_hip_ops
   logical, save :: opt_fused = .true.        ! use fused device kernels when all operands are on device
   integer, save :: opt_progress = 0          ! > 0: one line on stderr every opt_progress CG steps
   logical, save :: opt_grad_fp32 = .true.    ! BF16 phase: gradient and state products in FP32
-  logical, save :: opt_grad_b3 = .false.     ! BF16 phase: gradient in BF16x3 (state products in FP32)
+  logical, save :: opt_grad_b3 = .true.      ! BF16 phase: gradient in BF16x3 (state products in FP32)
   integer, save :: opt_lowp_stall = 60       ! low-precision phase ends after this many steps without a
                                              ! new lowest energy (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it

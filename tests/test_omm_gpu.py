@@ -137,10 +137,12 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
     ref = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True)
     Cm.upload(C0)
     F.set_option("lowp_bf16", 1 if lowp == "hbf" else 0)
+    F.set_option("lowp_grad_bf16x3", 0)  # the FP32-gradient form
     try:
         r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True, operation="hmx")
     finally:
         F.set_option("lowp_bf16", 1)
+        F.set_option("lowp_grad_bf16x3", 1)
     print("CASE mixed", lowp, "icg", r["icg"], "lowp", r["icg_lowp"], "fp64 icg", ref["icg"], "err",
           abs(r["e_min"] - exact) / abs(exact), "fp64 err", abs(ref["e_min"] - exact) / abs(exact), flush=True)
     assert r["converged"] and r["icg_lowp"] > 0
@@ -175,7 +177,7 @@ def test_mixed_precision_bf16x3_gradient():
     try:
         r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True, operation="hmx")
     finally:
-        F.set_option("lowp_grad_bf16x3", 0)
+        F.set_option("lowp_grad_bf16x3", 1)
         F.set_option("lowp_max_iter", 0)
     print("CASE mixed bf16x3 icg", r["icg"], "lowp", r["icg_lowp"], "fp64 icg", ref["icg"], "err",
           abs(r["e_min"] - exact) / abs(exact), "fp64 err", abs(ref["e_min"] - exact) / abs(exact), flush=True)
