@@ -185,11 +185,12 @@ int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode);
  * 0..nbc-1, host array copied to the device; NULL clears it).  A schedule hint only: every
  * output column is computed the same way in any order (no counterpart in the reference). */
 int omx_bsr_set_walk(omx_bsr A, const int *order);
-/* Force the GEMM tile configuration of later products on this context (-1 restores the choice
- * by shape) for the Gram and gradient products: 0 = 128x128 tiles, two workgroups per CU;
- * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256 (FP64 products); 4 = 256x256 with 8 waves
- * (FP32 and BF16 products on FP32 storage).  A configuration that does not apply to a product
- * falls back to 0.  Used by the tile autotuner; results do not depend on it. */
+/* Force the GEMM tile configuration of later products on this context (-1 restores the default)
+ * for the Gram and gradient products: 0 = 128x128 tiles, 4 waves, two workgroups per CU;
+ * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256; 5 = 128x128 with 8 waves of 32x64 (the FP64
+ * default); 6 = 128x128 with 8 waves of 64x32 (FP64 products); 4 = 256x256 with 8 waves (FP32 and
+ * BF16 products on FP32 storage).  A configuration that does not apply to a product falls back
+ * to 0.  Used by the tile autotuner; results do not depend on it. */
 int omx_gemm_set_tile(omx_ctx ctx, int cfg);
 /* Autotuner (project-written; no counterpart in the reference): with enable != 0, the first
  * product or SpMM of each shape on this context times every applicable tile configuration / SpMM
