@@ -185,6 +185,11 @@ int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode);
  * 0..nbc-1, host array copied to the device; NULL clears it).  A schedule hint only: every
  * output column is computed the same way in any order (no counterpart in the reference). */
 int omx_bsr_set_walk(omx_bsr A, const int *order);
+/* The same with LDS-staged gathers: later real SpMMs take the block columns `group` at a time (in
+ * walk order) and load the union of the basis-column blocks a group gathers into LDS once, for
+ * 64-state chunks.  Fails if a group gathers more than 40 distinct blocks (FP64; 80 for FP32 is
+ * not used).  group = 0 is omx_bsr_set_walk.  Results are bitwise those of the default kernel. */
+int omx_bsr_set_walk_groups(omx_bsr A, const int *order, int group);
 /* Force the GEMM tile configuration of later products on this context (-1 restores the default)
  * for the Gram and gradient products: 0 = 128x128 tiles, 4 waves, two workgroups per CU;
  * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256; 5 = 128x128 with 8 waves of 32x64 (the FP64
@@ -372,6 +377,9 @@ int omx_lattice_theta0(uint64_t seed, double *theta);
 /* Tiled walk order of the lattice sites (x-y tiles of tx x ty, each swept through z), a
  * permutation of 0..Lx*Ly*Lz-1 for omx_bsr_set_walk on the lattice operators. */
 int omx_lattice_walk(int Lx, int Ly, int Lz, int tx, int ty, int *order);
+/* Patch walk order: tx x ty x tz patches of sites listed one after another (groups of tx*ty*tz
+ * consecutive entries are compact patches), for omx_bsr_set_walk_groups. */
+int omx_lattice_walk3(int Lx, int Ly, int Lz, int tx, int ty, int tz, int *order);
 /* C0(i,k) = U(-0.5,0.5)/sqrt(nb), no x nb column-major. */
 int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc);
 /* The underlying counter-based uniform variate in [0,1). */

@@ -469,6 +469,9 @@ int omx_bsr_destroy(omx_bsr a) {
     if (a->d_rowidx) (void)hipFree(a->d_rowidx);
     if (a->d_val) (void)hipFree(a->d_val);
     if (a->d_walk) (void)hipFree(a->d_walk);
+    if (a->d_gptr) (void)hipFree(a->d_gptr);
+    if (a->d_uidx) (void)hipFree(a->d_uidx);
+    if (a->d_slot) (void)hipFree(a->d_slot);
     delete a;
     return OMX_OK;
 }

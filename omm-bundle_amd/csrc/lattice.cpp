@@ -273,6 +273,21 @@ int omx_lattice_theta0(uint64_t seed, double *theta) {
     return 0;
 }
 
+// Patch walk over the sites: tx x ty x tz patches listed one after another (inside a patch z, y,
+// x with x fastest), patches swept through z inside each x-y column of patches.  Groups of
+// tx*ty*tz consecutive entries are then compact 3D patches, whose neighbour blocks overlap most.
+int omx_lattice_walk3(int Lx, int Ly, int Lz, int tx, int ty, int tz, int *order) {
+    if (Lx < 1 || Ly < 1 || Lz < 1 || tx < 1 || ty < 1 || tz < 1 || !order) return OMX_ERR_ARG;
+    int64_t k = 0;
+    for (int y0 = 0; y0 < Ly; y0 += ty)
+        for (int x0 = 0; x0 < Lx; x0 += tx)
+            for (int z0 = 0; z0 < Lz; z0 += tz)
+                for (int z = z0; z < std::min(Lz, z0 + tz); ++z)
+                    for (int y = y0; y < std::min(Ly, y0 + ty); ++y)
+                        for (int x = x0; x < std::min(Lx, x0 + tx); ++x) order[k++] = x + Lx * (y + Ly * z);
+    return 0;
+}
+
 int omx_random_c0(int no, int nb, uint64_t seed, double *C, int64_t ldc) {
     if (no < 0 || nb < 1 || !C || ldc < no) return OMX_ERR_ARG;
     const double scale = 1.0 / std::sqrt((double)nb);

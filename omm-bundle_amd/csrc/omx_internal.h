@@ -96,6 +96,11 @@ struct omx_bsr_s {
     int *d_rowidx = nullptr;  // nnzb
     double *d_val = nullptr;  // 16*nnzb (x2 if complex)
     int *d_walk = nullptr;    // optional order in which SpMMs visit the block columns (nbc)
+    // LDS-staged gathers: block columns taken `group` at a time in walk order; group g gathers the
+    // distinct block rows uidx[gptr[g] .. gptr[g+1]) and block l reads slot[l] of them
+    int group = 0, ngroups = 0;
+    int *d_gptr = nullptr, *d_uidx = nullptr;
+    unsigned char *d_slot = nullptr;
 };
 
 namespace omx {

@@ -272,9 +272,113 @@ static int tuned_spmm_shape(omx_ctx ctx, int no, int nb, const T *X, int64_t ldx
     return OMX_OK;
 }
 
+// LDS-staged gathers (the operator carries a group plan, omx_bsr_set_walk_groups).  A workgroup
+// owns a chunk of PR = 64 states and one group of block columns (consecutive in walk order).  It
+// first loads, for its 64 states, the distinct basis-column blocks the whole group gathers into
+// LDS (one pass over each, 16-byte loads along the states), then each wave computes block columns
+// of the group from LDS with one state per lane.  Each block column's blocks are taken in ascending
+// (block, r) order with the same FMAs as k_spmm_bsr4, so the results are bitwise equal; what
+// changes is that a basis-column block shared by several block columns of the group crosses from
+// L2 once instead of once per block column.
+constexpr int PATCH_ROWS = 64, PATCH_UMAX = 40;
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_spmm_patch(int no, int nb, int nbc, double alpha, const T *__restrict__ X,
+                                                       int64_t ldx, const int *__restrict__ colptr,
+                                                       const double *__restrict__ val, const int *__restrict__ walk,
+                                                       int group, int ngroups, const int *__restrict__ gptr,
+                                                       const int *__restrict__ uidx,
+                                                       const unsigned char *__restrict__ slot, double beta,
+                                                       T *__restrict__ Y, int64_t ldy, int nrc,
+                                                       const int *__restrict__ mask) {
+    constexpr int PR = PATCH_ROWS;
+    constexpr int VEC = 16 / sizeof(T);
+    __shared__ T xs[PATCH_UMAX * 4 * PR];
+    if (mask && !*mask) return;
+    const int w = blockIdx.x;
+    int rc, g;
+    if ((nrc & 7) == 0) {  // each XCD walks one state chunk at a time through all groups
+        const int xcd = w & 7, local = w >> 3;
+        rc = (local / ngroups) * 8 + xcd;
+        g = local % ngroups;
+    } else {
+        rc = w % nrc;
+        g = w / nrc;
+    }
+    const int r0 = rc * PR;
+    const int u0 = gptr[g], nu = gptr[g + 1] - u0;  // nu <= PATCH_UMAX (checked on the host)
+    // phase 1: xs[(u*4 + c)*PR + r] = X(r0 + r, 4*uidx[u0+u] + c)
+    const bool vec = (r0 + PR <= no) && (ldx % VEC == 0) && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+    for (int q = threadIdx.x; q < nu * 4 * (PR / VEC); q += 256) {
+        const int rv = (q % (PR / VEC)) * VEC, uc = q / (PR / VEC);
+        const int col = 4 * uidx[u0 + (uc >> 2)] + (uc & 3);
+        T *dst = xs + uc * PR + rv;
+        const T *src = X + (int64_t)col * ldx + r0 + rv;
+        if (col < nb && vec) {
+            typedef typename VecT<T, VEC>::type V;
+            *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(src);
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) dst[e] = (col < nb && r0 + rv + e < no) ? src[e] : (T)0;
+        }
+    }
+    __syncthreads();
+    // phase 2: wave wv computes block columns wv, wv+4, ... of the group; lane = state
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = r0 + lane;
+    const int jb = g * group, je = min(nbc, jb + group);
+    for (int jj = jb + wv; jj < je; jj += 4) {
+        const int J = walk ? walk[jj] : jj;
+        T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+        const int lb = colptr[J], le = colptr[J + 1];
+        for (int l = lb; l < le; ++l) {
+            const T *xb = xs + (int)slot[l] * 4 * PR + lane;
+            const double *v = val + 16 * (int64_t)l;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const T x = xb[r * PR];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] = fma(x, (T)v[r + 4 * c], acc[c]);
+            }
+        }
+        if (i >= no) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int jcol = 4 * J + c;
+            if (jcol >= nb) break;
+            T *dst = Y + (int64_t)jcol * ldy + i;
+            double o = alpha * (double)acc[c];
+            if (beta != 0.0) o += beta * (double)*dst;
+            *dst = (T)o;
+        }
+    }
+}
+
+template <typename T>
+static void launch_spmm_patch(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64_t ldx, omx_bsr A,
+                              double beta, T *Y, int64_t ldy) {
+    const int nrc = (no + PATCH_ROWS - 1) / PATCH_ROWS;
+    const unsigned grid = (unsigned)((int64_t)nrc * A->ngroups);
+    hipLaunchKernelGGL(k_spmm_patch<T>, dim3(grid), dim3(256), 0, ctx->stream, no, nb, A->nbc, alpha, X, ldx,
+                       A->d_colptr, A->d_val, A->d_walk, A->group, A->ngroups, A->d_gptr, A->d_uidx, A->d_slot, beta,
+                       Y, ldy, nrc, ctx->launch_mask);
+}
+
 template <typename T, int VEC>
 static int run_spmm(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64_t ldx, omx_bsr A, double beta, T *Y,
                     int64_t ldy) {
+    if (A->group > 0 && !ctx->spmm_forced) {
+        OMX_REQUIRE((int64_t)((no + PATCH_ROWS - 1) / PATCH_ROWS) * A->ngroups < (1LL << 31), "omx_spmm: grid too large");
+        hipEvent_t ev;
+        prof_begin(ctx, OMX_K_SPMM, &ev);
+        prof_work(ctx, OMX_K_SPMM, 2.0 * 16.0 * A->nnzb * no,
+                  sizeof(T) * 2.0 * (double)no * nb * (beta != 0.0 ? 1.5 : 1.0) + A->nnzb * (16.0 * 8 + 4) +
+                      4.0 * (A->nbc + 1));
+        launch_spmm_patch<T>(ctx, no, nb, alpha, X, ldx, A, beta, Y, ldy);
+        prof_end(ctx, OMX_K_SPMM, ev);
+        OMX_CHECK_HIP(hipGetLastError());
+        return OMX_OK;
+    }
     SpmmShape sh;
     if (ctx->spmm_forced) {
         sh = ctx->spmm_shape;  // forced (omx_spmm_set_tile)
@@ -465,9 +569,20 @@ extern "C" int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int 
     return OMX_OK;
 }
 
+static void clear_patch_plan(omx_bsr A) {
+    if (A->d_gptr) (void)hipFree(A->d_gptr);
+    if (A->d_uidx) (void)hipFree(A->d_uidx);
+    if (A->d_slot) (void)hipFree(A->d_slot);
+    A->d_gptr = A->d_uidx = nullptr;
+    A->d_slot = nullptr;
+    A->group = A->ngroups = 0;
+}
+
 extern "C" int omx_bsr_set_walk(omx_bsr A, const int *order) {
     OMX_REQUIRE(A, "omx_bsr_set_walk: null operator");
     omx_ctx ctx = A->ctx;
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    clear_patch_plan(A);
     if (!order) {
         if (A->d_walk) OMX_CHECK_HIP(hipFree(A->d_walk));
         A->d_walk = nullptr;
@@ -645,4 +760,59 @@ extern "C" int omx_sddmm_bsr(omx_ctx ctx, int dt, int no, int nb, double alpha, 
     if (dt == OMX_F32) return run_sddmm<float, 2>(ctx, no, nb, alpha, (const float *)A, lda, (const float *)B, ldb, beta, C);
     set_error("omx_sddmm_bsr: unsupported element type %d", dt);
     return OMX_ERR_UNSUPPORTED;
+}
+
+extern "C" int omx_bsr_set_walk_groups(omx_bsr A, const int *order, int group) {
+    OMX_REQUIRE(A && group >= 0, "omx_bsr_set_walk_groups: bad arguments");
+    int rc = omx_bsr_set_walk(A, order);
+    if (rc || group == 0) return rc;
+    OMX_REQUIRE(!A->is_complex, "omx_bsr_set_walk_groups: real operators only");
+    omx_ctx ctx = A->ctx;
+    const int nbc = A->nbc;
+    std::vector<int> colptr(nbc + 1), rowidx(std::max<int64_t>(A->nnzb, 1));
+    OMX_CHECK_HIP(hipMemcpy(colptr.data(), A->d_colptr, sizeof(int) * (nbc + 1), hipMemcpyDeviceToHost));
+    if (A->nnzb) OMX_CHECK_HIP(hipMemcpy(rowidx.data(), A->d_rowidx, sizeof(int) * A->nnzb, hipMemcpyDeviceToHost));
+    const int ngroups = (nbc + group - 1) / group;
+    std::vector<int> gptr(ngroups + 1, 0), uidx;
+    std::vector<unsigned char> slot(std::max<int64_t>(A->nnzb, 1), 0);
+    std::vector<int> where(nbc, -1);
+    for (int g = 0; g < ngroups; ++g) {
+        const int jb = g * group, je = std::min(nbc, jb + group);
+        std::vector<int> u;
+        for (int jj = jb; jj < je; ++jj) {
+            const int J = order ? order[jj] : jj;
+            for (int l = colptr[J]; l < colptr[J + 1]; ++l) u.push_back(rowidx[l]);
+        }
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if ((int)u.size() > PATCH_UMAX) {
+            (void)omx_bsr_set_walk(A, order);  // keep the walk, without the plan
+            set_error("omx_bsr_set_walk_groups: group %d gathers %d distinct blocks (at most %d)", g, (int)u.size(),
+                      PATCH_UMAX);
+            return OMX_ERR_ARG;
+        }
+        for (size_t q = 0; q < u.size(); ++q) where[u[q]] = (int)q;
+        for (int jj = jb; jj < je; ++jj) {
+            const int J = order ? order[jj] : jj;
+            for (int l = colptr[J]; l < colptr[J + 1]; ++l) slot[l] = (unsigned char)where[rowidx[l]];
+        }
+        uidx.insert(uidx.end(), u.begin(), u.end());
+        gptr[g + 1] = (int)uidx.size();
+    }
+    bool ok = hipMalloc((void **)&A->d_gptr, sizeof(int) * (ngroups + 1)) == hipSuccess &&
+              hipMalloc((void **)&A->d_uidx, sizeof(int) * std::max<size_t>(uidx.size(), 1)) == hipSuccess &&
+              hipMalloc((void **)&A->d_slot, slot.size()) == hipSuccess;
+    if (!ok) {
+        clear_patch_plan(A);
+        set_error("omx_bsr_set_walk_groups: device allocation failed");
+        return OMX_ERR_NOMEM;
+    }
+    OMX_CHECK_HIP(hipMemcpy(A->d_gptr, gptr.data(), sizeof(int) * (ngroups + 1), hipMemcpyHostToDevice));
+    if (!uidx.empty())
+        OMX_CHECK_HIP(hipMemcpy(A->d_uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice));
+    OMX_CHECK_HIP(hipMemcpy(A->d_slot, slot.data(), slot.size(), hipMemcpyHostToDevice));
+    A->group = group;
+    A->ngroups = ngroups;
+    (void)ctx;
+    return OMX_OK;
 }

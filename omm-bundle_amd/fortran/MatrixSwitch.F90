@@ -71,7 +71,7 @@ module MatrixSwitch
   end interface
 
   public :: m_allocate, m_deallocate, m_copy, mm_multiply, m_add, m_trace, mm_trace, m_scale, m_set
-  public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr, m_register_gbsr_z
+  public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr, m_register_gbsr_z, m_set_walk
   public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download, ms_convert
   public :: ms_label_prec, die
 
@@ -367,6 +367,17 @@ contains
     call chk(rc, 'm_register_gbsr')
     m_name%is_initialized = .true.
   end subroutine m_register_gbsr
+
+  !> Schedule hint for a real device operator ('gdcsc'): the order in which products visit its
+  !> block columns (0-based permutation of the nb/4 block columns), taken `group` at a time with
+  !> LDS-staged gathers when group > 0.  Results are bitwise unchanged.  No reference counterpart.
+  subroutine m_set_walk(m_name, order, group)
+    type(matrix), intent(inout) :: m_name
+    integer(c_int), intent(in), target :: order(:)
+    integer, intent(in) :: group
+    if (.not. (m_name%is_device .and. m_name%is_sparse)) call die('m_set_walk: not a device sparse operator')
+    call chk(omx_bsr_set_walk_groups(m_name%bsr, c_loc(order), int(group, c_int)), 'm_set_walk')
+  end subroutine m_set_walk
 
   !> m_register_gbsr for a complex operator: 32 doubles per block, (re, im) interleaved ('gzcsc').
   subroutine m_register_gbsr_z(m_name, nb, bcolptr, browidx, bval)

@@ -150,6 +150,17 @@ contains
     omm_c_mat_from_bsr = h
   end function omm_c_mat_from_bsr
 
+  !> Block-column walk (0-based order of the nb/4 block columns) with LDS-staged groups of `group`
+  !> block columns for a real device operator (m_set_walk).
+  integer(c_int) function omm_c_mat_set_walk(h, nbc, order, group) bind(c, name='omm_c_mat_set_walk')
+    integer(c_int), value :: h, nbc, group
+    integer(c_int), intent(in) :: order(*)
+    omm_c_mat_set_walk = -1
+    if (.not. valid(h)) return
+    call m_set_walk(table(h), order(1:nbc), int(group))
+    omm_c_mat_set_walk = 0
+  end function omm_c_mat_set_walk
+
   !> Complex device operator ('gzcsc') from 0-based BSR-4 block columns, 32 doubles per block.
   integer(c_int) function omm_c_mat_from_bsr_z(nb, bcolptr, browidx, bval) bind(c, name='omm_c_mat_from_bsr_z')
     integer(c_int), value :: nb

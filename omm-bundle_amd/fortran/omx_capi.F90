@@ -159,6 +159,11 @@ module omx_capi
       integer(c_int), value :: i, j
       real(c_double), intent(out) :: out
     end function
+    integer(c_int) function omx_bsr_set_walk_groups(a, order, group) bind(c, name='omx_bsr_set_walk_groups')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, order
+      integer(c_int), value :: group
+    end function
     integer(c_int) function omx_bsr_param_grad(ctx, Dp, nparam, seg, ent, coef, grad) &
         bind(c, name='omx_bsr_param_grad')
       import :: c_int, c_ptr, c_double

@@ -71,6 +71,8 @@ SIGNATURES = {
     "omx_gemm_set_tile": (_i, [_vp, _i]),
     "omx_spmm_set_tile": (_i, [_vp, _i, _i, _i, _i]),
     "omx_bsr_set_walk": (_i, [_vp, _vp]),
+    "omx_bsr_set_walk_groups": (_i, [_vp, _vp, _i]),
+    "omx_lattice_walk3": (_i, [_i, _i, _i, _i, _i, _i, _vp]),
     "omx_lattice_walk": (_i, [_i, _i, _i, _i, _i, _vp]),
     "omx_autotune": (_i, [_vp, _i]),
     "omx_group_create": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
@@ -210,6 +212,14 @@ def lattice_walk(L, tx: int, ty: int) -> np.ndarray:
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     order = np.zeros(Lx * Ly * Lz, np.int32)
     check(lib().omx_lattice_walk(Lx, Ly, Lz, tx, ty, _ptr(order)), "lattice walk")
+    return order
+
+
+def lattice_walk3(L, tx: int, ty: int, tz: int) -> np.ndarray:
+    """Patch visiting order of the lattice sites: tx x ty x tz patches listed one after another."""
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    order = np.zeros(Lx * Ly * Lz, np.int32)
+    check(lib().omx_lattice_walk3(Lx, Ly, Lz, tx, ty, tz, _ptr(order)), "lattice walk3")
     return order
 
 
@@ -423,13 +433,14 @@ class Bsr:
         check(lib().omx_bsr_get_pattern(self.h, _ptr(cp), _ptr(ri)), "bsr_get_pattern")
         return cp, ri[: self.nnzb]
 
-    def set_walk(self, order):
-        """Block-column visiting order of later SpMMs (None clears it); results do not change."""
+    def set_walk(self, order, group: int = 0):
+        """Block-column visiting order of later SpMMs (None clears it), with LDS-staged groups of
+        `group` block columns when group > 0; results do not change."""
         if order is None:
             check(lib().omx_bsr_set_walk(self.h, None), "bsr_set_walk")
             return
         o = np.ascontiguousarray(order, np.int32)
-        check(lib().omx_bsr_set_walk(self.h, _ptr(o)), "bsr_set_walk")
+        check(lib().omx_bsr_set_walk_groups(self.h, _ptr(o), int(group)), "bsr_set_walk")
 
     def free(self):
         if self.h:

@@ -28,6 +28,8 @@ def lib():
         L.omm_c_setup.argtypes = [_i]
         L.omm_c_mat_alloc.argtypes = [_i, _i, C.c_char_p]
         L.omm_c_mat_from_csc.argtypes = [_i, _vp, _vp, _vp, C.c_char_p]
+        L.omm_c_mat_set_walk.argtypes = [_i, _i, _vp, _i]
+        L.omm_c_mat_set_walk.restype = _i
         L.omm_c_mat_from_bsr.argtypes = [_i, _vp, _vp, _vp]
         L.omm_c_mat_upload.argtypes = [_i, _vp]
         L.omm_c_mat_download.argtypes = [_i, _vp]
@@ -93,6 +95,12 @@ class Mat:
         if self.h <= 0:
             raise RuntimeError("omm_c_mat_from_bsr failed")
         return self
+
+    def set_walk(self, order, group=0):
+        """Block-column visiting order of a real device operator, LDS-staged groups of `group`."""
+        o = np.ascontiguousarray(order, np.int32)
+        if lib().omm_c_mat_set_walk(self.h, len(o), _p(o), int(group)) != 0:
+            raise RuntimeError("omm_c_mat_set_walk failed")
 
     @property
     def is_complex(self):

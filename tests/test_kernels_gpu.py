@@ -230,6 +230,47 @@ def test_spmm_shapes_and_walks(ctx, dt, no):
         A.set_walk(None)
 
 
+@pytest.mark.parametrize("dt", [oh.OMX_F64, oh.OMX_F32])
+@pytest.mark.parametrize("no", [1030, 2048])
+def test_spmm_lds_groups_bitwise(ctx, dt, no):
+    # LDS-staged gathers over compact patches of block columns: each output element is the same
+    # ascending (block, r) FMA sequence as the default kernel, so the results are bitwise equal
+    L = 6
+    _note("spmm_lds_groups", dt, no)
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    nb = 4 * L ** 3
+    npdt = np.float64 if dt == oh.OMX_F64 else np.float32
+    X = rng.standard_normal((no, nb)).astype(npdt)
+    Y0 = rng.standard_normal((no, nb)).astype(npdt)
+    A = oh.Bsr(ctx, nb, cp, ri, bv)
+    dX, dY = oh.DMat.from_host(ctx, X, dt), oh.DMat.from_host(ctx, Y0, dt)
+    oh.spmm(ctx, dX, A, dY, alpha=0.75, beta=-1.5)
+    ref = dY.download()
+    try:
+        for (tx, ty, tz) in [(2, 2, 2), (3, 3, 1), (1, 1, 1), (2, 3, 1)]:
+            A.set_walk(oh.lattice_walk3(L, tx, ty, tz), group=tx * ty * tz)
+            dY.upload(Y0)
+            oh.spmm(ctx, dX, A, dY, alpha=0.75, beta=-1.5)
+            assert np.array_equal(dY.download(), ref), (tx, ty, tz)
+        A.set_walk(None)  # plan cleared: the default kernel again
+        dY.upload(Y0)
+        oh.spmm(ctx, dX, A, dY, alpha=0.75, beta=-1.5)
+        assert np.array_equal(dY.download(), ref)
+    finally:
+        A.set_walk(None)
+    for M in (dX, dY, A):
+        M.free()
+
+
+def test_spmm_lds_groups_reject_large_unions(ctx):
+    L = 6
+    cp, ri, bv = oh.lattice_bsr(L, 0)
+    A = oh.Bsr(ctx, 4 * L ** 3, cp, ri, bv)
+    with pytest.raises(oh.OmxError, match="distinct blocks"):
+        A.set_walk(np.arange(L ** 3, dtype=np.int32), group=36)  # 6x6 sites of one plane: 36 + 72 blocks
+    A.free()
+
+
 def test_spmm_walk_rejects_non_permutation(ctx):
     L = 3
     cp, ri, bv = oh.lattice_bsr(L, 0)
