@@ -272,85 +272,125 @@ static int tuned_spmm_shape(omx_ctx ctx, int no, int nb, const T *X, int64_t ldx
     return OMX_OK;
 }
 
-// LDS-staged gathers (the operator carries a group plan, omx_bsr_set_walk_groups).  A workgroup
-// owns a chunk of PR = 64 states and one group of block columns (consecutive in walk order).  It
-// first loads, for its 64 states, the distinct basis-column blocks the whole group gathers into
-// LDS (one pass over each, 16-byte loads along the states), then each wave computes block columns
-// of the group from LDS with one state per lane.  Each block column's blocks are taken in ascending
+// LDS-staged gathers (the operator carries a group plan, omx_bsr_set_walk_groups).  The work
+// items are (chunk of PR = 64 states, group of block columns consecutive in walk order).  For an
+// item, a workgroup loads the distinct basis-column blocks the whole group gathers, for its 64
+// states, into LDS (16-byte loads along the states); then each wave computes block columns of the
+// group from LDS with one state per lane.  Each block column's blocks are taken in ascending
 // (block, r) order with the same FMAs as k_spmm_bsr4, so the results are bitwise equal; what
-// changes is that a basis-column block shared by several block columns of the group crosses from
-// L2 once instead of once per block column.
-constexpr int PATCH_ROWS = 64, PATCH_UMAX = 40;
+// changes is that a block shared by several block columns of the group crosses from L2 once.
+// The grid is persistent: 8*K workgroups, workgroup (x, i) on XCD x (dispatch is round-robin over
+// the 8 XCDs) takes the items of that XCD's state chunks (x, x+8, ...) with stride K, so the K
+// workgroups of an XCD walk neighbouring groups of one chunk together and share its L2.  The next
+// item's gathers are loaded into registers while the current item computes from LDS.
+constexpr int PATCH_ROWS = 64, PATCH_UMAX = 40, PATCH_NT = 256;
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_spmm_patch(int no, int nb, int nbc, double alpha, const T *__restrict__ X,
-                                                       int64_t ldx, const int *__restrict__ colptr,
-                                                       const double *__restrict__ val, const int *__restrict__ walk,
-                                                       int group, int ngroups, const int *__restrict__ gptr,
-                                                       const int *__restrict__ uidx,
-                                                       const unsigned char *__restrict__ slot, double beta,
-                                                       T *__restrict__ Y, int64_t ldy, int nrc,
-                                                       const int *__restrict__ mask) {
+__global__ __launch_bounds__(PATCH_NT, 2) void k_spmm_patch(int no, int nb, int nbc, double alpha,
+                                                            const T *__restrict__ X, int64_t ldx,
+                                                            const int *__restrict__ colptr,
+                                                            const double *__restrict__ val,
+                                                            const int *__restrict__ walk, int group, int ngroups,
+                                                            const int *__restrict__ gptr,
+                                                            const int *__restrict__ uidx,
+                                                            const unsigned char *__restrict__ slot, double beta,
+                                                            T *__restrict__ Y, int64_t ldy, int nrc,
+                                                            const int *__restrict__ mask) {
     constexpr int PR = PATCH_ROWS;
     constexpr int VEC = 16 / sizeof(T);
-    __shared__ T xs[PATCH_UMAX * 4 * PR];
+    constexpr int VPC = PR / VEC;                               // vectors per block column of a chunk
+    constexpr int NV = PATCH_UMAX * 4 * VPC / PATCH_NT;         // vectors per thread at most
+    static_assert(NV * PATCH_NT == PATCH_UMAX * 4 * VPC, "LDS tile not divisible");
+    typedef typename VecT<T, VEC>::type V;
+    __shared__ V xs[PATCH_UMAX * 4 * VPC];
     if (mask && !*mask) return;
-    const int w = blockIdx.x;
-    int rc, g;
-    if ((nrc & 7) == 0) {  // each XCD walks one state chunk at a time through all groups
-        const int xcd = w & 7, local = w >> 3;
-        rc = (local / ngroups) * 8 + xcd;
-        g = local % ngroups;
-    } else {
-        rc = w % nrc;
-        g = w / nrc;
-    }
-    const int r0 = rc * PR;
-    const int u0 = gptr[g], nu = gptr[g + 1] - u0;  // nu <= PATCH_UMAX (checked on the host)
-    // phase 1: xs[(u*4 + c)*PR + r] = X(r0 + r, 4*uidx[u0+u] + c)
-    const bool vec = (r0 + PR <= no) && (ldx % VEC == 0) && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
-    for (int q = threadIdx.x; q < nu * 4 * (PR / VEC); q += 256) {
-        const int rv = (q % (PR / VEC)) * VEC, uc = q / (PR / VEC);
-        const int col = 4 * uidx[u0 + (uc >> 2)] + (uc & 3);
-        T *dst = xs + uc * PR + rv;
-        const T *src = X + (int64_t)col * ldx + r0 + rv;
-        if (col < nb && vec) {
-            typedef typename VecT<T, VEC>::type V;
-            *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(src);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool xcd_map = (nrc & 7) == 0 && (gridDim.x & 7) == 0;
+    const int K = xcd_map ? (int)gridDim.x >> 3 : (int)gridDim.x;
+    const int x = xcd_map ? (int)blockIdx.x & 7 : 0, first = xcd_map ? (int)blockIdx.x >> 3 : (int)blockIdx.x;
+    const int64_t nitems = xcd_map ? (int64_t)(nrc >> 3) * ngroups : (int64_t)nrc * ngroups;
+    auto decode = [&](int64_t k, int &rc, int &g) {
+        if (xcd_map) {
+            rc = (int)(k / ngroups) * 8 + x;
+            g = (int)(k % ngroups);
         } else {
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) dst[e] = (col < nb && r0 + rv + e < no) ? src[e] : (T)0;
+            rc = (int)(k / ngroups);
+            g = (int)(k % ngroups);
         }
-    }
-    __syncthreads();
-    // phase 2: wave wv computes block columns wv, wv+4, ... of the group; lane = state
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = r0 + lane;
-    const int jb = g * group, je = min(nbc, jb + group);
-    for (int jj = jb + wv; jj < je; jj += 4) {
-        const int J = walk ? walk[jj] : jj;
-        T acc[4] = {(T)0, (T)0, (T)0, (T)0};
-        const int lb = colptr[J], le = colptr[J + 1];
-        for (int l = lb; l < le; ++l) {
-            const T *xb = xs + (int)slot[l] * 4 * PR + lane;
-            const double *v = val + 16 * (int64_t)l;
+    };
+    const bool xal = (ldx % VEC == 0) && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
+    V regs[NV];
+    auto fetch = [&](int rc, int g) {
+        const int r0 = rc * PR, u0 = gptr[g], nv = (gptr[g + 1] - u0) * 4 * VPC;
+        const bool full = xal && (r0 + PR <= no);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const T x = xb[r * PR];
+        for (int q = 0; q < NV; ++q) {
+            const int e = tid + PATCH_NT * q;
+            if (e < nv) {
+                const int rv = (e % VPC) * VEC, uc = e / VPC;
+                const int col = 4 * uidx[u0 + (uc >> 2)] + (uc & 3);
+                const T *src = X + (int64_t)col * ldx + r0 + rv;
+                if (col < nb && full) {
+                    regs[q] = *reinterpret_cast<const V *>(src);
+                } else {
+                    T t[VEC];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[c] = fma(x, (T)v[r + 4 * c], acc[c]);
+                    for (int h = 0; h < VEC; ++h) t[h] = (col < nb && r0 + rv + h < no) ? src[h] : (T)0;
+                    regs[q] = *reinterpret_cast<const V *>(t);
+                }
             }
         }
-        if (i >= no) continue;
+    };
+    int64_t k = first;
+    int rc = 0, g = 0;
+    if (k < nitems) {
+        decode(k, rc, g);
+        fetch(rc, g);
+    }
+    while (k < nitems) {
+        const int nv = (gptr[g + 1] - gptr[g]) * 4 * VPC;
+        __syncthreads();  // the previous item's reads of xs are done
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int jcol = 4 * J + c;
-            if (jcol >= nb) break;
-            T *dst = Y + (int64_t)jcol * ldy + i;
-            double o = alpha * (double)acc[c];
-            if (beta != 0.0) o += beta * (double)*dst;
-            *dst = (T)o;
+        for (int q = 0; q < NV; ++q) {
+            const int e = tid + PATCH_NT * q;
+            if (e < nv) xs[e] = regs[q];
         }
+        __syncthreads();
+        const int crc = rc, cg = g;
+        const int64_t kn = k + K;
+        if (kn < nitems) {  // next item's gathers in flight while this one computes
+            decode(kn, rc, g);
+            fetch(rc, g);
+        }
+        const T *xt = reinterpret_cast<const T *>(xs);
+        const int i = crc * PR + lane;
+        const int jb = cg * group, je = min(nbc, jb + group);
+        for (int jj = jb + wv; jj < je; jj += PATCH_NT / 64) {
+            const int J = walk ? walk[jj] : jj;
+            T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+            const int lb = colptr[J], le = colptr[J + 1];
+            for (int l = lb; l < le; ++l) {
+                const T *xb = xt + (int)slot[l] * 4 * PR + lane;
+                const double *v = val + 16 * (int64_t)l;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const T xv = xb[r * PR];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[c] = fma(xv, (T)v[r + 4 * c], acc[c]);
+                }
+            }
+            if (i >= no) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int jcol = 4 * J + c;
+                if (jcol >= nb) break;
+                T *dst = Y + (int64_t)jcol * ldy + i;
+                double o = alpha * (double)acc[c];
+                if (beta != 0.0) o += beta * (double)*dst;
+                *dst = (T)o;
+            }
+        }
+        k = kn;
     }
 }
 
@@ -358,8 +398,11 @@ template <typename T>
 static void launch_spmm_patch(omx_ctx ctx, int no, int nb, double alpha, const T *X, int64_t ldx, omx_bsr A,
                               double beta, T *Y, int64_t ldy) {
     const int nrc = (no + PATCH_ROWS - 1) / PATCH_ROWS;
-    const unsigned grid = (unsigned)((int64_t)nrc * A->ngroups);
-    hipLaunchKernelGGL(k_spmm_patch<T>, dim3(grid), dim3(256), 0, ctx->stream, no, nb, A->nbc, alpha, X, ldx,
+    const int64_t items = (int64_t)nrc * A->ngroups;
+    // two workgroups per CU, all resident; fewer when there are fewer items
+    const int64_t want = 8LL * std::max(1, 2 * ctx->num_cus / 8);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, items));
+    hipLaunchKernelGGL(k_spmm_patch<T>, dim3(grid), dim3(PATCH_NT), 0, ctx->stream, no, nb, A->nbc, alpha, X, ldx,
                        A->d_colptr, A->d_val, A->d_walk, A->group, A->ngroups, A->d_gptr, A->d_uidx, A->d_slot, beta,
                        Y, ldy, nrc, ctx->launch_mask);
 }
