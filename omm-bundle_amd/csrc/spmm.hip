@@ -369,14 +369,27 @@ __global__ __launch_bounds__(PATCH_NT, 2) void k_spmm_patch(int no, int nb, int 
             const int J = walk ? walk[jj] : jj;
             T acc[4] = {(T)0, (T)0, (T)0, (T)0};
             const int lb = colptr[J], le = colptr[J + 1];
-            for (int l = lb; l < le; ++l) {
-                const T *xb = xt + (int)slot[l] * 4 * PR + lane;
-                const double *v = val + 16 * (int64_t)l;
+            // blocks in batches of 8: their slots and LDS reads are issued together, then the FMAs
+            // run in ascending (block, r) order
+            for (int l0 = lb; l0 < le; l0 += 8) {
+                const int nblk = min(8, le - l0);
+                int sl[8];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const T xv = xb[r * PR];
+                for (int b = 0; b < 8; ++b) sl[b] = b < nblk ? (int)slot[l0 + b] : 0;
+                T xr[8][4];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) acc[c] = fma(xv, (T)v[r + 4 * c], acc[c]);
+                for (int b = 0; b < 8; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) xr[b][r] = b < nblk ? xt[(sl[b] * 4 + r) * PR + lane] : (T)0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    if (b < nblk) {
+                        const double *v = val + 16 * (int64_t)(l0 + b);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) acc[c] = fma(xr[b][r], (T)v[r + 4 * c], acc[c]);
+                    }
                 }
             }
             if (i >= no) continue;
