@@ -611,6 +611,43 @@ __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *_
     }
 }
 
+// One BK = 64 slice of the BF16 panel product, software-pipelined: the wave's MFMAs run in groups
+// (one A fragment x the FBN B fragments of a 32-wide k step), and the LDS reads for group g + LEAD
+// are issued before the MFMAs of group g, so that LEAD groups of reads are in flight behind the
+// matrix core (the waits become lgkmcnt(n > 0) instead of a full drain after every read).  The
+// scheduling fences keep the compiler from regrouping the reads.  Per accumulator the MFMA chain
+// is the one mma_slice issues, so results are bitwise the same.
+template <class CF, int LEAD>
+__device__ inline void mma_slice_bf16_pipe(const __bf16 *As, const __bf16 *Bs, f4v (&acc)[CF::FBM][CF::FBN], int wm,
+                                           int wn, int lane) {
+    typedef LdsT<OMX_PREC_BF16, CF::BM> LA;
+    typedef LdsT<OMX_PREC_BF16, CF::BN> LB;
+    constexpr int BK = Prec<OMX_PREC_BF16>::BK, NKS = BK / 32, NG = NKS * CF::FBM;
+    const int li = lane & 15, lk = lane >> 4;
+    bf8v fa[NKS][CF::FBM], fb[NKS][CF::FBN];
+#pragma unroll
+    for (int g = -LEAD; g < NG; ++g) {
+        const int lg = g + LEAD;
+        if (lg >= 0 && lg < NG) {
+            const int ks = lg / CF::FBM, fm = lg % CF::FBM;
+            if (fm == 0) {
+#pragma unroll
+                for (int f = 0; f < CF::FBN; ++f)
+                    fb[ks][f] = *reinterpret_cast<const bf8v *>(&Bs[LB::off(wn * CF::WN + f * 16 + li, ks * 32 + 8 * lk)]);
+            }
+            fa[ks][fm] = *reinterpret_cast<const bf8v *>(&As[LA::off(wm * CF::WM + fm * 16 + li, ks * 32 + 8 * lk)]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (g >= 0) {
+            const int ks = g / CF::FBM, fm = g % CF::FBM;
+#pragma unroll
+            for (int fn = 0; fn < CF::FBN; ++fn)
+                acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][fm], fb[ks][fn], acc[fm][fn], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // C := alpha * sum_kk At[kk + i*lda] * Bt[kk + j*ldb] + beta0*C0 + beta*C over kk < K (K % 64 == 0,
 // lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation; C, C0 of type TO (the
 // storage type, FP32 or FP64).
@@ -682,7 +719,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
             __bf16 *An = (s & 1) ? As0 : As1, *Bn = (s & 1) ? Bs0 : Bs1;
             const bool more = (s + 1 < nslices);
             if (more) fetch((s + 1) * BK);
-            mma_slice<OMX_PREC_BF16, CF>(Ac, Bc, acc, wm, wn, lane);
+            mma_slice_bf16_pipe<CF, 2>(Ac, Bc, acc, wm, wn, lane);
             if (more) stash(An, Bn);
             __syncthreads();
         }
