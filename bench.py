@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
     ap.add_argument("--ttc-fp32-gradient", action="store_true",
                     help="with --ttc, also time the mixed solve with an FP32 gradient (the default is BF16x3)")
+    ap.add_argument("--ttc-state-bf16x3", action="store_true",
+                    help="with --ttc, also time the mixed solve with the periodic state recomputation in BF16x3")
     ap.add_argument("--ttc-log", default=None, help="write the per-step energies of the --ttc solves (JSON) here")
     ap.add_argument("--kpoints", action="store_true",
                     help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
@@ -309,8 +311,9 @@ def main():
         line["mixed_low_precision_iters_per_s"] = round(res["mixed_lowp_ips"], 4)
     if args.ttc:
         line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log,
-                                                    variants=("mixed", "mixed_fp32_gradient") if args.ttc_fp32_gradient
-                                                    else ("mixed",))
+                                                    variants=("mixed",)
+                                                    + (("mixed_fp32_gradient",) if args.ttc_fp32_gradient else ())
+                                                    + (("mixed_state_bf16x3",) if args.ttc_state_bf16x3 else ()))
 
     if not args.no_cpu:
         t = time.time()
@@ -350,7 +353,8 @@ def exact_eigs(nb):
 
 def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None, variants=("mixed",)):
     """FP64 omm() and the mixed-precision variants ("mixed": the default BF16x3 gradient,
-    "mixed_fp32_gradient": FP32 gradient) from the same C0 to the reference's e_diff <= tol;
+    "mixed_fp32_gradient": FP32 gradient, "mixed_state_bf16x3": BF16x3 periodic state recomputation)
+    from the same C0 to the reference's e_diff <= tol;
     energies and density matrices compared with the FP64 result and, when tabulated, with the exact
     eigenvalue sum."""
     H, S, Cm = mats
@@ -375,6 +379,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
     logs = {"fp64": {"e": r["e_log"].tolist(), "t": (r["t_log"] - r["t0"]).tolist()}}
     for name in variants:
         F.set_option("lowp_grad_bf16x3", 0 if name == "mixed_fp32_gradient" else 1)
+        F.set_option("lowp_state_bf16x3", 1 if name == "mixed_state_bf16x3" else 0)
         Dmx = F.Mat(nb, nb)
         Cm.upload(C0)
         F.sync()
@@ -383,6 +388,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
         F.sync()
         tmx = time.perf_counter() - t
         F.set_option("lowp_grad_bf16x3", 1)
+        F.set_option("lowp_state_bf16x3", 0)
         # ||D_mx - D_64||_F on the device
         p2, _ = Dmx.dptr()
         oh.check(oh.lib().omx_add(ctxh, oh.OMX_F64, b"n", nb, nb, -1.0, p1, ld, 1.0, p2, ld), "add")
@@ -391,6 +397,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
         v = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
              "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
              "gradient": "fp32" if name == "mixed_fp32_gradient" else "bf16x3",
+             "state_recompute": "bf16x3" if name == "mixed_state_bf16x3" else "fp32",
              "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),
              "density_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0))),
              "density_rel_fro_err_vs_fp64": float(np.sqrt(max(fro.value, 0.0) / max(fro_ref.value, 1e-300))),
