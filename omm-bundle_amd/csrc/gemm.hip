@@ -87,7 +87,7 @@ typedef Cfg<128, 256, 2, 2, 1> Cfg3;  // 4 waves, 64x128 per wave
 typedef Cfg<256, 256, 2, 4, 1> Cfg4;  // 8 waves, 128x64 per wave (BF16 / FP32 products)
 typedef Cfg<128, 128, 4, 2, 2> Cfg5;  // 8 waves, 32x64 per wave, two workgroups per CU (FP64)
 typedef Cfg<128, 128, 2, 4, 2> Cfg6;  // 8 waves, 64x32 per wave, two workgroups per CU (FP64)
-constexpr int kNumCfg = 7;
+constexpr int kNumCfg = 8;  // 7: Cfg5 tiles with direct-to-LDS staging (FP64 A*B^T products)
 
 // LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
 // lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
@@ -391,6 +391,111 @@ __global__ void k_splitk_reduce(int m, int n, int splits, const AT *__restrict__
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// ---------------------------------------------------------------------------------------------
+// FP64 Gram with direct-to-LDS staging (configuration 7): C := alpha*A*B^T + beta0*C0 + beta*C,
+// A m x k (ld lda) and B n x k (ld ldb), both with the row index contiguous.  Every k-row of a
+// 128-wide operand tile is then 1 KB, which one wave moves with one global_load_lds_dwordx4 (16
+// bytes per lane, lane l landing at byte 16*l of the LDS row, checked on the device) straight into
+// the k-major LDS tile: no staging registers and no LDS write instructions.  Two LDS stages; the
+// loads of slice s+1 are in flight while slice s feeds the MFMAs.  Tiles, waves and the MFMA
+// chain per element are those of the staged kernel with Cfg5, so the results are bitwise the same.
+// Requires m, n % 128 == 0, k % 16 == 0, even leading dimensions and 16-byte aligned operands.
+// ---------------------------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+// s_waitcnt immediate (gfx9 encoding): vmcnt(vm), expcnt and lgkmcnt left at their maxima
+constexpr int waitcnt_vmcnt(int vm) { return (vm & 0xF) | ((vm >> 4) << 14) | (7 << 4) | (0xF << 8); }
+
+template <class CF>
+__global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_nt_dl(int m, int n, int k, double alpha,
+                                                                  const double *__restrict__ A, int64_t lda,
+                                                                  const double *__restrict__ B, int64_t ldb,
+                                                                  double beta0, const double *__restrict__ C0,
+                                                                  double beta, double *__restrict__ C, int64_t ldc) {
+    static_assert(CF::BM == 128 && CF::BN == 128, "one 1 KB row per wave load");
+    constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, ROWS = 2 * BK, LPW = ROWS / NW;
+    constexpr int STAGE = ROWS * 128;  // doubles: BK rows of A, then BK rows of B
+    static_assert(LPW * NW == ROWS, "rows per wave");
+    __shared__ double smem[2 * STAGE];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave / CF::WGN, wn = wave % CF::WGN;
+    int tm, tn;
+    tile_of(blockIdx.x, m / 128, n / 128, tm, tn);
+    const int i0 = tm * 128, j0 = tn * 128;
+    d4v acc[CF::FBM][CF::FBN];
+#pragma unroll
+    for (int a = 0; a < CF::FBM; ++a)
+#pragma unroll
+        for (int b = 0; b < CF::FBN; ++b) acc[a][b] = d4v{0, 0, 0, 0};
+    auto issue = [&](int s) {
+        double *st = smem + (s & 1) * STAGE;
+        const int k0 = s * BK;
+#pragma unroll
+        for (int q = 0; q < LPW; ++q) {
+            const int row = wave * LPW + q;  // < BK: row k of A; else row k - BK of B
+            const double *src = row < BK ? A + (int64_t)(i0 + 2 * lane) + (int64_t)(k0 + row) * lda
+                                         : B + (int64_t)(j0 + 2 * lane) + (int64_t)(k0 + row - BK) * ldb;
+            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + row * 128), 16, 0, 0);
+        }
+    };
+    const int li = lane & 15, lk = lane >> 4;
+    const int ns = k / BK;
+    issue(0);
+    for (int s = 0; s < ns; ++s) {
+        __builtin_amdgcn_s_waitcnt(waitcnt_vmcnt(0));  // this wave's rows of slice s have landed
+        __syncthreads();                               // everyone's have; slice s-1 is no longer read
+        if (s + 1 < ns) issue(s + 1);
+        const double *As = smem + (s & 1) * STAGE, *Bs = As + BK * 128;
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double a[CF::FBM], b[CF::FBN];
+#pragma unroll
+            for (int f = 0; f < CF::FBM; ++f) a[f] = As[(ks + lk) * 128 + wm * CF::WM + f * 16 + li];
+#pragma unroll
+            for (int f = 0; f < CF::FBN; ++f) b[f] = Bs[(ks + lk) * 128 + wn * CF::WN + f * 16 + li];
+#pragma unroll
+            for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < CF::FBN; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+        }
+    }
+    const int lq = lane >> 4;
+#pragma unroll
+    for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < CF::FBN; ++fn) {
+            const int j = j0 + wn * CF::WN + fn * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wm * CF::WM + fm * 16 + Prec<OMX_PREC_F64>::out_row(lq, v);
+                double r = alpha * acc[fm][fn][v];
+                if (C0) r += beta0 * C0[(int64_t)i + (int64_t)j * ldc];
+                if (beta != 0.0) r += beta * C[(int64_t)i + (int64_t)j * ldc];
+                C[(int64_t)i + (int64_t)j * ldc] = r;
+            }
+        }
+}
+
+// Launches configuration 7 when the product qualifies; returns OMX_ERR_UNSUPPORTED otherwise.
+static int run_dgemm_nt_dl(omx_ctx ctx, int m, int n, int k, double alpha, const double *A, int64_t lda,
+                           const double *B, int64_t ldb, double beta0, const double *C0, double beta, double *C,
+                           int64_t ldc) {
+    typedef Cfg5 CF;
+    const int tiles = (m / 128) * (n / 128);
+    if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F64>::BK || lda % 2 || ldb % 2 ||
+        !aligned16(A) || !aligned16(B) || ctx->launch_mask || tiles < 2 * ctx->num_cus)
+        return OMX_ERR_UNSUPPORTED;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_GEMM, &ev);
+    prof_work(ctx, OMX_K_GEMM, 2.0 * m * n * (double)k,
+              sizeof(double) * ((double)m * k + (double)k * n + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
+    hipLaunchKernelGGL((k_dgemm_nt_dl<CF>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A, lda, B, ldb,
+                       beta0, C0, beta, C, ldc);
+    prof_end(ctx, OMX_K_GEMM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR, class CF>
 static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
@@ -440,12 +545,15 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
 }
 
 // Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3, 5, 6,
-// the FP32 and BF16 ones on FP32 storage 0 and 4; every other product uses 0.
+// the FP64 Gram (A*B^T on FP64 storage) also 7; the FP32 and BF16 ones on FP32 storage 0 and 4;
+// every other product uses 0.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
 struct CfgSet {
     static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
+    static constexpr bool dl = f64 && TRB && NPAIR == 1 && std::is_same<TS, double>::value;
     static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
     static std::vector<int> ids() {
+        if (dl) return {7, 0, 1, 2, 3, 5, 6};
         if (f64) return {0, 1, 2, 3, 5, 6};
         if (lowp) return {0, 4};
         return {0};
@@ -457,6 +565,13 @@ static int run_cfg_id(int id, omx_ctx ctx, int m, int n, int k, double alpha, co
                       int64_t lda, const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta,
                       TS *C, int64_t ldc) {
     typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
+    if constexpr (CS::dl) {
+        if (id == 7) {
+            int rc = run_dgemm_nt_dl(ctx, m, n, k, alpha, A1, lda, B1, ldb, beta0, C0, beta, C, ldc);
+            if (rc != OMX_ERR_UNSUPPORTED) return rc;
+            id = 5;  // the staged kernel with the same tiles covers every other shape
+        }
+    }
     if constexpr (CS::f64) {
         switch (id) {
             case 1:
@@ -550,8 +665,10 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                     const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
                     int64_t ldc) {
     // default: 128x128 with 8 waves and two workgroups per CU for the FP64 Gram and gradient products
-    // (3-4 % ahead of the 4-wave tile at n=16384, bitwise the same result), 0 otherwise
-    int id = CfgSet<TS, PREC, TRA, TRB, NPAIR>::f64 ? 5 : 0;
+    // (3-4 % ahead of the 4-wave tile at n=16384, bitwise the same result), with direct-to-LDS
+    // staging for the Gram where it applies (a further 2-4 %, bitwise the same); 0 otherwise
+    typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
+    int id = CS::dl ? 7 : CS::f64 ? 5 : 0;
     if (ctx->gemm_cfg >= 0) {
         id = ctx->gemm_cfg;  // forced (omx_gemm_set_tile); a configuration that does not apply falls back to 0
     } else if (ctx->autotune) {

@@ -311,6 +311,34 @@ def test_autotune_matches_default():
             M.free()
 
 
+@pytest.mark.parametrize("shape", [(4096, 2048, 272), (4096 + 64, 2048, 272), (4096, 2048, 270), (256, 256, 64)])
+def test_gemm_f64_gram_direct_lds(shape):
+    """FP64 Gram C = -2 A B^T + 0.5 C0 (op 'n','t'): the default configuration 7 loads operand rows
+    straight into LDS where m, n % 128 == 0, k % 16 == 0 and the tiles fill the chip (the first
+    shape); the others fall back to the staged kernel with the same tiles (configuration 5).
+    Results are bitwise those of configuration 5 and within the FP64 bar of numpy."""
+    m, n, k = shape
+    _note("gemm_f64 gram direct-to-LDS", shape)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    C0 = rng.standard_normal((m, n))
+    ref = -2.0 * (A @ B.T) + 0.5 * C0
+    with oh.Ctx(0) as c:
+        dA, dB = oh.DMat.from_host(c, A), oh.DMat.from_host(c, B)
+        out = []
+        for cfg in (-1, 5):
+            oh.check(oh.lib().omx_gemm_set_tile(c.h, cfg), "gemm_set_tile")
+            dC = oh.DMat.from_host(c, C0)
+            oh.gemm(c, "n", "t", dA, dB, dC, alpha=-2.0, beta=0.5)
+            out.append(dC.download())
+            dC.free()
+        oh.check(oh.lib().omx_gemm_set_tile(c.h, -1), "gemm_set_tile")
+        dA.free()
+        dB.free()
+    assert np.array_equal(out[0], out[1])
+    assert np.max(np.abs(out[0] - ref)) <= 1e-12 * np.sqrt(k) * 4
+
+
 @pytest.mark.parametrize("op", ["n", "t"])
 def test_add_scale_set(ctx, op):
     m, n = 70, 45
