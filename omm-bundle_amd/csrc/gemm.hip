@@ -728,17 +728,20 @@ __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *_
     }
 }
 
+// LDS tile of the BF16 panel product: i-major rows of BK = 64 BF16 (128 bytes, 8 chunks of 16 B),
+// no padding.  Logical chunk c of row r is stored at physical chunk c ^ bg_swz(r).  A fragment
+// read takes one chunk of 16 consecutive rows; with this swizzle those 16 reads cover 16
+// distinct 16-byte bank groups, so the read is free of bank conflicts.
+__device__ inline int bg_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
 // One BK = 64 slice of the BF16 panel product, software-pipelined: the wave's MFMAs run in groups
 // (one A fragment x the FBN B fragments of a 32-wide k step), and the LDS reads for group g + LEAD
 // are issued before the MFMAs of group g, so that LEAD groups of reads are in flight behind the
 // matrix core (the waits become lgkmcnt(n > 0) instead of a full drain after every read).  The
-// scheduling fences keep the compiler from regrouping the reads.  Per accumulator the MFMA chain
-// is the one mma_slice issues, so results are bitwise the same.
+// scheduling fences keep the compiler from regrouping the reads.
 template <class CF, int LEAD>
 __device__ inline void mma_slice_bf16_pipe(const __bf16 *As, const __bf16 *Bs, f4v (&acc)[CF::FBM][CF::FBN], int wm,
                                            int wn, int lane) {
-    typedef LdsT<OMX_PREC_BF16, CF::BM> LA;
-    typedef LdsT<OMX_PREC_BF16, CF::BN> LB;
     constexpr int BK = Prec<OMX_PREC_BF16>::BK, NKS = BK / 32, NG = NKS * CF::FBM;
     const int li = lane & 15, lk = lane >> 4;
     bf8v fa[NKS][CF::FBM], fb[NKS][CF::FBN];
@@ -747,12 +750,16 @@ __device__ inline void mma_slice_bf16_pipe(const __bf16 *As, const __bf16 *Bs, f
         const int lg = g + LEAD;
         if (lg >= 0 && lg < NG) {
             const int ks = lg / CF::FBM, fm = lg % CF::FBM;
+            const int c = ks * 4 + lk;  // logical 16-byte chunk: k = 8c .. 8c+7
             if (fm == 0) {
 #pragma unroll
-                for (int f = 0; f < CF::FBN; ++f)
-                    fb[ks][f] = *reinterpret_cast<const bf8v *>(&Bs[LB::off(wn * CF::WN + f * 16 + li, ks * 32 + 8 * lk)]);
+                for (int f = 0; f < CF::FBN; ++f) {
+                    const int r = wn * CF::WN + f * 16 + li;
+                    fb[ks][f] = *reinterpret_cast<const bf8v *>(&Bs[r * BK + (c ^ bg_swz(r)) * 8]);
+                }
             }
-            fa[ks][fm] = *reinterpret_cast<const bf8v *>(&As[LA::off(wm * CF::WM + fm * 16 + li, ks * 32 + 8 * lk)]);
+            const int r = wm * CF::WM + fm * 16 + li;
+            fa[ks][fm] = *reinterpret_cast<const bf8v *>(&As[r * BK + (c ^ bg_swz(r)) * 8]);
         }
         __builtin_amdgcn_sched_barrier(0);
         if (g >= 0) {
@@ -768,19 +775,25 @@ __device__ inline void mma_slice_bf16_pipe(const __bf16 *As, const __bf16 *Bs, f
 // C := alpha * sum_kk At[kk + i*lda] * Bt[kk + j*ldb] + beta0*C0 + beta*C over kk < K (K % 64 == 0,
 // lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation; C, C0 of type TO (the
 // storage type, FP32 or FP64).
+//
+// Operand rows go straight from the panels into LDS: one global_load_lds_dwordx4 per wave moves 8
+// rows of a slice (1 KB, lane l landing at byte 16*l), each lane fetching the logical chunk that
+// belongs at its physical position under bg_swz.  No staging registers and no LDS write
+// instructions; two LDS stages, the loads of slice s+1 in flight while slice s feeds the MFMAs.
+// Rows past m or n are fetched from the last valid row and their results are not stored.
 template <class CF, typename TO>
 __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int K, double alpha,
                                                      const __bf16 *__restrict__ At, int64_t lda,
                                                      const __bf16 *__restrict__ Bt, int64_t ldb, double beta0,
                                                      const TO *__restrict__ C0, double beta, TO *__restrict__ C,
                                                      int64_t ldc) {
-    typedef LdsT<OMX_PREC_BF16, CF::BM> LA;
-    typedef LdsT<OMX_PREC_BF16, CF::BN> LB;
     constexpr int BK = Prec<OMX_PREC_BF16>::BK;
-    constexpr int VPR = BK / 8;  // 16-byte vectors per row of a slice
-    constexpr int NVA = CF::BM * VPR / CF::NT, NVB = CF::BN * VPR / CF::NT;
-    static_assert(NVA * CF::NT == CF::BM * VPR && NVB * CF::NT == CF::BN * VPR, "slice not divisible");
-    __shared__ __bf16 smem[2 * (LA::ELEMS + LB::ELEMS)];
+    constexpr int NW = CF::WGM * CF::WGN;
+    constexpr int INS = (CF::BM + CF::BN) / 8;  // wave loads per slice, 8 rows each
+    constexpr int LPW = INS / NW;
+    constexpr int STAGE = (CF::BM + CF::BN) * BK;  // BF16 per LDS stage
+    static_assert(LPW * NW == INS && CF::BM % 8 == 0, "rows per wave");
+    __shared__ __bf16 smem[2 * STAGE];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
@@ -795,51 +808,34 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
 #pragma unroll
         for (int b = 0; b < CF::FBN; ++b) acc[a][b] = f4v{0, 0, 0, 0};
 
-    bf8v ra[NVA], rb[NVB];
-    const bf8v zero = {};
-    auto fetch = [&](int k0) {
+    auto issue = [&](int s) {
+        __bf16 *st = smem + (s & 1) * STAGE;
+        const int k0 = s * BK;
 #pragma unroll
-        for (int q = 0; q < NVA; ++q) {
-            const int idx = t + CF::NT * q, r = idx / VPR, kv = (idx % VPR) * 8;
-            const int gi = i0 + r;
-            ra[q] = gi < m ? *reinterpret_cast<const bf8v *>(At + (int64_t)gi * lda + k0 + kv) : zero;
-        }
-#pragma unroll
-        for (int q = 0; q < NVB; ++q) {
-            const int idx = t + CF::NT * q, r = idx / VPR, kv = (idx % VPR) * 8;
-            const int gj = j0 + r;
-            rb[q] = gj < n ? *reinterpret_cast<const bf8v *>(Bt + (int64_t)gj * ldb + k0 + kv) : zero;
-        }
-    };
-    auto stash = [&](__bf16 *As, __bf16 *Bs) {
-#pragma unroll
-        for (int q = 0; q < NVA; ++q) {
-            const int idx = t + CF::NT * q;
-            *reinterpret_cast<bf8v *>(&As[LA::off(idx / VPR, (idx % VPR) * 8)]) = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < NVB; ++q) {
-            const int idx = t + CF::NT * q;
-            *reinterpret_cast<bf8v *>(&Bs[LB::off(idx / VPR, (idx % VPR) * 8)]) = rb[q];
+        for (int q = 0; q < LPW; ++q) {
+            const int ins = wave * LPW + q;        // A rows first, then B rows
+            const int rr = ins * 8 + (lane >> 3);  // tile row this lane fills
+            const int p = lane & 7;                // physical chunk it fills
+            const __bf16 *src;
+            if (rr < CF::BM) {
+                const int gi = min(i0 + rr, m - 1);
+                src = At + (int64_t)gi * lda + k0 + (p ^ bg_swz(rr)) * 8;
+            } else {
+                const int r = rr - CF::BM, gj = min(j0 + r, n - 1);
+                src = Bt + (int64_t)gj * ldb + k0 + (p ^ bg_swz(r)) * 8;
+            }
+            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + ins * 8 * BK), 16, 0, 0);
         }
     };
 
-    __bf16 *As0 = smem, *Bs0 = smem + LA::ELEMS;
-    __bf16 *As1 = smem + LA::ELEMS + LB::ELEMS, *Bs1 = As1 + LA::ELEMS;
     const int nslices = K / BK;
-    if (nslices > 0) {
-        fetch(0);
-        stash(As0, Bs0);
-        __syncthreads();
-        for (int s = 0; s < nslices; ++s) {
-            __bf16 *Ac = (s & 1) ? As1 : As0, *Bc = (s & 1) ? Bs1 : Bs0;
-            __bf16 *An = (s & 1) ? As0 : As1, *Bn = (s & 1) ? Bs0 : Bs1;
-            const bool more = (s + 1 < nslices);
-            if (more) fetch((s + 1) * BK);
-            mma_slice_bf16_pipe<CF, 2>(Ac, Bc, acc, wm, wn, lane);
-            if (more) stash(An, Bn);
-            __syncthreads();
-        }
+    if (nslices > 0) issue(0);
+    for (int s = 0; s < nslices; ++s) {
+        __builtin_amdgcn_s_waitcnt(waitcnt_vmcnt(0));  // this wave's rows of slice s have landed
+        __syncthreads();                               // everyone's have; slice s-1 is no longer read
+        if (s + 1 < nslices) issue(s + 1);
+        const __bf16 *As = smem + (s & 1) * STAGE, *Bs = As + CF::BM * BK;
+        mma_slice_bf16_pipe<CF, 2>(As, Bs, acc, wm, wn, lane);
     }
 
     const int li = lane & 15, lq = lane >> 4;
