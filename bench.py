@@ -241,7 +241,7 @@ def main():
             res[lab] = (i2, d2, p2)
 
     if args.mixed_steps > 0:
-        # the mixed solve's low-precision phase (FP32 state, BF16 Grams, FP32 gradient): K steps,
+        # the mixed solve's low-precision phase (FP32 state, BF16 Grams, BF16x3 gradient): K steps,
         # then one FP64 refinement step; the rate comes from the per-step timestamps
         K = args.mixed_steps
         F.set_option("lowp_max_iter", K + 1)

@@ -194,8 +194,9 @@ int omx_bsr_set_walk_groups(omx_bsr A, const int *order, int group);
  * for the Gram and gradient products: 0 = 128x128 tiles, 4 waves, two workgroups per CU;
  * 1 = 128x128, one per CU; 2 = 256x128; 3 = 128x256; 5 = 128x128 with 8 waves of 32x64 (the FP64
  * default); 6 = 128x128 with 8 waves of 64x32 (FP64 products); 4 = 256x256 with 8 waves (FP32 and
- * BF16 products on FP32 storage); 7 = the tiles of 5 with operand rows loaded straight into LDS
- * (FP64 A*B^T with m, n % 128 == 0 and k % 16 == 0, the default there; other shapes take 5).  A
+ * BF16 products on FP32 storage); 7 = the tiles of 5 with the operands loaded straight into LDS
+ * (FP64 Gram and gradient products with m, n % 128 == 0 and k % 16 == 0, the default there; other
+ * shapes take 5).  A
  * configuration that does not apply to a product falls back to 0.  Used by the tile autotuner;
  * results do not depend on it. */
 int omx_gemm_set_tile(omx_ctx ctx, int cfg);

@@ -392,29 +392,37 @@ __global__ void k_splitk_reduce(int m, int n, int splits, const AT *__restrict__
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // ---------------------------------------------------------------------------------------------
-// FP64 Gram with direct-to-LDS staging (configuration 7): C := alpha*A*B^T + beta0*C0 + beta*C,
-// A m x k (ld lda) and B n x k (ld ldb), both with the row index contiguous.  Every k-row of a
-// 128-wide operand tile is then 1 KB, which one wave moves with one global_load_lds_dwordx4 (16
-// bytes per lane, lane l landing at byte 16*l of the LDS row, checked on the device) straight into
-// the k-major LDS tile: no staging registers and no LDS write instructions.  Two LDS stages; the
-// loads of slice s+1 are in flight while slice s feeds the MFMAs.  Tiles, waves and the MFMA
-// chain per element are those of the staged kernel with Cfg5, so the results are bitwise the same.
-// Requires m, n % 128 == 0, k % 16 == 0, even leading dimensions and 16-byte aligned operands.
+// FP64 products with direct-to-LDS staging (configuration 7): C := alpha*sum_p A_p*op(B_p) + beta0*C0
+// + beta*C over NPAIR products back to back along K (the Gram: NPAIR = 1, op 't'; calc_G's pair:
+// NPAIR = 2, op 'n').  A is m x k with the row index contiguous, so every k-row of a 128-wide tile
+// is 1 KB, which one wave moves with one global_load_lds_dwordx4 (16 bytes per lane, lane l landing
+// at byte 16*l of the destination, checked on the device) straight into the k-major LDS tile.
+//   B 't' (n x k, row index contiguous): the same k-major rows.
+//   B 'n' (k x n, k contiguous): a j-major tile, one 128-byte row of 16 k values per column j, 8
+//   columns per wave load.  Logical 16-byte chunk c of column j is stored at physical chunk
+//   c ^ ((j >> 1) & 7), so the 16 columns one fragment read touches fall in distinct bank groups.
+// No staging registers and no LDS write instructions; two LDS stages, the loads of slice s+1 in
+// flight while slice s feeds the MFMAs.  Tiles, waves and the MFMA chain per element are those of
+// the staged kernel with Cfg5, so the results are bitwise the same.  Requires m, n % 128 == 0,
+// k % 16 == 0, even leading dimensions and 16-byte aligned operands.
 // ---------------------------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
 // s_waitcnt immediate (gfx9 encoding): vmcnt(vm), expcnt and lgkmcnt left at their maxima
 constexpr int waitcnt_vmcnt(int vm) { return (vm & 0xF) | ((vm >> 4) << 14) | (7 << 4) | (0xF << 8); }
+__device__ inline int dl_swz(int j) { return (j >> 1) & 7; }
 
-template <class CF>
-__global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_nt_dl(int m, int n, int k, double alpha,
-                                                                  const double *__restrict__ A, int64_t lda,
-                                                                  const double *__restrict__ B, int64_t ldb,
-                                                                  double beta0, const double *__restrict__ C0,
-                                                                  double beta, double *__restrict__ C, int64_t ldc) {
+template <class CF, bool TRB, int NPAIR>
+__global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int k, double alpha,
+                                                               const double *__restrict__ A1,
+                                                               const double *__restrict__ A2, int64_t lda,
+                                                               const double *__restrict__ B1,
+                                                               const double *__restrict__ B2, int64_t ldb,
+                                                               double beta0, const double *__restrict__ C0,
+                                                               double beta, double *__restrict__ C, int64_t ldc) {
     static_assert(CF::BM == 128 && CF::BN == 128, "one 1 KB row per wave load");
-    constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, ROWS = 2 * BK, LPW = ROWS / NW;
-    constexpr int STAGE = ROWS * 128;  // doubles: BK rows of A, then BK rows of B
-    static_assert(LPW * NW == ROWS, "rows per wave");
+    constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, INS = 2 * BK, LPW = INS / NW;
+    constexpr int STAGE = INS * 128;  // doubles: A rows (BK x 128), then B (BK x 128 or 128 x BK)
+    static_assert(LPW * NW == INS, "loads per wave");
     __shared__ double smem[2 * STAGE];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
@@ -428,17 +436,31 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_nt_dl(int m, int n, 
         for (int b = 0; b < CF::FBN; ++b) acc[a][b] = d4v{0, 0, 0, 0};
     auto issue = [&](int s) {
         double *st = smem + (s & 1) * STAGE;
-        const int k0 = s * BK;
+        int kl = s * BK;
+        const double *A = A1, *B = B1;
+        if (NPAIR == 2 && kl >= k) {
+            A = A2;
+            B = B2;
+            kl -= k;
+        }
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
-            const int row = wave * LPW + q;  // < BK: row k of A; else row k - BK of B
-            const double *src = row < BK ? A + (int64_t)(i0 + 2 * lane) + (int64_t)(k0 + row) * lda
-                                         : B + (int64_t)(j0 + 2 * lane) + (int64_t)(k0 + row - BK) * ldb;
-            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + row * 128), 16, 0, 0);
+            const int ins = wave * LPW + q;  // < BK: row k of A; else the B part
+            const double *src;
+            if (ins < BK) {
+                src = A + (int64_t)(i0 + 2 * lane) + (int64_t)(kl + ins) * lda;
+            } else if (TRB) {
+                src = B + (int64_t)(j0 + 2 * lane) + (int64_t)(kl + ins - BK) * ldb;
+            } else {
+                const int j = (ins - BK) * 8 + (lane >> 3);  // column of the tile
+                const int c = (lane & 7) ^ dl_swz(j);         // logical chunk this lane holds
+                src = B + (int64_t)(j0 + j) * ldb + kl + 2 * c;
+            }
+            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + ins * 128), 16, 0, 0);
         }
     };
     const int li = lane & 15, lk = lane >> 4;
-    const int ns = k / BK;
+    const int ns = NPAIR * k / BK;
     issue(0);
     for (int s = 0; s < ns; ++s) {
         __builtin_amdgcn_s_waitcnt(waitcnt_vmcnt(0));  // this wave's rows of slice s have landed
@@ -451,7 +473,10 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_nt_dl(int m, int n, 
 #pragma unroll
             for (int f = 0; f < CF::FBM; ++f) a[f] = As[(ks + lk) * 128 + wm * CF::WM + f * 16 + li];
 #pragma unroll
-            for (int f = 0; f < CF::FBN; ++f) b[f] = Bs[(ks + lk) * 128 + wn * CF::WN + f * 16 + li];
+            for (int f = 0; f < CF::FBN; ++f) {
+                const int j = wn * CF::WN + f * 16 + li, kk = ks + lk;
+                b[f] = TRB ? Bs[kk * 128 + j] : Bs[j * BK + ((kk >> 1) ^ dl_swz(j)) * 2 + (kk & 1)];
+            }
 #pragma unroll
             for (int fm = 0; fm < CF::FBM; ++fm)
 #pragma unroll
@@ -477,21 +502,25 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_nt_dl(int m, int n, 
 }
 
 // Launches configuration 7 when the product qualifies; returns OMX_ERR_UNSUPPORTED otherwise.
-static int run_dgemm_nt_dl(omx_ctx ctx, int m, int n, int k, double alpha, const double *A, int64_t lda,
-                           const double *B, int64_t ldb, double beta0, const double *C0, double beta, double *C,
-                           int64_t ldc) {
+template <bool TRB, int NPAIR>
+static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const double *A1, const double *A2,
+                        int64_t lda, const double *B1, const double *B2, int64_t ldb, double beta0, const double *C0,
+                        double beta, double *C, int64_t ldc) {
     typedef Cfg5 CF;
     const int tiles = (m / 128) * (n / 128);
     if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F64>::BK || lda % 2 || ldb % 2 ||
-        !aligned16(A) || !aligned16(B) || ctx->launch_mask || tiles < 2 * ctx->num_cus)
+        !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || ctx->launch_mask ||
+        tiles < 2 * ctx->num_cus)
         return OMX_ERR_UNSUPPORTED;
+    const int cls = NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM;
     hipEvent_t ev;
-    prof_begin(ctx, OMX_K_GEMM, &ev);
-    prof_work(ctx, OMX_K_GEMM, 2.0 * m * n * (double)k,
-              sizeof(double) * ((double)m * k + (double)k * n + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
-    hipLaunchKernelGGL((k_dgemm_nt_dl<CF>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A, lda, B, ldb,
-                       beta0, C0, beta, C, ldc);
-    prof_end(ctx, OMX_K_GEMM, ev);
+    prof_begin(ctx, cls, &ev);
+    prof_work(ctx, cls, 2.0 * m * n * (double)NPAIR * k,
+              sizeof(double) * ((double)NPAIR * ((double)m * k + (double)k * n) +
+                                (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
+    hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1, A2,
+                       lda, B1, B2, ldb, beta0, C0, beta, C, ldc);
+    prof_end(ctx, cls, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
@@ -544,13 +573,13 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     return OMX_OK;
 }
 
-// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3, 5, 6,
-// the FP64 Gram (A*B^T on FP64 storage) also 7; the FP32 and BF16 ones on FP32 storage 0 and 4;
-// every other product uses 0.
+// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3 and
+// 5-7 (7 on FP64 storage); the FP32 and BF16 ones on FP32 storage 0 and 4; every other product
+// uses 0.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
 struct CfgSet {
     static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
-    static constexpr bool dl = f64 && TRB && NPAIR == 1 && std::is_same<TS, double>::value;
+    static constexpr bool dl = f64 && std::is_same<TS, double>::value;
     static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
     static std::vector<int> ids() {
         if (dl) return {7, 0, 1, 2, 3, 5, 6};
@@ -567,7 +596,7 @@ static int run_cfg_id(int id, omx_ctx ctx, int m, int n, int k, double alpha, co
     typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
     if constexpr (CS::dl) {
         if (id == 7) {
-            int rc = run_dgemm_nt_dl(ctx, m, n, k, alpha, A1, lda, B1, ldb, beta0, C0, beta, C, ldc);
+            int rc = run_dgemm_dl<TRB, NPAIR>(ctx, m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc);
             if (rc != OMX_ERR_UNSUPPORTED) return rc;
             id = 5;  // the staged kernel with the same tiles covers every other shape
         }
@@ -666,7 +695,7 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                     int64_t ldc) {
     // default: 128x128 with 8 waves and two workgroups per CU for the FP64 Gram and gradient products
     // (3-4 % ahead of the 4-wave tile at n=16384, bitwise the same result), with direct-to-LDS
-    // staging for the Gram where it applies (a further 2-4 %, bitwise the same); 0 otherwise
+    // staging where it applies (bitwise the same); 0 otherwise
     typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
     int id = CS::dl ? 7 : CS::f64 ? 5 : 0;
     if (ctx->gemm_cfg >= 0) {

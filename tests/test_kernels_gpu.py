@@ -339,6 +339,34 @@ def test_gemm_f64_gram_direct_lds(shape):
     assert np.max(np.abs(out[0] - ref)) <= 1e-12 * np.sqrt(k) * 4
 
 
+@pytest.mark.parametrize("no,nb", [(1024, 8192), (1024 + 64, 8192), (256, 512)])
+def test_calc_G_f64_direct_lds(no, nb):
+    """FP64 calc_G pair G = 4HC - 2 SW HC - 2 HW SC: configuration 7 (operands loaded straight into
+    LDS, HC/SC through the swizzled column-major tile) where m, n % 128 == 0, k % 16 == 0 and the
+    tiles fill the chip (the first shape), the staged kernel otherwise.  Bitwise the same as
+    configuration 5; sampled columns within the FP64 bar of numpy."""
+    _note("calc_G f64 direct-to-LDS", no, nb)
+    HC, SC = rng.standard_normal((no, nb)), rng.standard_normal((no, nb))
+    W1, W2 = rng.standard_normal((no, no)), rng.standard_normal((no, no))
+    HW, SW = W1 + W1.T, W2 + W2.T
+    cols = rng.choice(nb, 64, replace=False)
+    ref = 4 * HC[:, cols] - 2 * SW @ HC[:, cols] - 2 * HW @ SC[:, cols]
+    with oh.Ctx(0) as c:
+        d = {k_: oh.DMat.from_host(c, v) for k_, v in (("HC", HC), ("SC", SC), ("HW", HW), ("SW", SW))}
+        out = []
+        for cfg in (-1, 5):
+            oh.check(oh.lib().omx_gemm_set_tile(c.h, cfg), "gemm_set_tile")
+            G = oh.DMat(c, no, nb)
+            oh.omm_calc_G(c, d["HW"], d["SW"], d["HC"], d["SC"], G)
+            out.append(G.download())
+            G.free()
+        oh.check(oh.lib().omx_gemm_set_tile(c.h, -1), "gemm_set_tile")
+        for v in d.values():
+            v.free()
+    assert np.array_equal(out[0], out[1])
+    assert np.max(np.abs(out[0][:, cols] - ref)) <= 1e-12 * np.sqrt(no) * 8 * np.max(np.abs(ref))
+
+
 @pytest.mark.parametrize("op", ["n", "t"])
 def test_add_scale_set(ctx, op):
     m, n = 70, 45
