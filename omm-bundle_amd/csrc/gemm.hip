@@ -720,39 +720,81 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
 // which keeps about 16 significant bits of each operand (the BF16x3 product).
 // ---------------------------------------------------------------------------------------------
 
-constexpr int BG_T = 32;  // transpose tile
+constexpr int BG_T = 64;  // conversion tile (rows x cols)
 
 // dst := bf16(src) (and dstlo := bf16(src - hi), in the source precision) with src rows x cols
 // (ld lds), FP32 or FP64, each rounded once from the stored value.  TRANS: element (r, c) goes to
-// dst[c + r*ldd], else to dst[r + c*ldd].
+// dst[c + r*ldd], else to dst[r + c*ldd].  One 64 x 64 tile per workgroup: 16-byte loads along r,
+// and 16-byte stores of 8 BF16 along the destination's contiguous index (through an LDS transpose
+// for TRANS).  Edges and unaligned operands take element-wise paths.
 template <bool TRANS, typename TS>
 __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *__restrict__ src, int64_t lds,
                                                  __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, int64_t ldd) {
-    const int tx = threadIdx.x % BG_T, ty = threadIdx.x / BG_T;
+    constexpr int VEC = 16 / sizeof(TS);  // elements per 16-byte load
+    const int t = threadIdx.x;
     const int r0 = blockIdx.x * BG_T, c0 = blockIdx.y * BG_T;
+    const bool vld = (lds % VEC) == 0 && ((uintptr_t)src & 15) == 0;
+    const bool vst = (ldd % 8) == 0 && ((uintptr_t)hi & 15) == 0 && (!lo || ((uintptr_t)lo & 15) == 0);
+    auto load = [&](int gr, int gc, int count, TS *x) {  // x[0..count) = src(gr.., gc), zero outside
+        const TS *p = src + (int64_t)gr + (int64_t)gc * lds;
+        if (vld && gc < cols && gr + count <= rows) {
+#pragma unroll
+            for (int e = 0; e < count; e += VEC) {
+                typedef TS vec_t __attribute__((ext_vector_type(VEC)));
+                const vec_t v = *reinterpret_cast<const vec_t *>(p + e);
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) x[e + q] = v[q];
+            }
+        } else {
+            for (int e = 0; e < count; ++e) x[e] = (gc < cols && gr + e < rows) ? p[e] : (TS)0;
+        }
+    };
+    auto store8 = [&](const TS *x, int64_t off, int avail) {  // 8 conversions to hi[off..], lo[off..]
+        bf8v h8, l8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const __bf16 h = (__bf16)x[e];
+            h8[e] = h;
+            l8[e] = (__bf16)(x[e] - (TS)h);
+        }
+        if (vst && avail >= 8) {
+            *reinterpret_cast<bf8v *>(hi + off) = h8;
+            if (lo) *reinterpret_cast<bf8v *>(lo + off) = l8;
+        } else {
+            for (int e = 0; e < 8 && e < avail; ++e) {
+                hi[off + e] = h8[e];
+                if (lo) lo[off + e] = l8[e];
+            }
+        }
+    };
     if constexpr (TRANS) {
-        __shared__ TS tile[BG_T][BG_T + 1];
-        for (int c = ty; c < BG_T; c += 256 / BG_T) {
-            const int r = r0 + tx, cc = c0 + c;
-            tile[c][tx] = (r < rows && cc < cols) ? src[(int64_t)r + (int64_t)cc * lds] : (TS)0;
+        __shared__ TS tile[BG_T][BG_T + 1];  // [c][r]
+        constexpr int VPC = BG_T / VEC;      // load vectors per tile column
+        for (int v = t; v < BG_T * VPC; v += 256) {
+            const int c = v / VPC, rv = (v % VPC) * VEC;
+            TS x[VEC];
+            load(r0 + rv, c0 + c, VEC, x);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) tile[c][rv + e] = x[e];
         }
         __syncthreads();
-        for (int r = ty; r < BG_T; r += 256 / BG_T) {
-            const int rr = r0 + r, cc = c0 + tx;  // write along c (contiguous in dst)
-            if (rr >= rows || cc >= cols) continue;
-            const TS v = tile[tx][r];
-            const __bf16 h = (__bf16)v;
-            hi[(int64_t)cc + (int64_t)rr * ldd] = h;
-            if (lo) lo[(int64_t)cc + (int64_t)rr * ldd] = (__bf16)(v - (TS)h);
+        for (int u = t; u < BG_T * (BG_T / 8); u += 256) {
+            const int r = u / (BG_T / 8), c8 = (u % (BG_T / 8)) * 8;
+            const int gr = r0 + r, gc = c0 + c8;
+            if (gr >= rows || gc >= cols) continue;
+            TS x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = tile[c8 + e][r];
+            store8(x, (int64_t)gc + (int64_t)gr * ldd, cols - gc);
         }
     } else {
-        for (int c = ty; c < BG_T; c += 256 / BG_T) {
-            const int r = r0 + tx, cc = c0 + c;
-            if (r >= rows || cc >= cols) continue;
-            const TS v = src[(int64_t)r + (int64_t)cc * lds];
-            const __bf16 h = (__bf16)v;
-            hi[(int64_t)r + (int64_t)cc * ldd] = h;
-            if (lo) lo[(int64_t)r + (int64_t)cc * ldd] = (__bf16)(v - (TS)h);
+        for (int u = t; u < BG_T * (BG_T / 8); u += 256) {
+            const int c = u / (BG_T / 8), r8 = (u % (BG_T / 8)) * 8;
+            const int gr = r0 + r8, gc = c0 + c;
+            if (gr >= rows || gc >= cols) continue;
+            TS x[8];
+            load(gr, gc, 8, x);
+            store8(x, (int64_t)gr + (int64_t)gc * ldd, rows - gr);
         }
     }
 }

@@ -90,12 +90,14 @@ def _split3(x):
 @pytest.mark.parametrize("opA,opB", OPS)
 @pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
 @pytest.mark.parametrize("dt", [oh.OMX_F32, oh.OMX_F64])
-def test_gemm_bf16_panels(ctx, opA, opB, prec, dt):
+@pytest.mark.parametrize("shape", [(300, 260, 512), (301, 259, 576)])
+def test_gemm_bf16_panels(ctx, opA, opB, prec, dt, shape):
     # shapes the k-contiguous BF16 panels cover (m, n >= 256, k % 64 == 0), FP32 or FP64 storage:
     # BF16 against the product of the operands rounded once from the stored value; BF16x3 against
-    # hi*hi + hi*lo + lo*hi and, much tighter than BF16, against the exact product
-    m, n, k = 300, 260, 512
-    _note("gemm_bf16_panels", dt, prec, opA, opB)
+    # hi*hi + hi*lo + lo*hi and, much tighter than BF16, against the exact product.  The odd shape
+    # gives odd leading dimensions, so the panel conversion takes its element-wise paths.
+    m, n, k = shape
+    _note("gemm_bf16_panels", dt, prec, opA, opB, shape)
     npdt = np.float32 if dt == oh.OMX_F32 else np.float64
     A = rng.standard_normal((k, m) if opA == "t" else (m, k)).astype(npdt)
     B = rng.standard_normal((n, k) if opB == "t" else (k, n)).astype(npdt)
