@@ -50,6 +50,11 @@ extern "C" {
  * BF16 MFMA with FP32 accumulation, about 16 significant bits per operand (real FP32 or FP64
  * storage; shapes the BF16 panels do not cover run as OMX_PREC_F32) */
 #define OMX_PREC_BF16X3 3
+/* Operands split into three BF16 terms (hi, mid = bf16(a - hi), lo = bf16(a - hi - mid); exact for
+ * FP32 values); the six term products of order >= 2^-16 on BF16 MFMA with FP32 accumulation:
+ * FP32-grade products (real FP32 or FP64 storage; shapes the BF16 panels do not cover run as
+ * OMX_PREC_F32) */
+#define OMX_PREC_BF16X6 4
 
 /* ---- counter-based generator streams (lattice inputs) ------------------------------------- */
 #define OMX_STREAM_ONSITE 1

@@ -722,19 +722,58 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
 
 constexpr int BG_T = 64;  // conversion tile (rows x cols)
 
-// dst := bf16(src) (and dstlo := bf16(src - hi), in the source precision) with src rows x cols
-// (ld lds), FP32 or FP64, each rounded once from the stored value.  TRANS: element (r, c) goes to
-// dst[c + r*ldd], else to dst[r + c*ldd].  One 64 x 64 tile per workgroup: 16-byte loads along r,
-// and 16-byte stores of 8 BF16 along the destination's contiguous index (through an LDS transpose
-// for TRANS).  Edges and unaligned operands take element-wise paths.
-template <bool TRANS, typename TS>
+// Destinations of one operand's conversion: BgLay<L>::term[q] (1 = hi = bf16(a), 2 = mid =
+// bf16(a - hi), 3 = lo = bf16(a - hi - mid), each difference taken exactly in the source
+// precision) goes to p[q], q < BgLay<L>::n.  For FP32 sources hi + mid + lo is the value exactly;
+// hi + mid keeps about 16 significant bits.  Layouts: 0 = BF16; 1, 2 = BF16x3 A and B; 3, 4 =
+// BF16x6 A and B (the k blocks of the panel product, in order).
+struct BgDst {
+    __bf16 *p[6];
+};
+template <int L>
+struct BgLay;
+template <>
+struct BgLay<0> {
+    static constexpr int n = 1;
+    __device__ static constexpr int term(int) { return 1; }
+};
+template <>
+struct BgLay<1> {
+    static constexpr int n = 3;
+    __device__ static constexpr int term(int q) { return q == 2 ? 2 : 1; }  // hi hi mid
+};
+template <>
+struct BgLay<2> {
+    static constexpr int n = 3;
+    __device__ static constexpr int term(int q) { return q == 1 ? 2 : 1; }  // hi mid hi
+};
+template <>
+struct BgLay<3> {
+    static constexpr int n = 6;
+    __device__ static constexpr int term(int q) { return q == 1 ? 3 : (q == 2 || q == 4) ? 2 : 1; }  // 1 3 2 1 2 1
+};
+template <>
+struct BgLay<4> {
+    static constexpr int n = 6;
+    __device__ static constexpr int term(int q) { return q == 0 ? 3 : (q == 2 || q == 3) ? 2 : 1; }  // 3 1 2 2 1 1
+};
+
+// dst terms of src (rows x cols, ld lds, FP32 or FP64), each rounded once from the stored value:
+// TRANS: element (r, c) goes to dst[c + r*ldd], else to dst[r + c*ldd].  One 64 x 64 tile per
+// workgroup: 16-byte loads along r, and 16-byte stores of 8 BF16 along the destination's
+// contiguous index (through an LDS transpose for TRANS).  Edges and unaligned operands take
+// element-wise paths.
+template <bool TRANS, typename TS, int L>
 __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *__restrict__ src, int64_t lds,
-                                                 __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, int64_t ldd) {
+                                                 BgDst dst, int64_t ldd) {
+    constexpr int ND = BgLay<L>::n;
     constexpr int VEC = 16 / sizeof(TS);  // elements per 16-byte load
     const int t = threadIdx.x;
     const int r0 = blockIdx.x * BG_T, c0 = blockIdx.y * BG_T;
     const bool vld = (lds % VEC) == 0 && ((uintptr_t)src & 15) == 0;
-    const bool vst = (ldd % 8) == 0 && ((uintptr_t)hi & 15) == 0 && (!lo || ((uintptr_t)lo & 15) == 0);
+    bool vst = (ldd % 8) == 0;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) vst = vst && ((uintptr_t)dst.p[q] & 15) == 0;
     auto load = [&](int gr, int gc, int count, TS *x) {  // x[0..count) = src(gr.., gc), zero outside
         const TS *p = src + (int64_t)gr + (int64_t)gc * lds;
         if (vld && gc < cols && gr + count <= rows) {
@@ -749,21 +788,25 @@ __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *_
             for (int e = 0; e < count; ++e) x[e] = (gc < cols && gr + e < rows) ? p[e] : (TS)0;
         }
     };
-    auto store8 = [&](const TS *x, int64_t off, int avail) {  // 8 conversions to hi[off..], lo[off..]
-        bf8v h8, l8;
+    auto store8 = [&](const TS *x, int64_t off, int avail) {  // the terms of 8 values to every destination
+        bf8v t1, t2, t3;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const __bf16 h = (__bf16)x[e];
-            h8[e] = h;
-            l8[e] = (__bf16)(x[e] - (TS)h);
+            const TS r1 = x[e] - (TS)h;
+            const __bf16 md = (__bf16)r1;
+            t1[e] = h;
+            t2[e] = md;
+            t3[e] = (__bf16)(r1 - (TS)md);
         }
-        if (vst && avail >= 8) {
-            *reinterpret_cast<bf8v *>(hi + off) = h8;
-            if (lo) *reinterpret_cast<bf8v *>(lo + off) = l8;
-        } else {
-            for (int e = 0; e < 8 && e < avail; ++e) {
-                hi[off + e] = h8[e];
-                if (lo) lo[off + e] = l8[e];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int tq = BgLay<L>::term(q);
+            const bf8v v = tq == 1 ? t1 : (tq == 2 ? t2 : t3);
+            if (vst && avail >= 8) {
+                *reinterpret_cast<bf8v *>(dst.p[q] + off) = v;
+            } else {
+                for (int e = 0; e < 8 && e < avail; ++e) dst.p[q][off + e] = v[e];
             }
         }
     };
@@ -928,16 +971,6 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
         }
 }
 
-// dst row r := src row r over k elements (rows of k-contiguous panels; k a multiple of 8, rows
-// 16-byte aligned).
-__global__ __launch_bounds__(256) void k_copy_cols_bf16(int rows, int k, const __bf16 *__restrict__ src, int64_t lds,
-                                                        __bf16 *__restrict__ dst, int64_t ldd) {
-    for (int r = blockIdx.x; r < rows; r += gridDim.x)
-        for (int q = threadIdx.x * 8; q < k; q += 256 * 8)
-            *reinterpret_cast<bf8v *>(dst + (int64_t)r * ldd + q) =
-                *reinterpret_cast<const bf8v *>(src + (int64_t)r * lds + q);
-}
-
 // One operand term of a BF16 panel product: op(X) with X rows x cols (ld ldx); as a k-contiguous
 // panel it is `extent` x K (extent = m for A, n for B).
 struct BgTerm {
@@ -946,32 +979,49 @@ struct BgTerm {
     bool trans;  // true: X is stored K x extent (already k-contiguous)
 };
 
-template <typename TS>
-static int bg_convert(omx_ctx ctx, const BgTerm &tm, int extent, int K, __bf16 *hi, __bf16 *lo, int64_t ldd) {
+template <typename TS, int L>
+static void bg_convert_l(omx_ctx ctx, const BgTerm &tm, int extent, int K, const BgDst &dst, int64_t ldd) {
     const TS *X = (const TS *)tm.X;
     if (tm.trans) {
         // X is K x extent with k contiguous: same layout, dst[kk + e*ldd]
         dim3 grid((K + BG_T - 1) / BG_T, (extent + BG_T - 1) / BG_T);
-        hipLaunchKernelGGL((k_to_bf16<false, TS>), grid, dim3(256), 0, ctx->stream, K, extent, X, tm.ldx, hi, lo, ldd);
+        hipLaunchKernelGGL((k_to_bf16<false, TS, L>), grid, dim3(256), 0, ctx->stream, K, extent, X, tm.ldx, dst, ldd);
     } else {
         // X is extent x K: transpose so that dst[kk + e*ldd] = X(e, kk)
         dim3 grid((extent + BG_T - 1) / BG_T, (K + BG_T - 1) / BG_T);
-        hipLaunchKernelGGL((k_to_bf16<true, TS>), grid, dim3(256), 0, ctx->stream, extent, K, X, tm.ldx, hi, lo, ldd);
+        hipLaunchKernelGGL((k_to_bf16<true, TS, L>), grid, dim3(256), 0, ctx->stream, extent, K, X, tm.ldx, dst, ldd);
+    }
+}
+
+// Converts one operand into the panel blocks of layout `lay` (see BgLay).
+template <typename TS>
+static int bg_convert(omx_ctx ctx, const BgTerm &tm, int extent, int K, const BgDst &dst, int64_t ldd, int lay) {
+    switch (lay) {
+        case 0: bg_convert_l<TS, 0>(ctx, tm, extent, K, dst, ldd); break;
+        case 1: bg_convert_l<TS, 1>(ctx, tm, extent, K, dst, ldd); break;
+        case 2: bg_convert_l<TS, 2>(ctx, tm, extent, K, dst, ldd); break;
+        case 3: bg_convert_l<TS, 3>(ctx, tm, extent, K, dst, ldd); break;
+        default: bg_convert_l<TS, 4>(ctx, tm, extent, K, dst, ldd); break;
     }
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }
 
 // C := alpha * sum_t op(A_t) op(B_t) + beta0*C0 + beta*C for nterm products of inner size k on
-// TS storage (FP32 or FP64), in BF16 (split = false) or BF16x3 (split = true) through
-// k-contiguous panels in the workspace.  Returns OMX_ERR_UNSUPPORTED (nothing launched) when the
-// shape does not suit the panels.
+// TS storage (FP32 or FP64) through k-contiguous BF16 panels in the workspace.  nsplit = 1: BF16
+// operands.  nsplit = 2 (BF16x3): hi*hi + hi*mid + mid*hi, about 16 significant bits per operand.
+// nsplit = 3 (BF16x6): the six term products of order >= 2^-16 of the operands' three-term splits,
+// smallest first (hi*lo + lo*hi + mid*mid + hi*mid + mid*hi + hi*hi), FP32-grade products.
+// Returns OMX_ERR_UNSUPPORTED (nothing launched) when the shape does not suit the panels.
 template <typename TS>
-static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTerm *A, const BgTerm *B, bool split,
+static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTerm *A, const BgTerm *B, int nsplit,
                         double alpha, double beta0, const TS *C0, double beta, TS *C, int64_t ldc) {
     constexpr int BK = Prec<OMX_PREC_BF16>::BK;
     if (ctx->launch_mask || k % BK != 0 || m < 256 || n < 256 || k < 4 * BK) return OMX_ERR_UNSUPPORTED;
-    const int reps = split ? 3 : 1;
+    // k blocks per product term and the conversion layouts of A and B (BgLay)
+    static const int nblk[3] = {1, 3, 6}, lay_a[3] = {0, 1, 3}, lay_b[3] = {0, 2, 4};
+    OMX_REQUIRE(nsplit >= 1 && nsplit <= 3, "bf16 panels: split %d", nsplit);
+    const int reps = nblk[nsplit - 1];
     const int64_t K = (int64_t)k * nterm * reps;
     OMX_REQUIRE(K < (1LL << 31), "bf16 panels: K too large");
     const size_t bytes_a = sizeof(__bf16) * (size_t)K * m, bytes_b = sizeof(__bf16) * (size_t)K * n;
@@ -983,26 +1033,17 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     prof_begin(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
     prof_work(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)k * nterm,
               sizeof(TS) * (nterm * ((double)m * k + (double)k * n) + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0))));
-    // Panel layout along K: for term t, reps blocks of k: split -> [hi, hi, lo] for A and
-    // [hi, lo, hi] for B.  Rows of At are m panels of length K (ld K), rows of Bt n of length K.
+    // Rows of At are m panels of length K (ld K), rows of Bt n of length K; product term tt takes
+    // the K range [tt*reps*k, (tt+1)*reps*k), block b of it holding the A and B terms of pair b.
     for (int tt = 0; tt < nterm && !rc; ++tt) {
         const int64_t kb = (int64_t)tt * reps * k;
-        if (!split) {
-            rc = bg_convert<TS>(ctx, A[tt], m, k, At + kb, nullptr, K);
-            if (!rc) rc = bg_convert<TS>(ctx, B[tt], n, k, Bt + kb, nullptr, K);
-        } else {
-            // A: hi at kb and lo at kb + 2k, then hi copied to kb + k; B: hi at kb, lo at kb + k,
-            // hi copied to kb + 2k
-            rc = bg_convert<TS>(ctx, A[tt], m, k, At + kb, At + kb + 2 * k, K);
-            if (!rc) {
-                hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)m, 65535)), dim3(256), 0,
-                                   ctx->stream, m, k, At + kb, K, At + kb + k, K);
-                rc = bg_convert<TS>(ctx, B[tt], n, k, Bt + kb, Bt + kb + k, K);
-            }
-            if (!rc)
-                hipLaunchKernelGGL(k_copy_cols_bf16, dim3((unsigned)std::min<int64_t>((int64_t)n, 65535)), dim3(256), 0,
-                                   ctx->stream, n, k, Bt + kb, K, Bt + kb + 2 * k, K);
+        BgDst da{}, db{};
+        for (int b = 0; b < reps; ++b) {
+            da.p[b] = At + kb + (int64_t)b * k;
+            db.p[b] = Bt + kb + (int64_t)b * k;
         }
+        rc = bg_convert<TS>(ctx, A[tt], m, k, da, K, lay_a[nsplit - 1]);
+        if (!rc) rc = bg_convert<TS>(ctx, B[tt], n, k, db, K, lay_b[nsplit - 1]);
     }
     if (!rc) {
         typedef Cfg4 CF;
@@ -1015,6 +1056,9 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     prof_end(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, ev);
     return rc;
 }
+
+// BF16 panel split of a product precision: 1 term (BF16), 2 (BF16x3) or 3 (BF16x6).
+static int split_of(int prec) { return prec == OMX_PREC_BF16X6 ? 3 : prec == OMX_PREC_BF16X3 ? 2 : 1; }
 
 template <typename TS, int PREC>
 static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
@@ -1033,12 +1077,12 @@ static int dispatch_ops(omx_ctx ctx, bool trA, bool trB, int m, int n, int k, do
 int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha, const void *A,
                 int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc) {
     if (dt == OMX_F64) {
-        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3) {
+        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3 || prec == OMX_PREC_BF16X6) {
             const BgTerm a{A, lda, trA}, b{B, ldb, !trB};
-            int rc = bgemm_panels<double>(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
+            int rc = bgemm_panels<double>(ctx, m, n, k, 1, &a, &b, split_of(prec), alpha, 0.0, nullptr, beta,
                                           (double *)C, ldc);
             if (rc != OMX_ERR_UNSUPPORTED) return rc;
-            if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;  // the panels do not cover this shape
+            if (prec != OMX_PREC_BF16) prec = OMX_PREC_F32;  // the panels do not cover this shape
         }
         if (prec == OMX_PREC_F64)
             return dispatch_ops<double, OMX_PREC_F64>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -1047,13 +1091,13 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
         if (prec == OMX_PREC_BF16)
             return dispatch_ops<double, OMX_PREC_BF16>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     } else if (dt == OMX_F32) {
-        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3) {
+        if (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3 || prec == OMX_PREC_BF16X6) {
             // k-contiguous BF16 panels (A 't' and B 'n' are already k-contiguous)
             const BgTerm a{A, lda, trA}, b{B, ldb, !trB};
-            int rc = bgemm_panels<float>(ctx, m, n, k, 1, &a, &b, prec == OMX_PREC_BF16X3, alpha, 0.0, nullptr, beta,
+            int rc = bgemm_panels<float>(ctx, m, n, k, 1, &a, &b, split_of(prec), alpha, 0.0, nullptr, beta,
                                          (float *)C, ldc);
             if (rc != OMX_ERR_UNSUPPORTED) return rc;
-            if (prec == OMX_PREC_BF16X3) prec = OMX_PREC_F32;  // the panels do not cover this shape
+            if (prec != OMX_PREC_BF16) prec = OMX_PREC_F32;  // the panels do not cover this shape
         }
         if (prec == OMX_PREC_F32)
             return dispatch_ops<float, OMX_PREC_F32>(ctx, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -1068,16 +1112,17 @@ int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n,
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc) {
-    if ((dt == OMX_F32 || dt == OMX_F64) && (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3)) {
+    if ((dt == OMX_F32 || dt == OMX_F64) &&
+        (prec == OMX_PREC_BF16 || prec == OMX_PREC_BF16X3 || prec == OMX_PREC_BF16X6)) {
         const BgTerm a[2] = {{A1, lda, false}, {A2, lda, false}};
         const BgTerm b[2] = {{B1, ldb, true}, {B2, ldb, true}};
-        const bool split = prec == OMX_PREC_BF16X3;
+        const int split = split_of(prec);
         int rc = dt == OMX_F32 ? bgemm_panels<float>(ctx, m, n, k, 2, a, b, split, alpha, beta0, (const float *)C0,
                                                      beta, (float *)C, ldc)
                                : bgemm_panels<double>(ctx, m, n, k, 2, a, b, split, alpha, beta0, (const double *)C0,
                                                       beta, (double *)C, ldc);
         if (rc != OMX_ERR_UNSUPPORTED) return rc;
-        if (split) prec = OMX_PREC_F32;
+        if (prec != OMX_PREC_BF16) prec = OMX_PREC_F32;
     }
     const int bk = (prec == OMX_PREC_F64)   ? Prec<OMX_PREC_F64>::BK
                    : (prec == OMX_PREC_F32) ? Prec<OMX_PREC_F32>::BK

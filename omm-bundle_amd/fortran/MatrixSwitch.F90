@@ -11,6 +11,7 @@
 !   'hip'  FP64 products on the matrix cores          'h32'  FP32 products                         !
 !   'hbf'  BF16 products with FP32 accumulation       (traces, adds, sets: always FP64 accumulate) !
 !   'hb3'  BF16x3 products (hi*hi + hi*lo + lo*hi of FP32 operands split in two BF16 terms)        !
+!   'hb6'  BF16x6 products (six term products of a three-term BF16 split; FP32-grade)              !
 ! Host labels 'sdden' / 'sdcsc' keep the reference semantics ('ref' loops); 'lap' is served by the !
 ! same loops (no BLAS is linked).  Device matrices cannot alias host memory: m_register_sden stays !
 ! host-only, and m_copy moves data between host and device labels (upload / download).            !
@@ -138,6 +139,8 @@ contains
       prec = OMX_PREC_BF16
     case ('hb3')
       prec = OMX_PREC_BF16X3
+    case ('hb6')
+      prec = OMX_PREC_BF16X6
     case default
       call die('invalid implementation label '//label)
     end select
@@ -147,7 +150,7 @@ contains
     character(3), intent(in), optional :: label
     character(*), intent(in) :: where
     if (present(label)) then
-      if (label /= 'hip' .and. label /= 'h32' .and. label /= 'hbf' .and. label /= 'hb3') &
+      if (label /= 'hip' .and. label /= 'h32' .and. label /= 'hbf' .and. label /= 'hb3' .and. label /= 'hb6') &
         call die(where//': invalid implementation')
     end if
   end subroutine check_device_label

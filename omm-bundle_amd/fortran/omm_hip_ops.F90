@@ -29,6 +29,7 @@ module omm_hip_ops
   logical, save :: opt_grad_fp32 = .true.    ! BF16 phase: gradient and state products in FP32
   logical, save :: opt_grad_b3 = .true.      ! BF16 phase: gradient in BF16x3 (state products in FP32)
   logical, save :: opt_state_b3 = .false.    ! BF16 phase: periodic state recomputation in BF16x3
+  logical, save :: opt_state_b6 = .false.    ! BF16 phase: periodic state recomputation in BF16x6
   integer, save :: opt_lowp_stall = 60       ! low-precision phase ends after this many steps without a
                                              ! new lowest energy (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
@@ -151,13 +152,15 @@ contains
   end function grad_label
 
   !> Label of the periodic recomputation of HC, SC, HW, SW in the low-precision phase: FP32 unless
-  !> the gradient itself runs in plain BF16, or BF16x3 with lowp_state_bf16x3.
+  !> the gradient itself runs in plain BF16; BF16x3 with lowp_state_bf16x3, BF16x6 (FP32-grade
+  !> products on BF16 MFMA) with lowp_state_bf16x6.
   function state_label(lab) result(g)
     character(3), intent(in) :: lab
     character(3) :: g
     g = lab
     if (lab == 'hbf' .and. (opt_grad_fp32 .or. opt_grad_b3)) g = 'h32'
     if (lab == 'hbf' .and. opt_state_b3) g = 'hb3'
+    if (lab == 'hbf' .and. opt_state_b6) g = 'hb6'
   end function state_label
 
   !> Reductions always accumulate in FP64; low-precision product labels map to 'hip' for them.
@@ -165,7 +168,7 @@ contains
     character(3), intent(in) :: lab
     character(3) :: t
     t = lab
-    if (lab == 'h32' .or. lab == 'hbf' .or. lab == 'hb3') t = 'hip'
+    if (lab == 'h32' .or. lab == 'hbf' .or. lab == 'hb3' .or. lab == 'hb6') t = 'hip'
   end function trace_label
 
   !> Minimum of the quartic line-search polynomial (omm_quartic.F90:60-153, Numerical Recipes cubic).
@@ -365,6 +368,8 @@ contains
       opt_grad_b3 = (value /= 0.0_dp)
     case ('lowp_state_bf16x3')
       opt_state_b3 = (value /= 0.0_dp)
+    case ('lowp_state_bf16x6')
+      opt_state_b6 = (value /= 0.0_dp)
     case ('lowp_stall')
       opt_lowp_stall = max(0, nint(value))
     case default

@@ -7,7 +7,8 @@ module omx_capi
   implicit none
 
   integer(c_int), parameter :: OMX_F64=0, OMX_F32=1, OMX_BF16=2, OMX_C128=3, OMX_C64=4
-  integer(c_int), parameter :: OMX_PREC_F64=0, OMX_PREC_F32=1, OMX_PREC_BF16=2, OMX_PREC_BF16X3=3
+  integer(c_int), parameter :: OMX_PREC_F64=0, OMX_PREC_F32=1, OMX_PREC_BF16=2, OMX_PREC_BF16X3=3, &
+                                OMX_PREC_BF16X6=4
   integer(c_int), parameter :: OMX_K_SPMM=0, OMX_K_GEMM=1, OMX_K_COEFF=2, OMX_K_ELEM=3, OMX_K_DOT=4
 
   interface

@@ -24,14 +24,15 @@ def main(argv=None):
     ap.add_argument("kernel", choices=["spmm", "gemm", "calc_g"])
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
-    ap.add_argument("--prec", choices=["f64", "f32", "bf16", "bf16x3"], default=None)
+    ap.add_argument("--prec", choices=["f64", "f32", "bf16", "bf16x3", "bf16x6"], default=None)
     ap.add_argument("--pad", type=int, default=0, help="extra rows in the leading dimension of X and Y")
     ap.add_argument("--shape", default=None, help="SpMM shape batch,wpg,jper,mode (default: library default)")
     ap.add_argument("--launches", type=int, default=3)
     ap.add_argument("--copies", type=int, default=3)
     a = ap.parse_args(argv)
     dt = oh.OMX_F64 if a.dtype == "f64" else oh.OMX_F32
-    prec = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3}[a.prec or a.dtype]
+    prec = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3,
+            "bf16x6": oh.PREC_BF16X6}[a.prec or a.dtype]
     nb = 4 * a.L ** 3
     no = nb // 2
     ld = no + a.pad

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libommhip.so")
 
 OMX_F64, OMX_F32, OMX_BF16, OMX_C128, OMX_C64 = 0, 1, 2, 3, 4
-PREC_F64, PREC_F32, PREC_BF16, PREC_BF16X3 = 0, 1, 2, 3
+PREC_F64, PREC_F32, PREC_BF16, PREC_BF16X3, PREC_BF16X6 = 0, 1, 2, 3, 4
 K_SPMM, K_GEMM, K_COEFF, K_ELEM, K_DOT, K_GEMM_PAIR = 0, 1, 2, 3, 4, 5
 
 _NP = {OMX_F64: np.float64, OMX_F32: np.float32, OMX_C128: np.complex128, OMX_C64: np.complex64}

@@ -87,8 +87,24 @@ def _split3(x):
     return hi, lo
 
 
+def _terms3(x):
+    """The three BF16 terms of values in their storage precision (float32: exact split; float64:
+    each difference exact in float64): hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)."""
+    if x.dtype == np.float32:
+        hi = _bf16_of_f32(x)
+        r1 = (x - hi.astype(np.float32)).astype(np.float32)
+        mid = _bf16_of_f32(r1)
+        lo = _bf16_of_f32((r1 - mid.astype(np.float32)).astype(np.float32))
+    else:
+        hi = _bf16(x)
+        r1 = x - hi
+        mid = _bf16(r1)
+        lo = _bf16(r1 - mid)
+    return hi, mid, lo
+
+
 @pytest.mark.parametrize("opA,opB", OPS)
-@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
+@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3, oh.PREC_BF16X6])
 @pytest.mark.parametrize("dt", [oh.OMX_F32, oh.OMX_F64])
 @pytest.mark.parametrize("shape", [(300, 260, 512), (301, 259, 576)])
 def test_gemm_bf16_panels(ctx, opA, opB, prec, dt, shape):
@@ -112,6 +128,13 @@ def test_gemm_bf16_panels(ctx, opA, opB, prec, dt, shape):
     if prec == oh.PREC_BF16:
         ref = 0.5 * op(rnd(A), opA) @ op(rnd(B), opB) - C0
         assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
+    elif prec == oh.PREC_BF16X6:
+        # the six term products of order >= 2^-16; FP32-grade against the exact product
+        (A1, A2, A3), (B1, B2, B3) = _terms3(A), _terms3(B)
+        pairs = [(A1, B3), (A3, B1), (A2, B2), (A1, B2), (A2, B1), (A1, B1)]
+        ref = 0.5 * sum(op(a, opA) @ op(b, opB) for a, b in pairs) - C0
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(k)
+        assert np.max(np.abs(got - exact)) <= 1e-5 * np.sqrt(k)
     else:
         if dt == oh.OMX_F32:
             Ah, Al = _split3(A)
@@ -125,7 +148,7 @@ def test_gemm_bf16_panels(ctx, opA, opB, prec, dt, shape):
         assert np.max(np.abs(got - exact)) < np.max(np.abs(bf - exact)) / 50
 
 
-@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3])
+@pytest.mark.parametrize("prec", [oh.PREC_BF16, oh.PREC_BF16X3, oh.PREC_BF16X6])
 @pytest.mark.parametrize("n,m", [(256, 512), (200, 300)])
 def test_calc_G_bf16_panels(ctx, prec, n, m):
     # the gradient pair on FP32 storage: n % 64 == 0 runs on the panels, n = 200 falls back (BF16
@@ -143,6 +166,8 @@ def test_calc_G_bf16_panels(ctx, prec, n, m):
         b = _bf16_of_f32
         ref = 4 * f(HC) - 2 * b(SW) @ b(HC) - 2 * b(HW) @ b(SC)
         assert np.max(np.abs(got - ref)) <= 1e-5 * np.sqrt(n) * 8
+    elif prec == oh.PREC_BF16X6:
+        assert np.max(np.abs(got - exact)) <= 2e-5 * np.sqrt(n)  # FP32-grade
     else:
         assert np.max(np.abs(got - exact)) <= 2e-4 * np.sqrt(n)
 

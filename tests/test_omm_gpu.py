@@ -152,12 +152,14 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
         M.free()
 
 
-def test_mixed_precision_bf16x3_gradient():
+@pytest.mark.parametrize("state", ["fp32", "bf16x6"])
+def test_mixed_precision_bf16x3_gradient(state):
     """'hmx' with the BF16x3 gradient (hi*hi + hi*lo + lo*hi of the FP32 operands on BF16 MFMA) at
     the 8x8x8 lattice, where the Gram and gradient shapes run on the k-contiguous BF16 panels.  The
     gapped parameterised lattice (onsite levels -6, -6, 6, 6) is used, so that both solves can
-    reach the exact minimum: the same bars as the FP32-gradient form."""
-    _note("mixed bf16x3")
+    reach the exact minimum: the same bars as the FP32-gradient form.  With state = "bf16x6" the
+    periodic state recomputation runs as BF16x6 panel products (FP32-grade) instead of FP32."""
+    _note("mixed bf16x3", state)
     L = 8
     nb, no = 4 * L ** 3, 2 * L ** 3
     cp, ri, h = oh.lattice_bsr_theta(L, oh.lattice_theta0())
@@ -174,12 +176,14 @@ def test_mixed_precision_bf16x3_gradient():
     Cm.upload(C0)
     F.set_option("lowp_grad_bf16x3", 1)
     F.set_option("lowp_max_iter", 3000)
+    F.set_option("lowp_state_bf16x6", 1 if state == "bf16x6" else 0)
     try:
         r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-10, dealloc=True, operation="hmx")
     finally:
         F.set_option("lowp_grad_bf16x3", 1)
         F.set_option("lowp_max_iter", 0)
-    print("CASE mixed bf16x3 icg", r["icg"], "lowp", r["icg_lowp"], "fp64 icg", ref["icg"], "err",
+        F.set_option("lowp_state_bf16x6", 0)
+    print("CASE mixed bf16x3", state, "icg", r["icg"], "lowp", r["icg_lowp"], "fp64 icg", ref["icg"], "err",
           abs(r["e_min"] - exact) / abs(exact), "fp64 err", abs(ref["e_min"] - exact) / abs(exact), flush=True)
     assert r["converged"] and r["icg_lowp"] > 0
     assert abs(r["e_min"] - exact) <= max(2.0 * abs(ref["e_min"] - exact), 1e-10 * abs(exact))
