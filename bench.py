@@ -283,7 +283,8 @@ def main():
                                + (" (mid-gap)" if L in ETA_MID_GAP else ""),
                    "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
         "roofline": {"bound": "mfma",
-                     "kernel": "FP64 Gram and state products (k_dgemm_nt_dl direct-to-LDS and k_gemm, v_mfma_f64_16x16x4f64, 128x128 tiles)",
+                     "kernel": "FP64 Gram and state products (k_dgemm_dl direct-to-LDS; the symmetric calc_AW Grams on their "
+                               "lower-triangle tiles; v_mfma_f64_16x16x4f64, 128x128 tiles)",
                      "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
                      "traffic": profiled_traffic(GEMM_PMC_FILE) if L == 16 else None,

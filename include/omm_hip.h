@@ -180,6 +180,14 @@ int omx_bsr_param_grad(omx_ctx ctx, omx_bsr Dp, int nparam, const int64_t *seg, 
  */
 int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
              int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+/* C := alpha*A*B^T for a product the caller knows to be symmetric, n x n with A and B n x k ('n',
+ * 't'): omm()'s calc_AW Gram AW = (C*H)*C^T with H symmetric (libOMM/src/omm_ops.F90:44-64, the
+ * mm_multiply(AC,'n',C,'c',AW) of :62 with beta = 0).  Only the lower-triangle tiles are computed
+ * where the kernel allows it, and the lower triangle is mirrored, so C is exactly symmetric.
+ * Real element types (OMX_F64, OMX_F32); prec as in omx_gemm.  The reference computes the full
+ * product with dgemm; this entry point is the symmetric-result form of it. */
+int omx_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, const void *A, int64_t lda, const void *B,
+                 int64_t ldb, void *C, int64_t ldc);
 /* SpMM launch shape of later real products on this context: blocks whose loads are issued
  * together (batch 1, 4 or 8), waves stacked in rows per workgroup (wpg 1, 2 or 4), block columns
  * walked per workgroup (jper >= 1) and XCD map (mode 0: each XCD owns a contiguous eighth of the

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_gemm(int m, int n, int k, 
                                                  const TS *__restrict__ A2, int64_t lda, const TS *__restrict__ B1,
                                                  const TS *__restrict__ B2, int64_t ldb, double beta0,
                                                  const TS *__restrict__ C0, double beta, TS *__restrict__ C,
-                                                 int64_t ldc, int k_per_split, int vec_a, int vec_b,
+                                                 int64_t ldc, int k_per_split, int vec_a, int vec_b, int sym,
                                                  typename Prec<PREC>::acc_t *__restrict__ partial,
                                                  const int *__restrict__ mask) {
     typedef typename Prec<PREC>::lds_t LT;
@@ -293,7 +293,10 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_gemm(int m, int n, int k, 
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
     const int tiles_m = (m + CF::BM - 1) / CF::BM, tiles_n = (n + CF::BN - 1) / CF::BN;
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if (sym)
+        tile_of_lower(blockIdx.x, tiles_m, tm, tn);
+    else
+        tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int i0 = tm * CF::BM, j0 = tn * CF::BN;
 
     // K range of this split over the virtual contraction [0, NPAIR*k).
@@ -391,6 +394,14 @@ __global__ void k_splitk_reduce(int m, int n, int splits, const AT *__restrict__
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// Output area (elements) of the lower-triangle tiles of an n x n product with bm x bm tiles: the
+// flop count of a symmetric product is 2*k times this (the MFMA work actually issued).
+static double sym_area(int n, int bm) {
+    double area = 0.0;
+    for (int a = 0; a * bm < n; ++a) area += (double)std::min(bm, n - a * bm) * std::min(n, (a + 1) * bm);
+    return area;
+}
+
 // ---------------------------------------------------------------------------------------------
 // FP64 products with direct-to-LDS staging (configuration 7): C := alpha*sum_p A_p*op(B_p) + beta0*C0
 // + beta*C over NPAIR products back to back along K (the Gram: NPAIR = 1, op 't'; calc_G's pair:
@@ -418,7 +429,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
                                                                const double *__restrict__ B1,
                                                                const double *__restrict__ B2, int64_t ldb,
                                                                double beta0, const double *__restrict__ C0,
-                                                               double beta, double *__restrict__ C, int64_t ldc) {
+                                                               double beta, double *__restrict__ C, int64_t ldc,
+                                                               int sym) {
     static_assert(CF::BM == 128 && CF::BN == 128, "one 1 KB row per wave load");
     constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, INS = 2 * BK, LPW = INS / NW;
     constexpr int STAGE = INS * 128;  // doubles: A rows (BK x 128), then B (BK x 128 or 128 x BK)
@@ -427,7 +439,10 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
     int tm, tn;
-    tile_of(blockIdx.x, m / 128, n / 128, tm, tn);
+    if (sym)
+        tile_of_lower(blockIdx.x, m / 128, tm, tn);
+    else
+        tile_of(blockIdx.x, m / 128, n / 128, tm, tn);
     const int i0 = tm * 128, j0 = tn * 128;
     d4v acc[CF::FBM][CF::FBN];
 #pragma unroll
@@ -507,7 +522,8 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
                         int64_t lda, const double *B1, const double *B2, int64_t ldb, double beta0, const double *C0,
                         double beta, double *C, int64_t ldc) {
     typedef Cfg5 CF;
-    const int tiles = (m / 128) * (n / 128);
+    const bool sym = ctx->gemm_sym && m == n && NPAIR == 1;
+    const int tiles = sym ? (m / 128) * (m / 128 + 1) / 2 : (m / 128) * (n / 128);
     if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F64>::BK || lda % 2 || ldb % 2 ||
         !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || ctx->launch_mask ||
         tiles < 2 * ctx->num_cus)
@@ -515,11 +531,11 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     const int cls = NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM;
     hipEvent_t ev;
     prof_begin(ctx, cls, &ev);
-    prof_work(ctx, cls, 2.0 * m * n * (double)NPAIR * k,
+    prof_work(ctx, cls, 2.0 * (sym ? sym_area(m, 128) : (double)m * n) * (double)NPAIR * k,
               sizeof(double) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                                 (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
     hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1, A2,
-                       lda, B1, B2, ldb, beta0, C0, beta, C, ldc);
+                       lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym);
     prof_end(ctx, cls, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
@@ -554,14 +570,19 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     }
     const int vec_a = (lda % VEC == 0) && aligned16(A1) && aligned16(A2);
     const int vec_b = (ldb % VEC == 0) && aligned16(B1) && aligned16(B2);
+    // products known to be symmetric (omx_gemm_sym): lower-triangle tiles only, when one split
+    // covers K and the tiles are square
+    const int tiles_m = (m + CF::BM - 1) / CF::BM;
+    const bool sym = ctx->gemm_sym && m == n && CF::BM == CF::BN && splits == 1;
+    const int grid = sym ? tiles_m * (tiles_m + 1) / 2 : tiles;
     hipEvent_t ev;
     prof_begin(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
-    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)ktot,
+    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * (sym ? sym_area(m, CF::BM) : (double)m * n) * (double)ktot,
               sizeof(TS) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                             (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
-    hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR, CF>), dim3(tiles, 1, splits), dim3(CF::NT), 0, ctx->stream,
-                       m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, vec_a, vec_b, partial,
-                       ctx->launch_mask);
+    hipLaunchKernelGGL((k_gemm<TS, PREC, TRA, TRB, NPAIR, CF>), dim3(grid, 1, splits), dim3(CF::NT), 0, ctx->stream,
+                       m, n, k, alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, kps, vec_a, vec_b, (int)sym,
+                       partial, ctx->launch_mask);
     if (splits > 1) {
         int64_t total = (int64_t)m * n;
         int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -900,7 +921,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
                                                      const __bf16 *__restrict__ At, int64_t lda,
                                                      const __bf16 *__restrict__ Bt, int64_t ldb, double beta0,
                                                      const TO *__restrict__ C0, double beta, TO *__restrict__ C,
-                                                     int64_t ldc) {
+                                                     int64_t ldc, int sym) {
     constexpr int BK = Prec<OMX_PREC_BF16>::BK;
     constexpr int NW = CF::WGM * CF::WGN;
     constexpr int INS = (CF::BM + CF::BN) / 8;  // wave loads per slice, 8 rows each
@@ -913,7 +934,10 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
     const int tiles_m = (m + CF::BM - 1) / CF::BM, tiles_n = (n + CF::BN - 1) / CF::BN;
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if (sym)
+        tile_of_lower(blockIdx.x, tiles_m, tm, tn);
+    else
+        tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int i0 = tm * CF::BM, j0 = tn * CF::BN;
 
     f4v acc[CF::FBM][CF::FBN];
@@ -1029,9 +1053,11 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     if (rc) return rc;
     __bf16 *At = (__bf16 *)ctx->ws;
     __bf16 *Bt = (__bf16 *)((char *)ctx->ws + ((bytes_a + 255) / 256) * 256);
+    const bool sym = ctx->gemm_sym && m == n && nterm == 1;  // lower-triangle tiles of a symmetric product
     hipEvent_t ev;
     prof_begin(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
-    prof_work(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, 2.0 * m * n * (double)k * nterm,
+    prof_work(ctx, nterm == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM,
+              2.0 * (sym ? sym_area(m, Cfg4::BM) : (double)m * n) * (double)k * nterm,
               sizeof(TS) * (nterm * ((double)m * k + (double)k * n) + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0))));
     // Rows of At are m panels of length K (ld K), rows of Bt n of length K; product term tt takes
     // the K range [tt*reps*k, (tt+1)*reps*k), block b of it holding the A and B terms of pair b.
@@ -1047,9 +1073,10 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     }
     if (!rc) {
         typedef Cfg4 CF;
-        const int tiles = ((m + CF::BM - 1) / CF::BM) * ((n + CF::BN - 1) / CF::BN);
+        const int tm = (m + CF::BM - 1) / CF::BM;
+        const int tiles = sym ? tm * (tm + 1) / 2 : tm * ((n + CF::BN - 1) / CF::BN);
         hipLaunchKernelGGL((k_bgemm_tn<CF, TS>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, (int)K, alpha, At, K,
-                           Bt, K, beta0, C0, beta, C, ldc);
+                           Bt, K, beta0, C0, beta, C, ldc, (int)sym);
         rc = hipGetLastError() == hipSuccess ? OMX_OK : OMX_ERR_HIP;
         if (rc) set_error("bf16 panel GEMM launch failed");
     }
@@ -1155,6 +1182,48 @@ int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double 
     return OMX_ERR_UNSUPPORTED;
 }
 
+// C(i, j) := C(j, i) for i < j (n x n, ld ldc): copies the lower triangle onto the upper one.  One
+// 64 x 64 tile pair per workgroup (the upper tile and the lower one it mirrors, the diagonal tiles
+// on themselves), through an LDS transpose so that reads and writes run along columns.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_sym_mirror(int n, TS *__restrict__ C, int64_t ldc) {
+    constexpr int T = 64;
+    __shared__ TS tile[T][T + 1];
+    int a, b;  // lower tile (a, b), a >= b; it fills the upper tile (b, a)
+    tile_of_lower(blockIdx.x, (n + T - 1) / T, a, b);
+    const int r0 = a * T, c0 = b * T;  // lower tile origin (rows r0.., columns c0..)
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+        const int r = q % T, c = q / T;  // r fastest: along a column
+        if (r0 + r < n && c0 + c < n) tile[c][r] = C[(int64_t)(r0 + r) + (int64_t)(c0 + c) * ldc];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+        const int i = q % T, j = q / T;  // upper element (c0 + i, r0 + j), i fastest: along a column
+        const int gi = c0 + i, gj = r0 + j;
+        if (gi < gj && gj < n) C[(int64_t)gi + (int64_t)gj * ldc] = tile[i][j];
+    }
+}
+
+// C := alpha * A * B^T for a product known to be symmetric (n x n; A, B n x k, 'n' and 't'), as
+// AW = (C*H)*C^T with H symmetric in omm()'s calc_AW (omm_ops.F90:44-64).  The kernels compute the
+// lower-triangle tiles only where they can (ctx->gemm_sym), otherwise the full product; the lower
+// triangle is then mirrored, so C is exactly symmetric.
+int launch_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, const void *A, int64_t lda,
+                    const void *B, int64_t ldb, void *C, int64_t ldc) {
+    ctx->gemm_sym = true;
+    int rc = launch_gemm(ctx, prec, dt, false, true, n, n, k, alpha, A, lda, B, ldb, 0.0, C, ldc);
+    ctx->gemm_sym = false;
+    if (rc) return rc;
+    const int nt = (n + 63) / 64;
+    const unsigned grid = (unsigned)(nt * (nt + 1) / 2);
+    if (dt == OMX_F64)
+        hipLaunchKernelGGL(k_sym_mirror<double>, dim3(grid), dim3(256), 0, ctx->stream, n, (double *)C, ldc);
+    else
+        hipLaunchKernelGGL(k_sym_mirror<float>, dim3(grid), dim3(256), 0, ctx->stream, n, (float *)C, ldc);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
 }  // namespace omx
 
 using namespace omx;
@@ -1173,6 +1242,19 @@ extern "C" int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m
     if (dt == OMX_C128 || dt == OMX_C64)  // complex: FP64 / FP32 MFMA by element type
         return launch_zgemm(ctx, dt, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     return launch_gemm(ctx, prec, dt, trA, trB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int omx_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, const void *A, int64_t lda,
+                            const void *B, int64_t ldb, void *C, int64_t ldc) {
+    OMX_REQUIRE(ctx, "omx_gemm_sym: null context");
+    OMX_REQUIRE(n >= 0 && k >= 0, "omx_gemm_sym: negative dimension");
+    OMX_REQUIRE(lda >= std::max(1, n) && ldb >= std::max(1, n) && ldc >= std::max(1, n),
+                "omx_gemm_sym: leading dimension too small");
+    OMX_REQUIRE(dt == OMX_F64 || dt == OMX_F32, "omx_gemm_sym: real element types only (dt=%d)", dt);
+    if (n == 0) return OMX_OK;
+    if (k == 0) return omx_scale(ctx, dt, n, n, 0.0, C, ldc);
+    OMX_REQUIRE(A && B && C, "omx_gemm_sym: null operand");
+    return launch_gemm_sym(ctx, prec, dt, n, k, alpha, A, lda, B, ldb, C, ldc);
 }
 
 extern "C" int omx_gemm_set_tile(omx_ctx ctx, int cfg) {

@@ -74,6 +74,7 @@ struct omx_ctx_s {
     double prof_bytes[OMX_K_COUNT] = {0};  // algorithmic (minimum) HBM bytes of the timed launches
     int num_cus = 256;
     int gemm_cfg = -1;    // forced GEMM tile configuration (-1: chosen by shape)
+    bool gemm_sym = false;  // set by omx_gemm_sym around its product: lower-triangle tiles only
     omx_spmm_shape_s spmm_shape;  // forced SpMM launch shape (when spmm_forced)
     bool spmm_forced = false;
     // Device flag for the grouped solver: while set, GEMM, SpMM and reduction launches do nothing
@@ -183,6 +184,20 @@ __device__ inline void tile_of(int w, int tiles_m, int tiles_n, int &tm, int &tn
     int in = t % (GM * tiles_n);
     tm = first_m + in % gm;
     tn = in / gm;
+}
+
+// Lower-triangle tiles (tm >= tn) of an nt x nt tile grid, nt(nt+1)/2 of them, dealt over the
+// XCDs as tile_of does and enumerated by tile rows (products known to be symmetric compute only
+// these and mirror them).
+__device__ inline void tile_of_lower(int w, int nt, int &tm, int &tn) {
+    const int T = nt * (nt + 1) / 2;
+    int t = w;
+    if ((T & 7) == 0) t = (w & 7) * (T >> 3) + (w >> 3);
+    int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (r > 0 && r * (r + 1) / 2 > t) --r;
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    tm = r;
+    tn = t - r * (r + 1) / 2;
 }
 
 // Launch helpers implemented in the .hip files.

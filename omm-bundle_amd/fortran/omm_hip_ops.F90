@@ -33,6 +33,8 @@ module omm_hip_ops
   integer, save :: opt_lowp_stall = 60       ! low-precision phase ends after this many steps without a
                                              ! new lowest energy (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
+  integer, save :: opt_sym_gram = 1          ! symmetric calc_AW Grams (omx_gemm_sym): 0 never, 1 FP64
+                                             ! products ('hip'), 2 every real device product
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
 
@@ -78,11 +80,22 @@ contains
   end subroutine calc_G
 
   !> AC := C*A; AW := AC*C^T  (omm_ops.F90:44-64)
+  !> AW = C*A*C^T is symmetric (A = H or S is): on real device storage only its lower-triangle tiles
+  !> are computed and mirrored (omx_gemm_sym).
   subroutine calc_AW(A, C, AW, AC, m_operation)
     character(3), intent(in) :: m_operation
     type(matrix), intent(in) :: A, C
     type(matrix), intent(inout) :: AC, AW
+    integer(c_int) :: rc
     call mm_multiply(C, 'n', A, 'n', AC, 1.0_dp, 0.0_dp, m_operation)
+    if (opt_fused .and. on_device(AC) .and. on_device(C) .and. on_device(AW) .and. AC%dtype == AW%dtype &
+        .and. C%dtype == AW%dtype .and. (AW%dtype == OMX_F64 .or. AW%dtype == OMX_F32) .and. &
+        (opt_sym_gram == 2 .or. (opt_sym_gram == 1 .and. m_operation == 'hip'))) then
+      rc = omx_gemm_sym(ms_hip_ctx(), ms_label_prec(m_operation), AW%dtype, int(AW%dim1, c_int), &
+                        int(C%dim2, c_int), 1.0_dp, AC%dptr, AC%ld, C%dptr, C%ld, AW%dptr, AW%ld)
+      if (rc /= 0) call die('calc_AW: '//omx_error_string())
+      return
+    end if
     call mm_multiply(AC, 'n', C, 'c', AW, 1.0_dp, 0.0_dp, m_operation)
   end subroutine calc_AW
 
@@ -370,6 +383,8 @@ contains
       opt_state_b3 = (value /= 0.0_dp)
     case ('lowp_state_bf16x6')
       opt_state_b6 = (value /= 0.0_dp)
+    case ('sym_gram')
+      opt_sym_gram = max(0, min(2, nint(value)))
     case ('lowp_stall')
       opt_lowp_stall = max(0, nint(value))
     case default

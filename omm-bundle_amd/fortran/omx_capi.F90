@@ -181,6 +181,14 @@ module omx_capi
       real(c_double), value :: alpha, beta
       integer(c_int64_t), value :: lda, ldb, ldc
     end function
+    integer(c_int) function omx_gemm_sym(ctx, prec, dt, n, k, alpha, A, lda, B, ldb, C, ldc) &
+        bind(c, name='omx_gemm_sym')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx, A, B, C
+      integer(c_int), value :: prec, dt, n, k
+      real(c_double), value :: alpha
+      integer(c_int64_t), value :: lda, ldb, ldc
+    end function
     integer(c_int) function omx_add(ctx, dt, opA, m, n, alpha, A, lda, beta, C, ldc) bind(c, name='omx_add')
       import :: c_int, c_ptr, c_int64_t, c_double, c_char
       type(c_ptr), value :: ctx, A, C

@@ -68,6 +68,7 @@ SIGNATURES = {
     "omx_bsr_get_element": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "omx_spmm": (_i, [_vp, _i, _i, _i, _d, _vp, _i64, _vp, _d, _vp, _i64]),
     "omx_gemm": (_i, [_vp, _i, _i, _c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
+    "omx_gemm_sym": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _vp, _i64]),
     "omx_gemm_set_tile": (_i, [_vp, _i]),
     "omx_spmm_set_tile": (_i, [_vp, _i, _i, _i, _i]),
     "omx_bsr_set_walk": (_i, [_vp, _vp]),
@@ -474,6 +475,12 @@ def gemm(ctx, opA, opB, A: DMat, B: DMat, Cm: DMat, alpha=1.0, beta=0.0, prec=PR
     n = B.rows if opB in "tTcC" else B.cols
     check(lib().omx_gemm(ctx.h, prec, A.dt, opA.encode(), opB.encode(), m, n, k, alpha, A.p, A.ld, B.p, B.ld, beta,
                          Cm.p, Cm.ld), "omx_gemm")
+
+
+def gemm_sym(ctx, A: DMat, B: DMat, Cm: DMat, alpha=1.0, prec=PREC_F64):
+    """Cm := alpha*A*B^T for a product known to be symmetric (lower-triangle tiles, mirrored)."""
+    check(lib().omx_gemm_sym(ctx.h, prec, A.dt, A.rows, A.cols, alpha, A.p, A.ld, B.p, B.ld, Cm.p, Cm.ld),
+          "omx_gemm_sym")
 
 
 def add(ctx, opA, A: DMat, Cm: DMat, alpha, beta):

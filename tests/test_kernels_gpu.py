@@ -366,6 +366,36 @@ def test_gemm_f64_gram_direct_lds(shape):
     assert np.max(np.abs(out[0] - ref)) <= 1e-12 * np.sqrt(k) * 4
 
 
+@pytest.mark.parametrize("dt,prec,no,nb", [(oh.OMX_F64, oh.PREC_F64, 4096, 1024),
+                                           (oh.OMX_F64, oh.PREC_F64, 300, 512),
+                                           (oh.OMX_F64, oh.PREC_F64, 4096 + 64, 1024),
+                                           (oh.OMX_F32, oh.PREC_F32, 1024, 512),
+                                           (oh.OMX_F32, oh.PREC_BF16, 1024, 512),
+                                           (oh.OMX_F32, oh.PREC_BF16X3, 300, 512)])
+def test_gemm_sym(dt, prec, no, nb):
+    """omx_gemm_sym on AW = (C H) C^T with H symmetric: the lower triangle is bitwise the full
+    product's (same kernel per element), the upper triangle its mirror.  Shapes cover the
+    direct-to-LDS FP64 Gram (lower-triangle tiles), the staged FP64 kernel, FP32 and the BF16
+    panels; the odd sizes take partial edge tiles."""
+    _note("gemm_sym", dt, prec, no, nb)
+    npdt = np.float64 if dt == oh.OMX_F64 else np.float32
+    C = rng.standard_normal((no, nb)).astype(npdt)
+    Hm = rng.standard_normal((nb, nb))
+    Hm = (Hm + Hm.T).astype(npdt)
+    AC = (C.astype(np.float64) @ Hm.astype(np.float64)).astype(npdt)
+    with oh.Ctx(0) as c:
+        dAC, dC = oh.DMat.from_host(c, AC, dt), oh.DMat.from_host(c, C, dt)
+        full, sym = oh.DMat(c, no, no, dt), oh.DMat(c, no, no, dt)
+        oh.gemm(c, "n", "t", dAC, dC, full, prec=prec)
+        oh.gemm_sym(c, dAC, dC, sym, prec=prec)
+        F_, S_ = full.download(), sym.download()
+        for M in (dAC, dC, full, sym):
+            M.free()
+    il = np.tril_indices(no)
+    assert np.array_equal(S_[il], F_[il])
+    assert np.array_equal(S_, S_.T)
+
+
 @pytest.mark.parametrize("no,nb", [(1024, 8192), (1024 + 64, 8192), (256, 512)])
 def test_calc_G_f64_direct_lds(no, nb):
     """FP64 calc_G pair G = 4HC - 2 SW HC - 2 HW SC: configuration 7 (operands loaded straight into
