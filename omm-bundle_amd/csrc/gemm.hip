@@ -430,7 +430,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
                                                                const double *__restrict__ B2, int64_t ldb,
                                                                double beta0, const double *__restrict__ C0,
                                                                double beta, double *__restrict__ C, int64_t ldc,
-                                                               int sym) {
+                                                               int sym, int tbase, int nsplit,
+                                                               double *__restrict__ partial) {
     static_assert(CF::BM == 128 && CF::BN == 128, "one 1 KB row per wave load");
     constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, INS = 2 * BK, LPW = INS / NW;
     constexpr int STAGE = INS * 128;  // doubles: A rows (BK x 128), then B (BK x 128 or 128 x BK)
@@ -438,20 +439,23 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
     __shared__ double smem[2 * STAGE];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave / CF::WGN, wn = wave % CF::WGN;
+    // tile w of the launch's tile order and, with nsplit > 1, the part z of K it takes
+    const int w = tbase + (int)blockIdx.x / nsplit, z = (int)blockIdx.x % nsplit;
     int tm, tn;
     if (sym)
-        tile_of_lower(blockIdx.x, m / 128, tm, tn);
+        tile_of_lower(w, m / 128, tm, tn);
     else
-        tile_of(blockIdx.x, m / 128, n / 128, tm, tn);
+        tile_of(w, m / 128, n / 128, tm, tn);
     const int i0 = tm * 128, j0 = tn * 128;
     d4v acc[CF::FBM][CF::FBN];
 #pragma unroll
     for (int a = 0; a < CF::FBM; ++a)
 #pragma unroll
         for (int b = 0; b < CF::FBN; ++b) acc[a][b] = d4v{0, 0, 0, 0};
+    const int kpart = NPAIR * k / nsplit, s0 = z * (kpart / BK);  // this part's first slice
     auto issue = [&](int s) {
         double *st = smem + (s & 1) * STAGE;
-        int kl = s * BK;
+        int kl = (s0 + s) * BK;
         const double *A = A1, *B = B1;
         if (NPAIR == 2 && kl >= k) {
             A = A2;
@@ -475,7 +479,7 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
         }
     };
     const int li = lane & 15, lk = lane >> 4;
-    const int ns = NPAIR * k / BK;
+    const int ns = kpart / BK;
     issue(0);
     for (int s = 0; s < ns; ++s) {
         __builtin_amdgcn_s_waitcnt(waitcnt_vmcnt(0));  // this wave's rows of slice s have landed
@@ -500,6 +504,18 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
         }
     }
     const int lq = lane >> 4;
+    if (partial) {  // a K part of a split tile: the raw sums, reduced in part order later
+        double *pt = partial + (int64_t)blockIdx.x * (128 * 128);
+#pragma unroll
+        for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < CF::FBN; ++fn)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    pt[(wm * CF::WM + fm * 16 + Prec<OMX_PREC_F64>::out_row(lq, v)) +
+                       (wn * CF::WN + fn * 16 + li) * 128] = acc[fm][fn][v];
+        return;
+    }
 #pragma unroll
     for (int fm = 0; fm < CF::FBM; ++fm)
 #pragma unroll
@@ -516,7 +532,31 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
         }
 }
 
+// C tile := alpha * (sum of its nsplit K parts, in part order) + beta0*C0 + beta*C for the split
+// tiles tbase.. of a symmetric product's lower-triangle order (blockIdx.x: tile, blockIdx.y: a
+// 1024-element slice of it).
+__global__ __launch_bounds__(256) void k_dl_split_reduce(int n, int tbase, int nsplit, const double *__restrict__ partial,
+                                                         double alpha, double beta0, const double *__restrict__ C0,
+                                                         double beta, double *__restrict__ C, int64_t ldc) {
+    int tm, tn;
+    tile_of_lower(tbase + (int)blockIdx.x, n / 128, tm, tn);
+    const double *pt = partial + (int64_t)blockIdx.x * nsplit * (128 * 128);
+    const int e0 = blockIdx.y * 1024;
+    for (int e = e0 + threadIdx.x; e < e0 + 1024; e += 256) {
+        double sum = 0.0;
+        for (int zz = 0; zz < nsplit; ++zz) sum += pt[(int64_t)zz * (128 * 128) + e];
+        const int64_t off = (int64_t)(tm * 128 + e % 128) + (int64_t)(tn * 128 + e / 128) * ldc;
+        double r = alpha * sum;
+        if (C0) r += beta0 * C0[off];
+        if (beta != 0.0) r += beta * C[off];
+        C[off] = r;
+    }
+}
+
 // Launches configuration 7 when the product qualifies; returns OMX_ERR_UNSUPPORTED otherwise.
+// A symmetric product's tiles that do not fill a whole round of resident workgroups (two per CU)
+// are split along K over up to 16 workgroups each and reduced in part order, so the round that
+// would hold only them costs a fraction of a tile.
 template <bool TRB, int NPAIR>
 static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const double *A1, const double *A2,
                         int64_t lda, const double *B1, const double *B2, int64_t ldb, double beta0, const double *C0,
@@ -534,8 +574,27 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     prof_work(ctx, cls, 2.0 * (sym ? sym_area(m, 128) : (double)m * n) * (double)NPAIR * k,
               sizeof(double) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                                 (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
-    hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1, A2,
-                       lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym);
+    const int slots = 2 * ctx->num_cus;
+    const int whole = sym ? tiles / slots * slots : tiles, rest = tiles - whole;
+    int nsplit = 1;
+    if (rest > 0) {
+        nsplit = std::max(1, std::min(16, slots / rest));
+        while (nsplit > 1 && (k / Prec<OMX_PREC_F64>::BK) % nsplit) --nsplit;
+    }
+    if (rest > 0 && nsplit > 1) {
+        int rc = ensure_ws(ctx, sizeof(double) * (size_t)rest * nsplit * 128 * 128);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(whole), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1,
+                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr);
+        hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(rest * nsplit), dim3(CF::NT), 0, ctx->stream, m, n, k,
+                           alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, whole, nsplit,
+                           (double *)ctx->ws);
+        hipLaunchKernelGGL(k_dl_split_reduce, dim3(rest, 128 * 128 / 1024), dim3(256), 0, ctx->stream, m, whole, nsplit,
+                           (const double *)ctx->ws, alpha, beta0, C0, beta, C, ldc);
+    } else {
+        hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1,
+                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr);
+    }
     prof_end(ctx, cls, ev);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;

@@ -373,10 +373,12 @@ def test_gemm_f64_gram_direct_lds(shape):
                                            (oh.OMX_F32, oh.PREC_BF16, 1024, 512),
                                            (oh.OMX_F32, oh.PREC_BF16X3, 300, 512)])
 def test_gemm_sym(dt, prec, no, nb):
-    """omx_gemm_sym on AW = (C H) C^T with H symmetric: the lower triangle is bitwise the full
-    product's (same kernel per element), the upper triangle its mirror.  Shapes cover the
-    direct-to-LDS FP64 Gram (lower-triangle tiles), the staged FP64 kernel, FP32 and the BF16
-    panels; the odd sizes take partial edge tiles."""
+    """omx_gemm_sym on AW = (C H) C^T with H symmetric: the upper triangle is the mirror of the lower
+    one, which is bitwise the full product's (same kernel per element), except on the FP64
+    direct-to-LDS path, whose tiles beyond the last whole round of workgroups are split along K
+    (rounding-level differences, 1e-12 of the product's scale).  Shapes cover that path (4096: 528
+    tiles, 16 of them split), the staged FP64 kernel, FP32 and the BF16 panels; the odd sizes take
+    partial edge tiles."""
     _note("gemm_sym", dt, prec, no, nb)
     npdt = np.float64 if dt == oh.OMX_F64 else np.float32
     C = rng.standard_normal((no, nb)).astype(npdt)
@@ -392,7 +394,10 @@ def test_gemm_sym(dt, prec, no, nb):
         for M in (dAC, dC, full, sym):
             M.free()
     il = np.tril_indices(no)
-    assert np.array_equal(S_[il], F_[il])
+    if dt == oh.OMX_F64 and no % 128 == 0:
+        assert np.max(np.abs(S_[il] - F_[il])) <= 1e-12 * np.max(np.abs(F_))
+    else:
+        assert np.array_equal(S_[il], F_[il])
     assert np.array_equal(S_, S_.T)
 
 
