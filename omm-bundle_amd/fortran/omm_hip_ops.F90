@@ -34,7 +34,8 @@ module omm_hip_ops
                                              ! new lowest energy (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
   integer, save :: opt_sym_gram = 1          ! symmetric calc_AW Grams (omx_gemm_sym): 0 never, 1 FP64
-                                             ! products ('hip'), 2 every real device product
+                                             ! products ('hip'), 2 every real device product, 3 FP64
+                                             ! and FP32 ('h32') products
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
 
@@ -90,7 +91,8 @@ contains
     call mm_multiply(C, 'n', A, 'n', AC, 1.0_dp, 0.0_dp, m_operation)
     if (opt_fused .and. on_device(AC) .and. on_device(C) .and. on_device(AW) .and. AC%dtype == AW%dtype &
         .and. C%dtype == AW%dtype .and. (AW%dtype == OMX_F64 .or. AW%dtype == OMX_F32) .and. &
-        (opt_sym_gram == 2 .or. (opt_sym_gram == 1 .and. m_operation == 'hip'))) then
+        (opt_sym_gram == 2 .or. ((opt_sym_gram == 1 .or. opt_sym_gram == 3) .and. m_operation == 'hip') .or. &
+         (opt_sym_gram == 3 .and. m_operation == 'h32'))) then
       rc = omx_gemm_sym(ms_hip_ctx(), ms_label_prec(m_operation), AW%dtype, int(AW%dim1, c_int), &
                         int(C%dim2, c_int), 1.0_dp, AC%dptr, AC%ld, C%dptr, C%ld, AW%dptr, AW%ld)
       if (rc /= 0) call die('calc_AW: '//omx_error_string())
@@ -384,7 +386,7 @@ contains
     case ('lowp_state_bf16x6')
       opt_state_b6 = (value /= 0.0_dp)
     case ('sym_gram')
-      opt_sym_gram = max(0, min(2, nint(value)))
+      opt_sym_gram = max(0, min(3, nint(value)))
     case ('lowp_stall')
       opt_lowp_stall = max(0, nint(value))
     case default
