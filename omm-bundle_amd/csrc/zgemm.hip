@@ -101,7 +101,7 @@ struct ZStage {
 };
 
 template <typename R, bool TRA, bool CJA, bool TRB, bool CJB, int NPAIR>
-__global__ __launch_bounds__(ZNT) void k_zgemm(int m, int n, int k, double alpha,
+__global__ __launch_bounds__(ZNT, 2) void k_zgemm(int m, int n, int k, double alpha,
                                                const typename ZPrec<R>::cplx *__restrict__ A1,
                                                const typename ZPrec<R>::cplx *__restrict__ A2, int64_t lda,
                                                const typename ZPrec<R>::cplx *__restrict__ B1,
