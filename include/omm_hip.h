@@ -184,8 +184,10 @@ int omx_gemm(omx_ctx ctx, int prec, int dt, char opA, char opB, int m, int n, in
  * 't'): omm()'s calc_AW Gram AW = (C*H)*C^T with H symmetric (libOMM/src/omm_ops.F90:44-64, the
  * mm_multiply(AC,'n',C,'c',AW) of :62 with beta = 0).  Only the lower-triangle tiles are computed
  * where the kernel allows it, and the lower triangle is mirrored, so C is exactly symmetric.
- * Real element types (OMX_F64, OMX_F32); prec as in omx_gemm.  The reference computes the full
- * product with dgemm; this entry point is the symmetric-result form of it. */
+ * Real element types (OMX_F64, OMX_F32) with prec as in omx_gemm; complex types (OMX_C128,
+ * OMX_C64; prec ignored) compute C := alpha*A*B^H, Hermitian, with the upper triangle the
+ * conjugate of the lower.  The reference computes the full product with dgemm/zgemm; this entry
+ * point is the symmetric-result form of it. */
 int omx_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, const void *A, int64_t lda, const void *B,
                  int64_t ldb, void *C, int64_t ldc);
 /* SpMM launch shape of later real products on this context: blocks whose loads are issued

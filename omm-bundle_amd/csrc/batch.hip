@@ -356,9 +356,12 @@ static int batch_step(omx_zbatch b) {
         return rc;
     const void *src[4] = {b->HC, b->SC, b->HG, b->SG};
     void *dst[4] = {b->HWd, b->SWd, b->HWdd, b->SWdd};
+    // HG D^H = D H D^H and SG D^H are Hermitian: lower-triangle tiles and a mirror
     for (int q = 0; q < 4; ++q)
-        if ((rc = launch_zgemm_batched(ctx, dt, 'n', 'c', no, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC, 0.0,
-                                       dst[q], no, lenW, ns, act)))
+        if ((rc = q < 2 ? launch_zgemm_batched(ctx, dt, 'n', 'c', no, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC,
+                                               0.0, dst[q], no, lenW, ns, act)
+                        : launch_zgemm_herm_batched(ctx, dt, no, nb, 1.0, src[q], no, lenC, b->D, no, lenC, dst[q],
+                                                    no, lenW, ns, act)))
             return rc;
     if ((rc = launch_coeff_batched(ctx, dt, no, no, lenW, b->HW, b->SW, b->HWd, b->SWd, b->HWdd, b->SWdd, ns,
                                    b->partial, b->coeff, nullptr)))
@@ -513,11 +516,11 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
     if ((rc = launch_spmm_z_batched(ctx, b->dt, no, nb, b->nbc, b->colptr, b->rowidx, b->sval, sV, b->C, no, lenC, b->SC, no,
                                     lenC, ns, act)))
         return rc;
-    if ((rc = launch_zgemm_batched(ctx, b->dt, 'n', 'c', no, no, nb, 1.0, b->HC, no, lenC, b->C, no, lenC, 0.0, b->HW, no,
-                                   lenW, ns, act)))
+    if ((rc = launch_zgemm_herm_batched(ctx, b->dt, no, nb, 1.0, b->HC, no, lenC, b->C, no, lenC, b->HW, no, lenW, ns,
+                                        act)))
         return rc;
-    if ((rc = launch_zgemm_batched(ctx, b->dt, 'n', 'c', no, no, nb, 1.0, b->SC, no, lenC, b->C, no, lenC, 0.0, b->SW, no,
-                                   lenW, ns, act)))
+    if ((rc = launch_zgemm_herm_batched(ctx, b->dt, no, nb, 1.0, b->SC, no, lenC, b->C, no, lenC, b->SW, no, lenW, ns,
+                                        act)))
         return rc;
     if ((rc = batch_calc_G_any(b, act))) return rc;
     // the CG steps: captured once, replayed; flags read every check_every steps

@@ -1309,10 +1309,13 @@ extern "C" int omx_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double 
     OMX_REQUIRE(n >= 0 && k >= 0, "omx_gemm_sym: negative dimension");
     OMX_REQUIRE(lda >= std::max(1, n) && ldb >= std::max(1, n) && ldc >= std::max(1, n),
                 "omx_gemm_sym: leading dimension too small");
-    OMX_REQUIRE(dt == OMX_F64 || dt == OMX_F32, "omx_gemm_sym: real element types only (dt=%d)", dt);
+    OMX_REQUIRE(dt == OMX_F64 || dt == OMX_F32 || dt == OMX_C128 || dt == OMX_C64,
+                "omx_gemm_sym: unsupported element type %d", dt);
     if (n == 0) return OMX_OK;
     if (k == 0) return omx_scale(ctx, dt, n, n, 0.0, C, ldc);
     OMX_REQUIRE(A && B && C, "omx_gemm_sym: null operand");
+    if (dt == OMX_C128 || dt == OMX_C64)  // Hermitian: C = alpha A B^H
+        return launch_zgemm_herm(ctx, dt, n, k, alpha, A, lda, B, ldb, C, ldc);
     return launch_gemm_sym(ctx, prec, dt, n, k, alpha, A, lda, B, ldb, C, ldc);
 }
 

@@ -218,6 +218,13 @@ int launch_zgemm_pair(omx_ctx ctx, int dt, int m, int n, int k, double alpha, co
 int launch_zgemm_batched(omx_ctx ctx, int dt, char opA, char opB, int m, int n, int k, double alpha, const void *A,
                          int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, double beta, void *C,
                          int64_t ldc, int64_t sC, int nbatch, const int *active);
+// C := alpha * A * B^H for products known to be Hermitian (n x n): lower-triangle tiles, then the
+// conjugated mirror.  The batched form takes the strides and mask of launch_zgemm_batched.
+int launch_zgemm_herm(omx_ctx ctx, int dt, int n, int k, double alpha, const void *A, int64_t lda, const void *B,
+                      int64_t ldb, void *C, int64_t ldc);
+int launch_zgemm_herm_batched(omx_ctx ctx, int dt, int n, int k, double alpha, const void *A, int64_t lda,
+                              int64_t sA, const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC,
+                              int nbatch, const int *active);
 int launch_zgemm_pair_batched(omx_ctx ctx, int dt, int m, int n, int k, double alpha, const void *A1, const void *A2,
                               int64_t lda, int64_t sA, const void *B1, const void *B2, int64_t ldb, int64_t sB,
                               double beta0, const void *C0, double beta, void *C, int64_t ldc, int64_t sC, int nbatch,

@@ -54,6 +54,27 @@ struct ZBatch {
     const int *active = nullptr;
 };
 constexpr int ZWM = ZBM / 2, ZWN = ZBN / 2, ZFBM = ZWM / 16, ZFBN = ZWN / 16;
+static_assert(ZBN == 2 * ZBM, "the lower-triangle tile count assumes tiles twice as wide as tall");
+
+// Tiles of an n x n product that hold lower-triangle elements (row >= column): tile row tm (ZBM
+// rows) reaches column ZBM*tm + ZBM-1, so it holds tm/2 + 1 tiles of ZBN columns, and tile rows
+// 2a and 2a+1 hold a+1 each.  Ids are dealt over the XCDs as tile_of does; within a band the
+// tiles go by rows.
+static inline int lower_tiles_z(int tiles_m) {
+    int t = 0;
+    for (int r = 0; r < tiles_m; ++r) t += r / 2 + 1;
+    return t;
+}
+__device__ inline void tile_of_lower_z(int w, int T, int &tm, int &tn) {
+    int t = w;
+    if ((T & 7) == 0) t = (w & 7) * (T >> 3) + (w >> 3);
+    int a = (int)((sqrtf(4.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (a > 0 && a * (a + 1) > t) --a;
+    while ((a + 1) * (a + 2) <= t) ++a;
+    const int r = t - a * (a + 1);
+    tm = 2 * a + r / (a + 1);
+    tn = r % (a + 1);
+}
 
 // One BK slice of op(A) (BM x BK, "rows" r = i) or op(B)^T (BN x BK, r = j) into registers.
 // CONTIG_R: r is the contiguous index in memory (A 'n', B 't'/'c'); otherwise k is.
@@ -108,7 +129,8 @@ __global__ __launch_bounds__(ZNT, 2) void k_zgemm(int m, int n, int k, double al
                                                const typename ZPrec<R>::cplx *__restrict__ B2, int64_t ldb,
                                                double beta0, const typename ZPrec<R>::cplx *__restrict__ C0,
                                                double beta, typename ZPrec<R>::cplx *__restrict__ C, int64_t ldc,
-                                               int k_per_split, R *__restrict__ partial, ZBatch bat) {
+                                               int k_per_split, R *__restrict__ partial, int sym_tiles,
+                                               ZBatch bat) {
     typedef typename ZPrec<R>::accv AV;
     typedef typename ZPrec<R>::cplx C2;
     constexpr int LA = ZBK * (ZBM + ZPAD), LB = ZBK * (ZBN + ZPAD);
@@ -127,7 +149,10 @@ __global__ __launch_bounds__(ZNT, 2) void k_zgemm(int m, int n, int k, double al
     const int wm = wave >> 1, wn = wave & 1;
     const int tiles_m = (m + ZBM - 1) / ZBM, tiles_n = (n + ZBN - 1) / ZBN;
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if (sym_tiles)
+        tile_of_lower_z(blockIdx.x, sym_tiles, tm, tn);
+    else
+        tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int i0 = tm * ZBM, j0 = tn * ZBN;
     const int ktot = NPAIR * k;
     const int kb = blockIdx.z * k_per_split;
@@ -282,14 +307,28 @@ static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void 
                      const void *B1, const void *B2, int64_t ldb, double beta0, const void *C0, double beta, void *C,
                      int64_t ldc, int nbatch = 1, ZBatch bat = ZBatch(), bool allow_split = true) {
     typedef typename ZPrec<R>::cplx C2;
-    const int tiles = ((m + ZBM - 1) / ZBM) * ((n + ZBN - 1) / ZBN);
+    const int full_tiles = ((m + ZBM - 1) / ZBM) * ((n + ZBN - 1) / ZBN);
     const int ktot = NPAIR * k;
     int splits = 1;
     const int target = 2 * ctx->num_cus;
+    // products known to be Hermitian (A B^H, ctx->gemm_sym): lower-triangle tiles only.  Split along
+    // K when the tiles fill less than two workgroups per CU; the reduction also writes the upper
+    // elements no tile computed, which the caller's mirror then overwrites.
+    const int sym_tiles =
+        (ctx->gemm_sym && NPAIR == 1 && m == n && !TRA && TRB && CJB) ? lower_tiles_z((m + ZBM - 1) / ZBM) : 0;
+    const int tiles = sym_tiles ? sym_tiles : full_tiles;
     // batched launches never split K: no workspace, so the launch can be graph-captured
     if (allow_split && nbatch == 1 && tiles < target) {
         const int max_splits = std::max(1, ktot / (4 * ZBK));
         splits = std::max(1, std::min(std::min(max_splits, (target + tiles - 1) / tiles), 64));
+        if (sym_tiles) {
+            // the fewest rounds of workgroups per unit of K, up to 4 splits (272 tiles: 3 splits,
+            // two rounds of thirds, not 2 splits and a second round of 32 workgroups)
+            auto cost = [&](int q) { return (double)((tiles * q + target - 1) / target) / q; };
+            splits = 1;
+            for (int q = 2; q <= std::min(max_splits, 4); ++q)
+                if (cost(q) < cost(splits)) splits = q;
+        }
     }
     int kps = (ktot + splits - 1) / splits;
     kps = (kps + ZBK - 1) / ZBK * ZBK;
@@ -302,12 +341,13 @@ static int run_zgemm(omx_ctx ctx, int m, int n, int k, double alpha, const void 
     }
     hipEvent_t ev;
     prof_begin(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, &ev);
-    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM, nbatch * 8.0 * m * n * (double)ktot,
+    prof_work(ctx, NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM,
+              nbatch * 8.0 * m * n * (double)ktot * ((double)tiles / full_tiles),
               nbatch * 2.0 * sizeof(R) * ((double)NPAIR * ((double)m * k + (double)k * n) +
                                           (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
     hipLaunchKernelGGL((k_zgemm<R, TRA, CJA, TRB, CJB, NPAIR>), dim3(tiles, nbatch, splits), dim3(ZNT), 0, ctx->stream,
                        m, n, k, alpha, (const C2 *)A1, (const C2 *)A2, lda, (const C2 *)B1, (const C2 *)B2, ldb, beta0,
-                       (const C2 *)C0, beta, (C2 *)C, ldc, kps, partial, bat);
+                       (const C2 *)C0, beta, (C2 *)C, ldc, kps, partial, sym_tiles, bat);
     if (splits > 1) {
         const int64_t total = (int64_t)m * n;
         const int grid = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -382,6 +422,77 @@ int launch_zgemm_batched(omx_ctx ctx, int dt, char opA, char opB, int m, int n, 
     if (dt == OMX_C64)
         return zgemm_batched_r<float>(ctx, opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nbatch, bat);
     set_error("batched complex product: unsupported element type %d", dt);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+// C(i, j) := conj(C(j, i)) for i < j in each system's n x n block: copies the lower triangle of a
+// Hermitian product onto the upper one.  One 32 x 32 tile pair per workgroup (the diagonal tiles
+// on themselves), through an LDS transpose so that reads and writes run along columns.
+template <typename R>
+__global__ __launch_bounds__(256) void k_herm_mirror(int n, typename ZPrec<R>::cplx *__restrict__ C, int64_t ldc,
+                                                     ZBatch bat) {
+    typedef typename ZPrec<R>::cplx C2;
+    constexpr int T = 32;
+    __shared__ C2 tile[T][T + 1];
+    const int bsys = blockIdx.y;
+    if (bat.active && !bat.active[bsys]) return;
+    C += bsys * bat.sC;
+    int a, b;  // lower tile (a, b), a >= b; it fills the upper tile (b, a)
+    tile_of_lower(blockIdx.x, (n + T - 1) / T, a, b);
+    const int r0 = a * T, c0 = b * T;
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+        const int r = q % T, c = q / T;
+        if (r0 + r < n && c0 + c < n) tile[c][r] = C[(int64_t)(r0 + r) + (int64_t)(c0 + c) * ldc];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < T * T; q += 256) {
+        const int i = q % T, j = q / T;  // upper element (c0 + i, r0 + j)
+        const int gi = c0 + i, gj = r0 + j;
+        if (gi < gj && gj < n) {
+            C2 v = tile[i][j];
+            v.y = -v.y;
+            C[(int64_t)gi + (int64_t)gj * ldc] = v;
+        }
+    }
+}
+
+// C := alpha * A * B^H for a product known to be Hermitian (n x n; A, B n x k), as AW = (C*H)*C^H in
+// omm()'s calc_AW (omm_ops.F90:44-64) on complex storage.  The kernel computes the lower-triangle
+// tiles where it can, and the lower triangle is then mirrored, so C is exactly Hermitian.
+template <typename R>
+static int zgemm_herm_r(omx_ctx ctx, int n, int k, double alpha, const void *A, int64_t lda, const void *B,
+                        int64_t ldb, void *C, int64_t ldc, int nbatch, const ZBatch &bat, bool allow_split) {
+    ctx->gemm_sym = true;
+    int rc = run_zgemm<R, false, false, true, true, 1>(ctx, n, n, k, alpha, A, A, lda, B, B, ldb, 0.0, nullptr, 0.0,
+                                                       C, ldc, nbatch, bat, allow_split);
+    ctx->gemm_sym = false;
+    if (rc) return rc;
+    const int nt = (n + 31) / 32;
+    hipLaunchKernelGGL(k_herm_mirror<R>, dim3(nt * (nt + 1) / 2, nbatch), dim3(256), 0, ctx->stream, n,
+                       (typename ZPrec<R>::cplx *)C, ldc, bat);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
+int launch_zgemm_herm(omx_ctx ctx, int dt, int n, int k, double alpha, const void *A, int64_t lda, const void *B,
+                      int64_t ldb, void *C, int64_t ldc) {
+    if (dt == OMX_C128) return zgemm_herm_r<double>(ctx, n, k, alpha, A, lda, B, ldb, C, ldc, 1, ZBatch(), true);
+    if (dt == OMX_C64) return zgemm_herm_r<float>(ctx, n, k, alpha, A, lda, B, ldb, C, ldc, 1, ZBatch(), true);
+    set_error("Hermitian product: unsupported element type %d", dt);
+    return OMX_ERR_UNSUPPORTED;
+}
+
+int launch_zgemm_herm_batched(omx_ctx ctx, int dt, int n, int k, double alpha, const void *A, int64_t lda,
+                              int64_t sA, const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC,
+                              int nbatch, const int *active) {
+    ZBatch bat;
+    bat.sA = sA;
+    bat.sB = sB;
+    bat.sC = sC;
+    bat.active = active;
+    if (dt == OMX_C128) return zgemm_herm_r<double>(ctx, n, k, alpha, A, lda, B, ldb, C, ldc, nbatch, bat, false);
+    if (dt == OMX_C64) return zgemm_herm_r<float>(ctx, n, k, alpha, A, lda, B, ldb, C, ldc, nbatch, bat, false);
+    set_error("batched Hermitian product: unsupported element type %d", dt);
     return OMX_ERR_UNSUPPORTED;
 }
 

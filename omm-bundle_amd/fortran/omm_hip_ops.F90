@@ -34,8 +34,8 @@ module omm_hip_ops
                                              ! new lowest energy (0 = never)
   integer, save :: opt_lowp_consec = 3       ! low-precision phase: consecutive steps below lowp_tol that end it
   integer, save :: opt_sym_gram = 1          ! symmetric calc_AW Grams (omx_gemm_sym): 0 never, 1 FP64
-                                             ! products ('hip'), 2 every real device product, 3 FP64
-                                             ! and FP32 ('h32') products
+                                             ! and complex products ('hip'), 2 every device product,
+                                             ! 3 those of 1 and FP32 ('h32') products
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
 
@@ -81,8 +81,8 @@ contains
   end subroutine calc_G
 
   !> AC := C*A; AW := AC*C^T  (omm_ops.F90:44-64)
-  !> AW = C*A*C^T is symmetric (A = H or S is): on real device storage only its lower-triangle tiles
-  !> are computed and mirrored (omx_gemm_sym).
+  !> AW = C*A*C^H is symmetric or Hermitian (A = H or S is): on device storage only its
+  !> lower-triangle tiles are computed and mirrored (omx_gemm_sym).
   subroutine calc_AW(A, C, AW, AC, m_operation)
     character(3), intent(in) :: m_operation
     type(matrix), intent(in) :: A, C
@@ -90,7 +90,8 @@ contains
     integer(c_int) :: rc
     call mm_multiply(C, 'n', A, 'n', AC, 1.0_dp, 0.0_dp, m_operation)
     if (opt_fused .and. on_device(AC) .and. on_device(C) .and. on_device(AW) .and. AC%dtype == AW%dtype &
-        .and. C%dtype == AW%dtype .and. (AW%dtype == OMX_F64 .or. AW%dtype == OMX_F32) .and. &
+        .and. C%dtype == AW%dtype .and. (AW%dtype == OMX_F64 .or. AW%dtype == OMX_F32 .or. &
+        AW%dtype == OMX_C128 .or. AW%dtype == OMX_C64) .and. &
         (opt_sym_gram == 2 .or. ((opt_sym_gram == 1 .or. opt_sym_gram == 3) .and. m_operation == 'hip') .or. &
          (opt_sym_gram == 3 .and. m_operation == 'h32'))) then
       rc = omx_gemm_sym(ms_hip_ctx(), ms_label_prec(m_operation), AW%dtype, int(AW%dim1, c_int), &

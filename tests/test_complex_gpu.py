@@ -45,6 +45,36 @@ def test_zgemm(ctx, dt, tol, opA, opB, shape):
     assert np.max(np.abs(dC.download() - ref)) <= tol * np.sqrt(k) * 8
 
 
+@pytest.mark.parametrize("dt", [oh.OMX_C128, oh.OMX_C64])
+@pytest.mark.parametrize("n,k", [(70, 40), (2048 + 70, 100), (2048, 64)])
+def test_zgemm_herm(ctx, dt, n, k):
+    """omx_gemm_sym on complex storage, AW = (C H) C^H with H Hermitian: the strict upper triangle is
+    exactly the conjugate mirror of the lower one.  The lower triangle is bitwise the full 'n','c'
+    product's where both launches take the same K split (the small shape); the large shapes take
+    the lower-triangle tiles (306 of 578 and 272 of 512 tiles of 64 x 128, one with partial edge
+    tiles), fewer than two per CU, so they are split along K where the full product is not:
+    rounding-level differences, 1e-12 (complex128) and 1e-5 (complex64) of the product's scale."""
+    _note("zgemm_herm", dt, n, k)
+    C = crandn(n, k)
+    Hm = crandn(k, k)
+    Hm = Hm + Hm.conj().T
+    AC = C @ Hm
+    dAC, dC = (oh.DMat.from_host(ctx, M, dt) for M in (AC, C))
+    full, herm = oh.DMat(ctx, n, n, dt), oh.DMat(ctx, n, n, dt)
+    oh.gemm(ctx, "n", "c", dAC, dC, full)
+    oh.gemm_sym(ctx, dAC, dC, herm)
+    F_, S_ = full.download(), herm.download()
+    il, iu = np.tril_indices(n), np.triu_indices(n, 1)
+    if n < 128:
+        assert np.array_equal(S_[il], F_[il])
+    else:
+        scale = np.max(np.abs(F_))
+        assert np.max(np.abs(S_[il] - F_[il])) <= (1e-12 if dt == oh.OMX_C128 else 1e-5) * scale
+    assert np.array_equal(S_[iu], S_.T[iu].conj())
+    tol = 1e-12 if dt == oh.OMX_C128 else 2e-5
+    assert np.max(np.abs(S_ - AC @ C.conj().T)) <= tol * np.sqrt(k) * 8 * np.max(np.abs(AC))
+
+
 @pytest.mark.parametrize("dt,tol", [(oh.OMX_C128, 1e-12), (oh.OMX_C64, 2e-5)])
 def test_zcalc_G(ctx, dt, tol):
     n, m = 48, 100
