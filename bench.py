@@ -171,6 +171,8 @@ def main():
                     help="k-point mode: 64 complex k-points (4x4x4 grid) of the 10x10x10 cell, batched vs sequential")
     ap.add_argument("--kpoint-steps", type=int, default=4, help="CG steps timed per k-point in --kpoints mode")
     ap.add_argument("--kpoint-L", type=int, default=10, help="cell edge of the k-point systems (n = 4 L^3)")
+    ap.add_argument("--la", action="store_true",
+                    help="time the SDDMM and the dense factorisations of the Cholesky/preconditioned flavours")
     ap.add_argument("--group", action="store_true",
                     help="uneven-size mode: 24 real systems, edges L drawn from U{6..20}, one graph launch per step "
                          "with autotuned tiles, against one omm() call per system")
@@ -198,6 +200,9 @@ def main():
         return 0
     if args.group:
         print(json.dumps(group_bench(oh, F, args.group_steps, args.group_count)), flush=True)
+        return 0
+    if args.la:
+        print(json.dumps(la_bench(oh, args.L)), flush=True)
         return 0
 
     if args.dims:
@@ -499,6 +504,88 @@ def group_bench(oh, F, steps, count, seed=None, stream=6):
     log(f"bench group: sequential {res['sequential_s_per_step_all_systems']:.3f} s per step; "
         f"group speedup {res['group_speedup']:.2f}x, {res['group_TFLOPs']:.1f} TF/s")
     return {"group_batch": res}
+
+
+def la_bench(oh, L=16, reps=5):
+    """The kernels outside the flavour-0 CG loop, at the n = 4 L^3 lattice (L = 16: n = 16384,
+    m = 8192), FP64, inputs resident in HBM, wall clock around synchronised launches:
+    * the SDDMM Cp := C^T B on H's pattern (the density matrix / dE/dH of a 'gdcsc' D_min), against
+      the HBM roofline on its algorithmic bytes (C and B read once, the blocks written once);
+    * the dense factorisations of the Cholesky and preconditioned flavours on the workload's own S
+      and H stored dense (potrf of S, sygst of H, the SPD inverse of S, the back-transform trsm of
+      an m x n state), in TFLOP/s at LAPACK's flop counts (potrf n^3/3, sygst n^3, potrf + potri
+      n^3, trsm m n^2)."""
+    import ctypes as C
+    nb = 4 * L ** 3
+    no = nb // 2
+    rng = np.random.default_rng(5)
+    ctx = oh.Ctx(0)
+    res = {"n": nb, "m": no, "dtype": "f64"}
+
+    def timed(fn, n_rep, prep=None):
+        best = []
+        for _ in range(n_rep):
+            if prep is not None:
+                prep()
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            best.append(time.perf_counter() - t0)
+        return float(np.median(best))
+
+    # SDDMM on H's pattern
+    cp, ri, hv = oh.lattice_bsr(L, 0)
+    Cp = oh.Bsr(ctx, nb, cp, ri, hv)
+    dA = oh.DMat.from_host(ctx, rng.standard_normal((no, nb)))
+    dB = oh.DMat.from_host(ctx, rng.standard_normal((no, nb)))
+    nnzb = len(ri)
+    oh.sddmm(ctx, dA, dB, Cp)  # warm-up
+    t = timed(lambda: oh.sddmm(ctx, dA, dB, Cp), reps)
+    alg = 2.0 * no * nb * 8 + nnzb * 128.0 + 4.0 * (nnzb + nb // 4 + 1)
+    res["sddmm"] = {"ms": t * 1e3, "bytes_per_launch": alg, "achieved_GBs": alg / t / 1e9,
+                    "frac_hbm": alg / t / 8.0e12, "nnzb": nnzb}
+    log(f"bench la: sddmm {t * 1e3:.3f} ms, {alg / t / 1e9:.0f} GB/s")
+    dA.free()
+    dB.free()
+
+    S = oh.bsr_to_dense(nb, *oh.lattice_bsr(L, 1))
+    H = oh.bsr_to_dense(nb, *oh.lattice_bsr(L, 0))
+    dS = oh.DMat.from_host(ctx, S)
+    dU = oh.DMat(ctx, nb, nb, oh.OMX_F64)
+    info = C.c_int(0)
+    n3 = float(nb) ** 3
+
+    def potrf(M):
+        oh.check(oh.lib().omx_potrf(ctx.h, oh.OMX_F64, nb, M.p, M.ld, C.byref(info)), "potrf")
+        assert info.value == 0, info.value
+
+    t = timed(lambda: potrf(dU), 3, prep=lambda: dU.upload(S))
+    res["potrf"] = {"ms": t * 1e3, "TFLOPs": n3 / 3 / t / 1e12}
+    log(f"bench la: potrf {t * 1e3:.1f} ms")
+    dH = oh.DMat(ctx, nb, nb, oh.OMX_F64)
+    t = timed(lambda: oh.check(oh.lib().omx_sygst(ctx.h, oh.OMX_F64, nb, dH.p, dH.ld, dU.p, dU.ld), "sygst"), 3,
+              prep=lambda: dH.upload(H))
+    res["sygst"] = {"ms": t * 1e3, "TFLOPs": n3 / t / 1e12}
+    log(f"bench la: sygst {t * 1e3:.1f} ms")
+    dH.free()
+    dX = oh.DMat.from_host(ctx, rng.standard_normal((no, nb)))
+    t = timed(lambda: oh.check(oh.lib().omx_trsm(ctx.h, oh.OMX_F64, b"r", b"t", no, nb, 1.0, dU.p, dU.ld, dX.p,
+                                                 dX.ld), "trsm"), 3)
+    res["trsm_back_transform"] = {"ms": t * 1e3, "TFLOPs": float(no) * nb * nb / t / 1e12}
+    log(f"bench la: trsm {t * 1e3:.1f} ms")
+    dX.free()
+
+    def inv():
+        oh.check(oh.lib().omx_inverse_spd(ctx.h, oh.OMX_F64, nb, dS.p, dS.ld, C.byref(info)), "inverse_spd")
+        assert info.value == 0, info.value
+
+    t = timed(inv, 3, prep=lambda: dS.upload(S))
+    res["inverse_spd"] = {"ms": t * 1e3, "TFLOPs": n3 / t / 1e12}
+    log(f"bench la: inverse {t * 1e3:.1f} ms")
+    for M in (dS, dU):
+        M.free()
+    return {"dense_la_and_sddmm": res}
 
 
 def kpoint_bench(oh, F, steps, L=10, nk1=4, eta=0.05):

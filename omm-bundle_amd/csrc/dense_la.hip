@@ -16,6 +16,9 @@
 namespace omx {
 
 constexpr int LA_NB = 32;
+// Outer block of the factorisations and solves: each outer step's update is one GEMM with K = 256
+// (compute-bound), and the 32-wide kernels run only inside the 256-wide diagonal blocks.
+constexpr int LA_OUTER = 256;
 constexpr int LA_NT = 256;
 
 // Unblocked upper Cholesky of the nb x nb diagonal block at A (in place), one workgroup; sets
@@ -179,18 +182,28 @@ static inline int la_grid(int64_t total) {
 }
 
 static int trsm_impl(omx_ctx ctx, char side, char trans, int m, int n, double alpha, const double *U, int64_t ldu,
-                     double *B, int64_t ldb) {
+                     double *B, int64_t ldb, int bs = LA_OUTER) {
     const bool left = (side == 'l' || side == 'L');
     const bool tr = is_trans(trans);
     const int nu = left ? m : n;  // order of U
-    const int nblk = (nu + LA_NB - 1) / LA_NB;
+    const int nblk = (nu + bs - 1) / bs;
     double a = alpha;
+    // diagonal blocks wider than the kernels' 32 are solved by the same routine in 32-wide blocks
+    auto sub = [&](int k0, int nb, double aa) {
+        return left ? trsm_impl(ctx, side, trans, nb, n, aa, U + k0 + (int64_t)k0 * ldu, ldu, B + k0, ldb, LA_NB)
+                    : trsm_impl(ctx, side, trans, m, nb, aa, U + k0 + (int64_t)k0 * ldu, ldu, B + (int64_t)k0 * ldb,
+                                ldb, LA_NB);
+    };
     if (left) {
         if (tr) {  // U^T X = B: forward over block rows
             for (int b = 0; b < nblk; ++b) {
-                const int k0 = b * LA_NB, nb = std::min(LA_NB, nu - k0);
-                hipLaunchKernelGGL(k_trsm_left_block<true>, dim3((n + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
-                                   ctx->stream, nb, n, U + k0 + (int64_t)k0 * ldu, ldu, a, B + k0, ldb);
+                const int k0 = b * bs, nb = std::min(bs, nu - k0);
+                if (nb > LA_NB) {
+                    if (int rc = sub(k0, nb, a)) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_trsm_left_block<true>, dim3((n + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
+                                       ctx->stream, nb, n, U + k0 + (int64_t)k0 * ldu, ldu, a, B + k0, ldb);
+                }
                 const int rest = nu - k0 - nb;
                 if (rest > 0) {
                     // B[k0+nb:, :] = a*B[k0+nb:, :] - U[k0:k0+nb, k0+nb:]^T X_b
@@ -202,9 +215,13 @@ static int trsm_impl(omx_ctx ctx, char side, char trans, int m, int n, double al
             }
         } else {  // U X = B: backward
             for (int b = nblk - 1; b >= 0; --b) {
-                const int k0 = b * LA_NB, nb = std::min(LA_NB, nu - k0);
-                hipLaunchKernelGGL(k_trsm_left_block<false>, dim3((n + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
-                                   ctx->stream, nb, n, U + k0 + (int64_t)k0 * ldu, ldu, a, B + k0, ldb);
+                const int k0 = b * bs, nb = std::min(bs, nu - k0);
+                if (nb > LA_NB) {
+                    if (int rc = sub(k0, nb, a)) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_trsm_left_block<false>, dim3((n + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
+                                       ctx->stream, nb, n, U + k0 + (int64_t)k0 * ldu, ldu, a, B + k0, ldb);
+                }
                 if (k0 > 0) {
                     // B[:k0, :] = a*B[:k0, :] - U[:k0, k0:k0+nb] X_b
                     int rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, false, false, k0, n, nb, -1.0,
@@ -217,10 +234,14 @@ static int trsm_impl(omx_ctx ctx, char side, char trans, int m, int n, double al
     } else {
         if (!tr) {  // X U = B: forward over block columns
             for (int b = 0; b < nblk; ++b) {
-                const int k0 = b * LA_NB, nb = std::min(LA_NB, nu - k0);
-                hipLaunchKernelGGL(k_trsm_right_block<false>, dim3((m + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
-                                   ctx->stream, nb, m, U + k0 + (int64_t)k0 * ldu, ldu, a,
-                                   B + (int64_t)k0 * ldb, ldb);
+                const int k0 = b * bs, nb = std::min(bs, nu - k0);
+                if (nb > LA_NB) {
+                    if (int rc = sub(k0, nb, a)) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_trsm_right_block<false>, dim3((m + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
+                                       ctx->stream, nb, m, U + k0 + (int64_t)k0 * ldu, ldu, a,
+                                       B + (int64_t)k0 * ldb, ldb);
+                }
                 const int rest = nu - k0 - nb;
                 if (rest > 0) {
                     // B[:, k0+nb:] = a*B[:, k0+nb:] - X_b U[k0:k0+nb, k0+nb:]
@@ -233,10 +254,14 @@ static int trsm_impl(omx_ctx ctx, char side, char trans, int m, int n, double al
             }
         } else {  // X U^T = B: backward over block columns
             for (int b = nblk - 1; b >= 0; --b) {
-                const int k0 = b * LA_NB, nb = std::min(LA_NB, nu - k0);
-                hipLaunchKernelGGL(k_trsm_right_block<true>, dim3((m + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
-                                   ctx->stream, nb, m, U + k0 + (int64_t)k0 * ldu, ldu, a,
-                                   B + (int64_t)k0 * ldb, ldb);
+                const int k0 = b * bs, nb = std::min(bs, nu - k0);
+                if (nb > LA_NB) {
+                    if (int rc = sub(k0, nb, a)) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_trsm_right_block<true>, dim3((m + LA_NT - 1) / LA_NT), dim3(LA_NT), 0,
+                                       ctx->stream, nb, m, U + k0 + (int64_t)k0 * ldu, ldu, a,
+                                       B + (int64_t)k0 * ldb, ldb);
+                }
                 if (k0 > 0) {
                     // B[:, :k0] = a*B[:, :k0] - X_b U[:k0, k0:k0+nb]^T
                     int rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, false, true, m, k0, nb, -1.0,
@@ -251,15 +276,13 @@ static int trsm_impl(omx_ctx ctx, char side, char trans, int m, int n, double al
     return OMX_OK;
 }
 
-static int potrf_impl(omx_ctx ctx, int n, double *A, int64_t lda, int *info) {
-    // the pivot flag lives in the last scalar slot (the GEMM updates use the workspace)
-    int rc = OMX_OK;
-    int *d_info = (int *)(ctx->d_scalars + (kScalarSlots - 1));
-    OMX_CHECK_HIP(hipMemsetAsync(d_info, 0, sizeof(int), ctx->stream));
+// The Cholesky of one diagonal block (order n <= LA_OUTER) in 32-wide steps; pivots are reported
+// at their global index koff + local index.
+static int potrf_block(omx_ctx ctx, int n, double *A, int64_t lda, int koff, int *d_info) {
     for (int k0 = 0; k0 < n; k0 += LA_NB) {
         const int nb = std::min(LA_NB, n - k0);
         hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(LA_NT), 0, ctx->stream, nb, A + k0 + (int64_t)k0 * lda, lda,
-                           k0, d_info);
+                           koff + k0, d_info);
         const int rest = n - k0 - nb;
         if (rest > 0) {
             // panel: U[k0:k0+nb, k0+nb:] = U_kk^-T A[k0:k0+nb, k0+nb:]
@@ -267,10 +290,32 @@ static int potrf_impl(omx_ctx ctx, int n, double *A, int64_t lda, int *info) {
                                ctx->stream, nb, rest, A + k0 + (int64_t)k0 * lda, lda, 1.0,
                                A + k0 + (int64_t)(k0 + nb) * lda, lda);
             // trailing: A[k0+nb:, k0+nb:] -= panel^T panel (both triangles; only the upper is used)
-            rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, true, false, rest, rest, nb, -1.0,
-                             A + k0 + (int64_t)(k0 + nb) * lda, lda, A + k0 + (int64_t)(k0 + nb) * lda, lda, 1.0,
-                             A + k0 + nb + (int64_t)(k0 + nb) * lda, lda);
+            int rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, true, false, rest, rest, nb, -1.0,
+                                 A + k0 + (int64_t)(k0 + nb) * lda, lda, A + k0 + (int64_t)(k0 + nb) * lda, lda, 1.0,
+                                 A + k0 + nb + (int64_t)(k0 + nb) * lda, lda);
             if (rc) return rc;
+        }
+    }
+    return OMX_OK;
+}
+
+static int potrf_impl(omx_ctx ctx, int n, double *A, int64_t lda, int *info) {
+    // the pivot flag lives in the last scalar slot (the GEMM updates use the workspace)
+    int *d_info = (int *)(ctx->d_scalars + (kScalarSlots - 1));
+    OMX_CHECK_HIP(hipMemsetAsync(d_info, 0, sizeof(int), ctx->stream));
+    for (int k0 = 0; k0 < n; k0 += LA_OUTER) {
+        const int kb = std::min(LA_OUTER, n - k0);
+        double *Akk = A + k0 + (int64_t)k0 * lda;
+        int rc = potrf_block(ctx, kb, Akk, lda, k0, d_info);
+        if (rc) return rc;
+        const int rest = n - k0 - kb;
+        if (rest > 0) {
+            // panel rows k0..k0+kb: U_kk^-T A, then the trailing update with K = kb
+            double *P = A + k0 + (int64_t)(k0 + kb) * lda;
+            if ((rc = trsm_impl(ctx, 'l', 't', kb, rest, 1.0, Akk, lda, P, lda))) return rc;
+            if ((rc = launch_gemm(ctx, OMX_PREC_F64, OMX_F64, true, false, rest, rest, kb, -1.0, P, lda, P, lda, 1.0,
+                                  A + k0 + kb + (int64_t)(k0 + kb) * lda, lda)))
+                return rc;
         }
     }
     OMX_CHECK_HIP(hipGetLastError());

@@ -18,7 +18,7 @@ def _spd(n):
     return A @ A.T / n + np.eye(n)
 
 
-@pytest.mark.parametrize("n", [1, 31, 64, 100, 257])
+@pytest.mark.parametrize("n", [1, 31, 64, 100, 257, 600])
 def test_potrf(ctx, n):
     print("CASE potrf", n, flush=True)
     S = _spd(n)
@@ -35,22 +35,25 @@ def test_potrf(ctx, n):
     assert np.array_equal(np.triu(dS2.download()), U)
 
 
-def test_potrf_not_spd(ctx):
-    print("CASE potrf not spd", flush=True)
-    n = 40
+@pytest.mark.parametrize("n,bad", [(40, 35), (600, 400)])
+def test_potrf_not_spd(ctx, n, bad):
+    """A negative diagonal entry makes that pivot the first non-positive one (earlier pivots do not
+    depend on it): info names it, 1-based, also past the first 256-wide outer block."""
+    print("CASE potrf not spd", n, bad, flush=True)
     S = _spd(n)
-    S[35, 35] = -1.0
+    S[bad, bad] = -1.0
     dS = oh.DMat.from_host(ctx, S)
     info = C.c_int(-1)
     oh.check(oh.lib().omx_potrf(ctx.h, oh.OMX_F64, n, dS.p, dS.ld, C.byref(info)), "potrf")
-    assert info.value > 0
+    assert info.value == bad + 1
 
 
 @pytest.mark.parametrize("side", ["l", "r"])
 @pytest.mark.parametrize("trans", ["n", "t"])
-def test_trsm_trmm(ctx, side, trans):
-    nu, other = 75, 50
-    print("CASE trsm", side, trans, flush=True)
+@pytest.mark.parametrize("nu", [75, 600])
+def test_trsm_trmm(ctx, side, trans, nu):
+    other = 50
+    print("CASE trsm", side, trans, nu, flush=True)
     U = sla.cholesky(_spd(nu), lower=False)
     Ug = U + np.tril(rng.standard_normal((nu, nu)), -1)  # garbage below the diagonal is ignored
     B = rng.standard_normal((nu, other) if side == "l" else (other, nu))
@@ -68,9 +71,9 @@ def test_trsm_trmm(ctx, side, trans):
     assert np.max(np.abs(dB.download() - ref)) < 1e-12 * np.max(np.abs(ref)) * nu
 
 
-def test_sygst_and_inverse(ctx):
-    n = 90
-    print("CASE sygst/inverse", flush=True)
+@pytest.mark.parametrize("n", [90, 600])
+def test_sygst_and_inverse(ctx, n):
+    print("CASE sygst/inverse", n, flush=True)
     S = _spd(n)
     H = rng.standard_normal((n, n))
     H = H + H.T
