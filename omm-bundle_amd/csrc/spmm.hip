@@ -695,14 +695,17 @@ extern "C" int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const
 // op = 'c','n'), used for the density matrix D = C^T (QW C) on the pattern of H
 // (libOMM/src/omm_ops.F90:104-124 with a sparse D_min) and for dE/dH.
 //
-// Mapping: a workgroup owns a chunk of SD_ROWS states and one block column J; it holds B's four
-// columns 4J..4J+3 in registers and, per stored block, reads A's four columns, forms the 16
-// products per thread and reduces them over the workgroup.  Per-chunk partial blocks go to the
+// Mapping: a workgroup owns a chunk of SD_NT*VEC*SD_STR states and one block column J; it holds
+// B's four columns 4J..4J+3 of the chunk in registers and, per stored block, reads A's four
+// columns, accumulates the 16 products per thread over its SD_STR strides and reduces them over
+// the workgroup once.  Per-chunk partial blocks go to the
 // workspace and a second kernel adds them in chunk order (deterministic).
 // =============================================================================================
 namespace omx {
 
 constexpr int SD_NT = 256;
+// strides of SD_NT*VEC states per thread: one workgroup reduction per stored block per SD_STR strides
+constexpr int SD_STR = 4;
 
 template <typename T, int VEC>
 __global__ __launch_bounds__(SD_NT) void k_sddmm_bsr4_partial(int no, int nb, const T *__restrict__ A, int64_t lda,
@@ -710,39 +713,46 @@ __global__ __launch_bounds__(SD_NT) void k_sddmm_bsr4_partial(int no, int nb, co
                                                               const int *__restrict__ colptr,
                                                               const int *__restrict__ rowidx, int rchunks,
                                                               int64_t nnzb, double *__restrict__ partial) {
-    constexpr int ROWS = SD_NT * VEC;
+    constexpr int ROWS = SD_NT * VEC * SD_STR;
     const int rc = blockIdx.x % rchunks, J = blockIdx.x / rchunks;
-    const int i0 = rc * ROWS + threadIdx.x * VEC;
-    const bool full = (i0 + VEC <= no);
     __shared__ double red[16][SD_NT / 64];
-    double bcol[4][VEC];
+    double bcol[SD_STR][4][VEC];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int j = 4 * J + c;
+    for (int st = 0; st < SD_STR; ++st) {
+        const int i0 = rc * ROWS + st * SD_NT * VEC + threadIdx.x * VEC;
 #pragma unroll
-        for (int q = 0; q < VEC; ++q) bcol[c][q] = (j < nb && i0 + q < no) ? (double)B[(int64_t)j * ldb + i0 + q] : 0.0;
+        for (int c = 0; c < 4; ++c) {
+            const int j = 4 * J + c;
+#pragma unroll
+            for (int q = 0; q < VEC; ++q)
+                bcol[st][c][q] = (j < nb && i0 + q < no) ? (double)B[(int64_t)j * ldb + i0 + q] : 0.0;
+        }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int l = colptr[J]; l < colptr[J + 1]; ++l) {
         const int I = rowidx[l];
-        double acol[4][VEC];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int k = 4 * I + r;
-            const T *src = A + (int64_t)k * lda + i0;
-#pragma unroll
-            for (int q = 0; q < VEC; ++q) acol[r][q] = (k < nb && (full || i0 + q < no)) ? (double)src[q] : 0.0;
-        }
         double s[16];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int e = 0; e < 16; ++e) s[e] = 0.0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                double t = 0.0;
+        for (int st = 0; st < SD_STR; ++st) {
+            const int i0 = rc * ROWS + st * SD_NT * VEC + threadIdx.x * VEC;
+            const bool full = (i0 + VEC <= no);
+            double acol[4][VEC];
 #pragma unroll
-                for (int q = 0; q < VEC; ++q) t = fma(acol[r][q], bcol[c][q], t);
-                s[r + 4 * c] = t;
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * I + r;
+                const T *src = A + (int64_t)k * lda + i0;
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) acol[r][q] = (k < nb && (full || i0 + q < no)) ? (double)src[q] : 0.0;
             }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int q = 0; q < VEC; ++q) s[r + 4 * c] = fma(acol[r][q], bcol[st][c][q], s[r + 4 * c]);
+        }
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             double t = s[e];
@@ -774,7 +784,7 @@ __global__ void k_sddmm_bsr4_final(int64_t nnzb, int rchunks, const double *__re
 template <typename T, int VEC>
 static int run_sddmm(omx_ctx ctx, int no, int nb, double alpha, const T *A, int64_t lda, const T *B, int64_t ldb,
                      double beta, omx_bsr C) {
-    constexpr int ROWS = SD_NT * VEC;
+    constexpr int ROWS = SD_NT * VEC * SD_STR;
     const int rchunks = (no + ROWS - 1) / ROWS;
     const int64_t grid = (int64_t)rchunks * C->nbc;
     OMX_REQUIRE(grid < (1LL << 31), "omx_sddmm_bsr: grid too large");
