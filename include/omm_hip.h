@@ -300,9 +300,13 @@ int omx_zbatch_destroy(omx_zbatch b);
  * column-major each). */
 int omx_zbatch_set_c(omx_zbatch b, const void *C);
 int omx_zbatch_get_c(omx_zbatch b, void *C);
-/* Run up to max_steps CG steps (reading the convergence flags every check_every steps; use_graph
- * replays one captured step).  Returns per system e_min + eta*TrQS, the step count icg and the
- * converged flag (e_diff <= cg_tol, cg_tol < 0: 1e-9). */
+/* Run up to max_steps CG steps per system.  use_graph captures check_every steps as one graph
+ * (re-captured only when check_every changes) and launches it until every system has converged
+ * or reached max_steps, reading the convergence flags once per launch; cg_tol and max_steps live
+ * in device memory, so every solve replays the graph with its own values.  Without use_graph the
+ * same steps are launched eagerly.  Returns per system e_min + eta*TrQS, the step count icg and
+ * the converged flag (e_diff <= cg_tol, cg_tol < 0: 1e-9); *steps_run counts the step sequences
+ * issued, including those that were no-ops for every system. */
 int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int check_every, int use_graph,
                      double *e_min, int *icg, int *converged, int *steps_run);
 
@@ -321,7 +325,8 @@ int omx_group_destroy(omx_group g);
 int omx_group_set_c(omx_group g, int s, const double *C);
 int omx_group_get_c(omx_group g, int s, double *C);
 /* Solve all systems from their current C (init_C semantics): e_min[s] (with + eta[s]*TrQS),
- * icg[s], converged[s]; the flags are read every check_every steps. */
+ * icg[s], converged[s]; graphs of check_every steps and the device-side cg_tol and max_steps as in
+ * omx_zbatch_solve. */
 int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps, int check_every, int use_graph,
                     double *e_min, int *icg, int *converged, int *steps_run);
 

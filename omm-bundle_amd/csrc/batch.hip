@@ -45,12 +45,12 @@ struct omx_zbatch_s {
     // per-system scalars
     double *coeff = nullptr;  // 18 per system
     double *dots = nullptr;   // 2 per system
-    double *sc = nullptr;     // [x, lambda, e_min, e_diff] per system
+    double *sc = nullptr;     // [x, lambda, e_min, e_diff] per system, then the solve's [cg_tol, max_steps]
     int *active = nullptr;    // per-system flags, F_COUNT arrays of nsys (see flag())
     double *partial = nullptr, *partial2 = nullptr;
-    double tol = 1e-9;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int graph_steps = 0;  // CG steps captured in exec
 };
 
 namespace omx {
@@ -110,13 +110,17 @@ __device__ inline void quartic_min(const double *c, double &x_min, bool &fail) {
 
 // Per system and step, written by k_batch_quartic:
 //   failf  the line search failed (coefficients rescaled, omm.F90:470-477)
-//   cont   stepped and not converged: HW, HC, SC and G are updated (omm.F90:506-534)
+//   cont   stepped, not converged and below the step cap: HW, HC, SC and G are updated
+//          (omm.F90:506-534)
 //   grad   cont and not failf: the gradient is recomputed and lambda follows (omm.F90:536-546)
-// sc per system: [0] x_min, [1] lambda, [2] e_min, [3] e_diff
+// sc per system: [0] x_min, [1] lambda, [2] e_min, [3] e_diff.  params = [cg_tol, max_steps] of the
+// current solve, in device memory so a captured step reads the values of the solve that replays it.
+// A system that reaches max_steps stops like a converged one (before the updates) but keeps
+// conv = 0; the steps that follow in the same graph are no-ops for it.
 __global__ void k_batch_quartic(int nsys, const double *__restrict__ coeff, double *__restrict__ sc,
                                 const int *__restrict__ active, int *__restrict__ failf, int *__restrict__ cont,
                                 int *__restrict__ grad, int *__restrict__ icg, int *__restrict__ conv,
-                                int *__restrict__ nfail, double tol) {
+                                int *__restrict__ nfail, const double *__restrict__ params) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     if (!active[s]) {
@@ -143,11 +147,12 @@ __global__ void k_batch_quartic(int nsys, const double *__restrict__ coeff, doub
     v[2] = e;
     v[3] = ed;
     icg[s] += 1;
-    const bool done = (ed <= tol);
+    const bool done = (ed <= params[0]);
+    const bool capped = icg[s] >= (int)params[1];
     if (done) conv[s] = 1;
     failf[s] = fail ? 1 : 0;
-    cont[s] = done ? 0 : 1;
-    grad[s] = (!done && !fail) ? 1 : 0;
+    cont[s] = (done || capped) ? 0 : 1;
+    grad[s] = (!done && !capped && !fail) ? 1 : 0;
 }
 
 // The element-wise kernels below are written over El<T> (double for real systems, double2 for
@@ -279,9 +284,15 @@ __global__ void k_batch_lambda(int nsys, const double *__restrict__ dots, double
     active[s] = cont[s];
 }
 
+// Start of a solve: every system active, counters cleared, and the solve's parameters
+// [cg_tol, max_steps] stored after the per-system scalars (params = sc + 4*nsys).
 __global__ void k_batch_init_flags(int nsys, int *__restrict__ active, int *__restrict__ icg, int *__restrict__ conv,
-                                   int *__restrict__ nfail, double *__restrict__ sc) {
+                                   int *__restrict__ nfail, double *__restrict__ sc, double tol, int max_steps) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        sc[4 * nsys] = tol;
+        sc[4 * nsys + 1] = (double)max_steps;
+    }
     if (s >= nsys) return;
     active[s] = 1;
     icg[s] = 0;
@@ -367,7 +378,7 @@ static int batch_step(omx_zbatch b) {
                                    b->partial, b->coeff, nullptr)))
         return rc;
     hipLaunchKernelGGL(k_batch_quartic, dim3((ns + 63) / 64), dim3(64), 0, ctx->stream, ns, b->coeff, b->sc, act,
-                       failf, cont, grad, flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->tol);
+                       failf, cont, grad, flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->sc + 4 * ns);
     // C and SW for every system that stepped (omm.F90:470-502)
     hipLaunchKernelGGL(k_batch_axpy<T>, bt_grid(lenC, ns), dim3(BT_NT), 0, ctx->stream, lenC, (const T *)b->D,
                        (T *)b->C, b->sc, act, failf, 0.5);
@@ -450,7 +461,7 @@ int omx_zbatch_create_dt(omx_ctx ctx, int dt, int nsys, int no, int nb, const in
     for (auto p : nn) ok = ok && hipMalloc(p, es * lenW) == hipSuccess;
     ok = ok && hipMalloc(&b->coeff, sizeof(double) * 18 * nsys) == hipSuccess &&
          hipMalloc(&b->dots, sizeof(double) * 2 * nsys) == hipSuccess &&
-         hipMalloc(&b->sc, sizeof(double) * 4 * nsys) == hipSuccess &&
+         hipMalloc(&b->sc, sizeof(double) * (4 * nsys + 2)) == hipSuccess &&
          hipMalloc(&b->active, sizeof(int) * F_COUNT * nsys) == hipSuccess &&
          hipMalloc(&b->partial, sizeof(double) * coeff_partial_doubles(no, nsys)) == hipSuccess &&
          hipMalloc(&b->partial2, sizeof(double) * pr_partial_doubles(no, nb, nsys)) == hipSuccess;
@@ -504,11 +515,11 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
     hipStream_t st = ctx->stream;
     const int ns = b->nsys, no = b->no, nb = b->nb;
     const int64_t lenC = (int64_t)no * nb, lenW = (int64_t)no * no, sV = 32 * b->nnzb;
-    b->tol = cg_tol < 0.0 ? 1.0e-9 : cg_tol;
+    const double tol = cg_tol < 0.0 ? 1.0e-9 : cg_tol;
     int rc;
     // initial state (omm.F90:350-370): HC = C H, SC = C S, HW = HC C^H, SW = SC C^H, G
     hipLaunchKernelGGL(k_batch_init_flags, dim3((ns + 63) / 64), dim3(64), 0, st, ns, flag(b, F_ACTIVE),
-                       flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->sc);
+                       flag(b, F_ICG), flag(b, F_CONV), flag(b, F_NFAIL), b->sc, tol, max_steps);
     int *act = flag(b, F_ACTIVE);
     if ((rc = launch_spmm_z_batched(ctx, b->dt, no, nb, b->nbc, b->colptr, b->rowidx, b->hval, sV, b->C, no, lenC, b->HC, no,
                                     lenC, ns, act)))
@@ -523,35 +534,48 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
                                         act)))
         return rc;
     if ((rc = batch_calc_G_any(b, act))) return rc;
-    // the CG steps: captured once, replayed; flags read every check_every steps
-    if (use_graph && !b->exec) {
+    // the CG steps: check_every steps captured as one graph (kept while check_every is unchanged),
+    // one graph launch per flag read; the step cap and cg_tol are read from device memory, so
+    // steps past a system's convergence or cap do nothing for it
+    if (use_graph && (!b->exec || b->graph_steps != check_every)) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        b->graph_steps = 0;
         OMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = batch_step_any(b);
+        rc = OMX_OK;
+        for (int q = 0; q < check_every && !rc; ++q) rc = batch_step_any(b);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
-        if (rc) return rc;
-        OMX_CHECK_HIP(e);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            OMX_CHECK_HIP(e);
+        }
         b->graph = g;
         OMX_CHECK_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+        b->graph_steps = check_every;
     }
     std::vector<int> h_act(ns);
     int done_steps = 0;
     while (done_steps < max_steps) {
-        const int burst = std::min(check_every, max_steps - done_steps);
-        for (int q = 0; q < burst; ++q) {
-            if (use_graph)
-                OMX_CHECK_HIP(hipGraphLaunch(b->exec, st));
-            else if ((rc = batch_step_any(b)))
-                return rc;
+        if (use_graph) {
+            OMX_CHECK_HIP(hipGraphLaunch(b->exec, st));
+            done_steps += check_every;
+        } else {
+            const int burst = std::min(check_every, max_steps - done_steps);
+            for (int q = 0; q < burst; ++q)
+                if ((rc = batch_step_any(b))) return rc;
+            done_steps += burst;
         }
-        done_steps += burst;
         OMX_CHECK_HIP(hipMemcpyAsync(h_act.data(), act, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
         OMX_CHECK_HIP(hipStreamSynchronize(st));
         bool any = false;
         for (int s = 0; s < ns; ++s) any = any || h_act[s];
         if (!any) break;
     }
-    if (steps_run) *steps_run = done_steps;
+    if (steps_run) *steps_run = done_steps;  // step sequences issued (masked ones included)
     // results: e_min + eta*TrQS with TrQS = Re tr[(2I - SW) SW] = 2 Re tr(SW) - <SW, SW>
     std::vector<double> sc(4 * ns);
     std::vector<int> h_icg(ns), h_conv(ns);
@@ -589,9 +613,9 @@ struct omx_group_s {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
-    double tol = 1e-9;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int graph_steps = 0;  // CG steps captured in exec
 };
 
 enum { M_C = 0, M_HC, M_SC, M_G, M_GP, M_D, M_HG, M_SG, M_HW, M_SW, M_HWD, M_SWD, M_HWDD, M_SWDD };
@@ -651,7 +675,8 @@ static int group_sys_step(omx_group g, int s) {
     ctx->launch_mask = nullptr;
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_quartic, dim3(1), dim3(64), 0, ctx->stream, 1, g->coeff + 18 * s, scs, act, failf,
-                       cont, grad, gflag(g, F_ICG, s), gflag(g, F_CONV, s), gflag(g, F_NFAIL, s), g->tol);
+                       cont, grad, gflag(g, F_ICG, s), gflag(g, F_CONV, s), gflag(g, F_NFAIL, s),
+                       g->sc + 4 * g->nsys);
     // C and SW when the system stepped; HW, HC, SC when it is still running (omm.F90:470-527)
     hipLaunchKernelGGL(k_batch_axpy<double>, bt_grid(lenC, 1), dim3(BT_NT), 0, ctx->stream, lenC, M[M_D], M[M_C],
                        scs, act, failf, 0.5);
@@ -765,7 +790,7 @@ int omx_group_create(omx_ctx ctx, int nsys, const int *no, const omx_bsr *H, con
     }
     ok = ok && hipMalloc(&g->coeff, sizeof(double) * 18 * nsys) == hipSuccess &&
          hipMalloc(&g->dots, sizeof(double) * 2 * nsys) == hipSuccess &&
-         hipMalloc(&g->sc, sizeof(double) * 4 * nsys) == hipSuccess &&
+         hipMalloc(&g->sc, sizeof(double) * (4 * nsys + 2)) == hipSuccess &&
          hipMalloc(&g->flags, sizeof(int) * F_COUNT * nsys) == hipSuccess &&
          hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
@@ -806,10 +831,10 @@ int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps
     omx_ctx ctx = g->ctx;
     hipStream_t st = ctx->stream;
     const int ns = g->nsys;
-    g->tol = cg_tol < 0.0 ? 1.0e-9 : cg_tol;
+    const double tol = cg_tol < 0.0 ? 1.0e-9 : cg_tol;
     int rc;
     hipLaunchKernelGGL(k_batch_init_flags, dim3((ns + 63) / 64), dim3(64), 0, st, ns, g->flags + F_ACTIVE * ns,
-                       g->flags + F_ICG * ns, g->flags + F_CONV * ns, g->flags + F_NFAIL * ns, g->sc);
+                       g->flags + F_ICG * ns, g->flags + F_CONV * ns, g->flags + F_NFAIL * ns, g->sc, tol, max_steps);
     // initial state of every system (omm.F90:350-370), eager: HC = C H, SC = C S, HW = HC C^T,
     // SW = SC C^T, G; with tuning on, each system's shapes are tuned here, before any capture
     rc = group_fanout(g, [&](int s) {
@@ -825,31 +850,41 @@ int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps
         return r;
     });
     if (rc) return rc;
-    if (use_graph && !g->exec) {
+    // check_every steps of every system captured as one graph (one branch per system and step,
+    // joined after each step), one launch per flag read; see omx_zbatch_solve
+    if (use_graph && (!g->exec || g->graph_steps != check_every)) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        g->exec = nullptr;
+        g->graph = nullptr;
+        g->graph_steps = 0;
         OMX_CHECK_HIP(hipStreamSynchronize(st));  // tuning (if on) has finished before the capture
         OMX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); });
+        rc = OMX_OK;
+        for (int q = 0; q < check_every && !rc; ++q) rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); });
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(st, &gr);
-        if (rc) {
+        if (rc || e != hipSuccess) {
             if (gr) (void)hipGraphDestroy(gr);
-            return rc;
+            if (rc) return rc;
+            OMX_CHECK_HIP(e);
         }
-        OMX_CHECK_HIP(e);
         g->graph = gr;
         OMX_CHECK_HIP(hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0));
+        g->graph_steps = check_every;
     }
     std::vector<int> h_act(ns);
     int done_steps = 0;
     while (done_steps < max_steps) {
-        const int burst = std::min(check_every, max_steps - done_steps);
-        for (int q = 0; q < burst; ++q) {
-            if (use_graph)
-                OMX_CHECK_HIP(hipGraphLaunch(g->exec, st));
-            else if ((rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); })))
-                return rc;
+        if (use_graph) {
+            OMX_CHECK_HIP(hipGraphLaunch(g->exec, st));
+            done_steps += check_every;
+        } else {
+            const int burst = std::min(check_every, max_steps - done_steps);
+            for (int q = 0; q < burst; ++q)
+                if ((rc = group_fanout(g, [&](int s) { return group_sys_step(g, s); }))) return rc;
+            done_steps += burst;
         }
-        done_steps += burst;
         OMX_CHECK_HIP(hipMemcpyAsync(h_act.data(), g->flags + F_ACTIVE * ns, sizeof(int) * ns, hipMemcpyDeviceToHost,
                                      st));
         OMX_CHECK_HIP(hipStreamSynchronize(st));
@@ -857,7 +892,7 @@ int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps
         for (int s = 0; s < ns; ++s) any = any || h_act[s];
         if (!any) break;
     }
-    if (steps_run) *steps_run = done_steps;
+    if (steps_run) *steps_run = done_steps;  // step sequences issued (masked ones included)
     // e_min + eta*TrQS, TrQS = tr[(2I - SW) SW] = 2 tr(SW) - <SW, SW> (omm.F90:576-593)
     std::vector<double> sc(4 * ns);
     std::vector<int> h_icg(ns), h_conv(ns);

@@ -3,9 +3,8 @@
 ! (libOMM/examples/example_gamma.F90:134-245, flavour 0) with H and S moved to HBM as 'gdcsc'       !
 ! operators and all dense matrices allocated with the 'gdden' label, operations 'hip'.            !
 ! Two SCF steps (the second with H_ii -= 0.001 and init_C), then the energy-weighted density      !
-! matrix.  Checks e_min, D_11 and ED_11 against the reference's constants (tolerance 1e-4; with    !
-! 'hmx' the density elements 3e-4: the low-precision phase ends on the energy, which is quadratic  !
-! in the error of C, while D is linear in it).                                                     !
+! matrix.  Checks e_min, D_11 and ED_11 against the reference's constants (tolerance 1e-4, as     !
+! libOMM/examples/example_gamma.F90:269, for every operation label).                               !
 ! Flavours 1 (Cholesky) and 3 (preconditioning) run on dense device H and S ('gdden'), as the     !
 ! reference's example runs them on 'sdden' (example_gamma.F90:119-132).                            !
 ! Usage: example_gamma_hip <example_gamma.dat> [operation label, default 'hip'] [flavour, default 0]!
@@ -39,7 +38,6 @@ program example_gamma_hip
     read(arg, *) flavour
   end if
   d_tol = 1.0e-4_dp
-  if (m_operation == 'hmx') d_tol = 3.0e-4_dp
 
   open(10, file=trim(path))
   read(10, '(i2,1x,i2,1x,i4)') m, n, num_elements

@@ -194,6 +194,8 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
   nfail = 0
   icg_lowp = 0
   stat_nlog = 0
+  stat_g0 = 0.0_dp
+  stat_gend = 0.0_dp
   if (m < 10000) then
     n_step_max_inner = m*n - 1
   else
@@ -224,7 +226,7 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
     call point_state()
     call alloc_locals(m_storage)
     call initial_energy('hip', .true.)
-    call cg('hip', cg_tol_internal, opt_max_iter, .false.)
+    call cg('hip', cg_tol_internal, opt_max_iter, .false., opt_refine_grad)
   else
     call point_state()
     call alloc_locals(m_storage)
@@ -364,20 +366,34 @@ contains
   !> Stops when e_diff <= tol, or after max_steps steps of this phase (0 = reference caps only).
   !> lowp: the low-precision phase of 'hmx' (periodic recomputation of the state from C, and the
   !> stop test on the decrease predicted by the line search).
-  subroutine cg(lab, tol, max_steps, lowp)
+  !> gtol (the FP64 refinement of 'hmx'): e_diff <= tol ends the loop only when the gradient of the
+  !> step, ||G_p||, is at most gtol times the gradient at the start of the loop.  The refinement
+  !> starts from the low-precision phase's precision floor, where the first FP64 steps (restarted
+  !> CG) can lower the energy by less than cg_tol while C, and so the density matrix, still carries
+  !> that phase's error; the energy is quadratic in that error, the gradient linear.
+  subroutine cg(lab, tol, max_steps, lowp, gtol)
     character(3), intent(in) :: lab
     real(dp), intent(in) :: tol
     integer, intent(in) :: max_steps
     logical, intent(in) :: lowp
+    real(dp), intent(in), optional :: gtol
     integer :: outer, inner, steps, n_below, n_stall
     logical :: ls_conv, capped, rewin_now
-    real(dp) :: e_pred, e_best
+    real(dp) :: e_pred, e_best, g2_start, g2
     conv = .false.
     capped = .false.
     steps = 0
     n_below = 0
     n_stall = 0
     e_best = huge(1.0_dp)
+    g2_start = -1.0_dp
+    if (present(gtol)) then
+      if (gtol > 0.0_dp) then
+        call mm_trace(G, G, g2_start, scale_label())
+        stat_g0 = sqrt(g2_start)
+        stat_gend = stat_g0
+      end if
+    end if
     stat_t0 = wall_seconds()  ! the coefficient readback that precedes this point left the stream idle
     do outer = 1, n_step_max_outer
       lambda = 0.0_dp
@@ -468,6 +484,11 @@ contains
           end if
         else if (e_diff <= tol) then
           conv = .true.
+          if (g2_start > 0.0_dp) then
+            call mm_trace(G_p, G_p, g2, scale_label())
+            stat_gend = sqrt(g2)
+            conv = (g2 <= gtol*gtol*g2_start)
+          end if
         end if
         if (conv) exit
         if (max_steps > 0 .and. steps >= max_steps) then

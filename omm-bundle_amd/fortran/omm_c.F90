@@ -333,6 +333,15 @@ contains
     omm_c_stats = 0
   end function omm_c_stats
 
+  !> Gradient norms of the last call's FP64 refinement ('hmx'): at its start and at its last stop
+  !> test (both 0 when the call had no refinement with a gradient criterion).
+  integer(c_int) function omm_c_refine_stats(g0, gend) bind(c, name='omm_c_refine_stats')
+    real(c_double), intent(out) :: g0, gend
+    g0 = stat_g0
+    gend = stat_gend
+    omm_c_refine_stats = 0
+  end function omm_c_refine_stats
+
   !> Per-step energies of the last call (up to cap); returns the number written.
   integer(c_int) function omm_c_elog(buf, cap) bind(c, name='omm_c_elog')
     real(c_double), intent(out) :: buf(*)

@@ -40,6 +40,7 @@ def lib():
                                   C.c_char_p, C.c_char_p]
         L.omm_c_stats.argtypes = [C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
         L.omm_c_elog.argtypes = [_vp, _i]
+        L.omm_c_refine_stats.argtypes = [C.POINTER(_d), C.POINTER(_d)]
         L.omm_c_set_option.argtypes = [C.c_char_p, _d]
         L.omm_c_tlog.argtypes = [_vp, _i]
         L.omm_c_ctx.restype = _vp
@@ -209,9 +210,11 @@ def omm(m, n, H: Mat, S: Mat | None, C_min: Mat, *, new_S=True, D_min: Mat | Non
     k = lib().omm_c_elog(_p(buf), len(buf))
     tbuf = np.zeros(100000)
     kt = lib().omm_c_tlog(_p(tbuf), len(tbuf))
+    g0, gend = C.c_double(0), C.c_double(0)
+    lib().omm_c_refine_stats(C.byref(g0), C.byref(gend))
     return {"e_min": e.value, "icg": icg.value, "converged": bool(conv.value), "nfail": nfail.value,
             "icg_lowp": icg_lowp.value, "TrQS": trqs.value, "e_log": buf[:k].copy(), "t0": tbuf[0],
-            "t_log": tbuf[1:kt].copy()}
+            "t_log": tbuf[1:kt].copy(), "refine_g0": g0.value, "refine_gend": gend.value}
 
 
 def param_grad(D: Mat, nparam: int, seg_p, ent_p, coef_p) -> np.ndarray:

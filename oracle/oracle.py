@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboommoracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+REF_MM = os.path.join(HERE, "_ref", "ref_mm")
 
 _vp, _i, _d, _c = C.c_void_p, C.c_int, C.c_double, C.c_char
 _lib = None
@@ -82,6 +83,20 @@ def spmm_csc(A, colptr, rowidx, val, Cm, alpha=1.0, beta=0.0, opA="n"):
     lib().or_spmm_csc(opA.encode(), m, nb, alpha, _p(A), A.shape[0], _p(colptr), _p(rowidx), _p(val), beta, _p(Cm),
                       m)
     return Cm
+
+
+def sddmm_csc(opA, opB, A, B, colptr, rowidx, cval, alpha=1.0, beta=0.0):
+    """C := alpha*op(A)*op(B) + beta*C on C's CSC pattern (0-based colptr/rowidx); returns the
+    updated values (MatrixSwitch_mm_multiply.F90:334-398)."""
+    A, B = _f(A), _f(B)
+    colptr = np.ascontiguousarray(colptr, np.int32)
+    rowidx = np.ascontiguousarray(rowidx, np.int32)
+    out = np.array(cval, np.float64, copy=True)
+    n = len(colptr) - 1
+    k = A.shape[0] if opA in "tTcC" else A.shape[1]
+    lib().or_sddmm_csc(opA.encode(), opB.encode(), k, alpha, _p(A), A.shape[0], _p(B), B.shape[0], n, _p(colptr),
+                       _p(rowidx), beta, _p(out))
+    return out
 
 
 def add(opA, A, Cm, alpha, beta):
@@ -225,6 +240,31 @@ def run_ref(m, n, calls, s_present=True, sparse=False, workdir=None):
         res.append({"e_min": float(e), "C": Cm.copy(), "D": D.copy()})
     log = open(os.path.join(wd, "libOMM.log")).read() if os.path.exists(os.path.join(wd, "libOMM.log")) else ""
     return res, log
+
+
+def run_ref_sddmm(opA, opB, A, B, C0, alpha, beta):
+    """One mm_multiply(A, opA, B, opB, C, alpha, beta, 'ref') of the reference build with C stored
+    as 'sdcsc' on the nonzeros of C0 (oracle/ref_mm.F90).  Returns (colptr 0-based, rowidx 0-based,
+    values)."""
+    m, n = C0.shape
+    k = A.shape[0] if opA in "tT" else A.shape[1]
+    wd = tempfile.mkdtemp(prefix="omm_refmm_")
+    fin, fout = os.path.join(wd, "in.bin"), os.path.join(wd, "out.bin")
+    with open(fin, "wb") as f:
+        f.write(struct.pack("=5i", m, n, k, int(opA in "tT"), int(opB in "tT")))
+        f.write(struct.pack("=2d", alpha, beta))
+        for M in (A, B, C0):
+            f.write(np.asfortranarray(M, np.float64).tobytes(order="F"))
+    subprocess.run([REF_MM, fin, fout], cwd=wd, check=True, capture_output=True)
+    raw = open(fout, "rb").read()
+    nnz = struct.unpack_from("=i", raw, 0)[0]
+    off = 4
+    colptr = np.frombuffer(raw, np.int32, n + 1, off)
+    off += 4 * (n + 1)
+    rowidx = np.frombuffer(raw, np.int32, nnz, off) - 1
+    off += 4 * nnz
+    val = np.frombuffer(raw, np.float64, nnz, off)
+    return colptr.copy(), rowidx.copy(), val.copy()
 
 
 def icg_from_log(log: str):

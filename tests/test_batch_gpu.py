@@ -122,3 +122,37 @@ def test_batch_complex64_matches_closed_form(ctx):
         assert abs(r["e_min"][k] - exact[k]) < 1e-5 * abs(exact[k])
     assert B.get_c()[0].dtype == np.complex64
     B.free()
+
+
+def test_batch_second_solve_uses_its_own_tol_and_cap(ctx):
+    # cg_tol and max_steps of each solve reach the captured steps through device memory: a batch
+    # solved at 1e-6, then again at 1e-10 and with a step cap, gives bitwise the result of a fresh
+    # eager solve with those values
+    L, eta = 3, 0.2
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    phis = [(0.0, 0.0, 0.0), (0.25, 0.5, 0.0)]
+    _note("zbatch re-solve tol/cap")
+    cp, ri, hv, sv = _systems(L, phis, eta)
+    C0 = oh.random_c0(no, nb).astype(np.complex128)
+    cases = [(1e-6, 20000), (1e-10, 20000), (1e-10, 9)]
+    B = oh.ZBatch(ctx, no, nb, cp, ri, hv, sv)
+    got = []
+    for tol, cap in cases:
+        B.set_c([C0] * len(phis))
+        got.append((B.solve(eta=eta, cg_tol=tol, max_steps=cap, check_every=8, use_graph=True), B.get_c()))
+    B.free()
+    for (tol, cap), (r, cs) in zip(cases, got):
+        E = oh.ZBatch(ctx, no, nb, cp, ri, hv, sv)
+        E.set_c([C0] * len(phis))
+        re_ = E.solve(eta=eta, cg_tol=tol, max_steps=cap, check_every=8, use_graph=False)
+        ce = E.get_c()
+        E.free()
+        _note("tol", tol, "cap", cap, "graph icg", r["icg"].tolist(), "eager icg", re_["icg"].tolist())
+        assert np.array_equal(r["icg"], re_["icg"]) and np.array_equal(r["e_min"], re_["e_min"])
+        assert np.array_equal(r["converged"], re_["converged"])
+        for a, b in zip(cs, ce):
+            assert np.array_equal(a, b)
+    (r6, _), (r10, _), (rc, _) = got
+    assert r6["converged"].all() and r10["converged"].all()
+    assert (r6["icg"] < r10["icg"]).all()
+    assert (rc["icg"] == 9).all() and not rc["converged"].any()

@@ -139,3 +139,34 @@ def test_group_rejects_mismatched_operators(ctx, systems):
         oh.Group(ctx, [x["no"]], [H], [S])
     H.free()
     S.free()
+
+
+def test_group_second_solve_uses_its_own_tol_and_cap(ctx, systems):
+    # as for the k-point batch: cg_tol and max_steps of each solve reach the captured graph
+    _note("group re-solve tol/cap")
+    sys2 = systems[:2]
+    etas = [x["eta"] for x in sys2]
+    C0 = [oh.random_c0(x["no"], x["nb"]) for x in sys2]
+    cases = [(1e-6, 20000), (1e-10, 20000), (1e-10, 11)]
+    g, ops = _group(ctx, sys2)
+    got = []
+    for tol, cap in cases:
+        for k, c in enumerate(C0):
+            g.set_c(k, c)
+        got.append(g.solve(etas, cg_tol=tol, max_steps=cap, check_every=8, use_graph=True))
+    g.free()
+    for o in ops:
+        o.free()
+    for (tol, cap), r in zip(cases, got):
+        e, ops = _group(ctx, sys2)
+        for k, c in enumerate(C0):
+            e.set_c(k, c)
+        re_ = e.solve(etas, cg_tol=tol, max_steps=cap, check_every=8, use_graph=False)
+        e.free()
+        for o in ops:
+            o.free()
+        _note("tol", tol, "cap", cap, "graph icg", r["icg"].tolist(), "eager icg", re_["icg"].tolist())
+        assert np.array_equal(r["icg"], re_["icg"]) and np.array_equal(r["e_min"], re_["e_min"])
+    assert got[0]["converged"].all() and got[1]["converged"].all()
+    assert (got[0]["icg"] < got[1]["icg"]).all()
+    assert (got[2]["icg"] == 11).all() and not got[2]["converged"].any()

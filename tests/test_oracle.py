@@ -92,6 +92,28 @@ def test_lattice_matches_reference_build(sparse, eta):
     assert O.icg_from_log(log) == [r["icg"]]
 
 
+@pytest.mark.skipif(not os.path.exists(O.REF_MM), reason="reference build oracle/_ref/ref_mm not present")
+@pytest.mark.parametrize("opA,opB", [("n", "n"), ("t", "n"), ("n", "t"), ("t", "t")])
+def test_sddmm_csc_matches_reference_build_bitwise(opA, opB):
+    """or_sddmm_csc against mm_multiply_sddensddensdcscref (MatrixSwitch_mm_multiply.F90:334-398),
+    dense x dense -> 'sdcsc', with beta applied to the stored values first."""
+    r = np.random.default_rng(11)
+    m, n, k = 37, 29, 23
+    A = r.standard_normal((k, m) if opA == "t" else (m, k))
+    B = r.standard_normal((n, k) if opB == "t" else (k, n))
+    C0 = r.standard_normal((m, n)) * (r.random((m, n)) < 0.3)
+    C0[:, 5] = 0.0  # an empty column
+    cp, ri, ref_val = O.run_ref_sddmm(opA, opB, A, B, C0, 0.7, -1.3)
+    cval0 = np.concatenate([C0[ri[cp[j]:cp[j + 1]], j] for j in range(n)])
+    got = O.sddmm_csc(opA, opB, A, B, cp, ri, cval0, 0.7, -1.3)
+    assert np.array_equal(got, ref_val)
+    # the same numbers from dense algebra (summation order aside)
+    P = 0.7 * ((A.T if opA == "t" else A) @ (B.T if opB == "t" else B)) - 1.3 * C0
+    dense = np.concatenate([P[ri[cp[j]:cp[j + 1]], j] for j in range(n)])
+    assert np.allclose(got, dense, rtol=1e-13, atol=1e-13)
+    assert cp[6] == cp[5]
+
+
 @needs_ref
 def test_no_S_matches_reference_build():
     nb, H, _ = small_lattice(2, seed=9)
