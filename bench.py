@@ -11,14 +11,18 @@ the fused line-search reduction, the quartic solve and the vector updates).  Inp
 in HBM before timing.  `value` = FP64 CG iterations/s over exactly --steps complete iterations,
 timed between the stream-idle points that end the coefficient readbacks of CG steps 1 and
 steps+1 of one omm() call.
-The same measurement is repeated with BF16 and FP32 products ('hbf', 'h32'); --ttc adds the
-time-to-converge (|dE|/E <= 1e-10) of the FP64 and the mixed ('hmx': BF16 phase + FP64
-refinement) solves with the energy and density-matrix errors of the mixed run.
+The same measurement is repeated with BF16 and FP32 products ('hbf', 'h32').  Then (unless
+--no-ttc) the time-to-converge (the reference's e_diff <= 1e-10) of the FP64 and the mixed
+('hmx': BF16 phase + FP64 refinement) solves from the same C0, with the energy and density-matrix
+errors of the mixed run against FP64 and the exact eigenvalue sum (ttc_* keys); and the same pair
+on the parameterised lattice of the same size with a wide gap, where both solves reach the exact
+sum (gapped_ttc_* keys).
 
 Roofline: the dominant kernel is the FP64 MFMA GEMM (Gram and gradient products); `achieved` =
 algorithmic flops of all its launches in the timed call / their summed HIP-event durations on the
-library stream.  The CPU baseline times the oracle (oracle/omm_oracle.c, the reference algorithm
-restated in C, 'ref' operation order, OpenMP) on a row sample of one CG iteration's products.
+library stream.  The CPU baseline times the reference's 'lap' path (host BLAS dgemm for the dense
+products, the CSC product in 'ref' order) on a row sample of one CG iteration's products, with
+the oracle's 'ref' loops for every product beside it (cpu_baseline.ref_loops).
 """
 import argparse
 import ctypes
@@ -122,11 +126,40 @@ def cpu_baseline(nb, no, seconds_hint=20.0):
     t_spmm = (time.perf_counter() - t) * frac
     # per iteration: 4 Gram + 2 gradient GEMMs + 2 SpMMs (element-wise passes and traces not counted)
     t_iter = 4 * t_gram + 2 * t_grad + 2 * t_spmm
-    return {"value": 1.0 / t_iter, "unit": "CG iters/s", "cores": O.num_threads(), "kind": "port",
-            "sample": (f"oracle (C restatement of omm()/MatrixSwitch 'ref', OpenMP) on rows 1..{rs} of one Gram "
-                       f"(x4), one gradient GEMM (x2) and one BSR SpMM (x2) of a CG iteration at n={nb}, "
-                       f"scaled by {frac}; element-wise passes and traces excluded (favours the CPU)"),
-            "seconds_per_iter": t_iter}
+    ref_loops = {"value": 1.0 / t_iter, "unit": "CG iters/s", "cores": O.num_threads(), "kind": "port",
+                 "sample": (f"oracle (C restatement of omm()/MatrixSwitch 'ref', OpenMP) on rows 1..{rs} of one "
+                            f"Gram (x4), one gradient GEMM (x2) and one CSC SpMM (x2) of a CG iteration at n={nb}, "
+                            f"scaled by {frac}; element-wise passes and traces excluded (favours the CPU)"),
+                 "seconds_per_iter": t_iter}
+    # the reference's 'lap' operation label: the dense products are dgemm calls of the host BLAS
+    # (MatrixSwitch.F90:1303); the CSC product has no BLAS form and stays the 'ref' loop
+    # (MatrixSwitch_mm_multiply.F90:263-327), timed above
+    from scipy.linalg import blas
+    from threadpoolctl import threadpool_info
+
+    bl = [x for x in threadpool_info() if x.get("user_api") == "blas"]
+    blas_name = (f"{bl[0].get('internal_api', 'blas')} {bl[0].get('version', '')} (scipy's)" if bl else "scipy BLAS")
+    blas_threads = int(bl[0].get("num_threads", 1)) if bl else 1
+    lfrac = 4
+    lr = max(1, no // lfrac)
+    A2 = np.asfortranarray(rng.standard_normal((lr, nb)))
+    W2 = np.asfortranarray(rng.standard_normal((lr, no)))
+    G2 = np.zeros((lr, nb), order="F")
+    blas.dgemm(1.0, A2[:64], B, trans_b=True)  # thread pool start-up outside the timing
+    t = time.perf_counter()
+    blas.dgemm(1.0, A2, B, trans_b=True)  # Gram sample: (no/4 x nb)(nb x no)
+    l_gram = (time.perf_counter() - t) * lfrac
+    t = time.perf_counter()
+    blas.dgemm(-2.0, W2, B, beta=1.0, c=G2, overwrite_c=True)  # gradient sample: (no/4 x no)(no x nb)
+    l_grad = (time.perf_counter() - t) * lfrac
+    l_iter = 4 * l_gram + 2 * l_grad + 2 * t_spmm
+    return {"value": 1.0 / l_iter, "unit": "CG iters/s", "cores": blas_threads, "kind": "port",
+            "sample": (f"the reference's 'lap' path on the host: dgemm of {blas_name} on rows 1..{lr} of one Gram (x4) "
+                       f"and one gradient product (x2), scaled by {lfrac}, plus the CSC SpMM in 'ref' order (x2, the "
+                       f"oracle, {O.num_threads()} threads) of a CG iteration at n={nb}; element-wise passes and "
+                       f"traces excluded (favours the CPU)"),
+            "blas": blas_name, "seconds_per_iter": l_iter, "gram_TFLOPs": 2.0 * no * no * nb / l_gram / 1e12,
+            "ref_loops": ref_loops}
 
 
 def bsr_to_csc_fast(nb, cp, ri, bv):
@@ -158,7 +191,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--L", type=int, default=16, help="lattice edge: n = 4 L^3 basis functions, m = n/2")
     ap.add_argument("--eta", type=float, default=None)
-    ap.add_argument("--ttc", action="store_true", help="also measure time-to-converge (FP64 and mixed)")
+    ap.add_argument("--ttc", action="store_true", help="(default) time-to-converge of the FP64 and mixed solves")
+    ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge solves")
+    ap.add_argument("--no-gapped", action="store_true",
+                    help="skip the time-to-converge pair on the gapped parameterised lattice of the same size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true", help="skip the BF16 / FP32 step-rate runs")
     ap.add_argument("--compute-eta", type=int, default=None, help="print the mid-gap shift for lattice edge L")
@@ -272,7 +308,8 @@ def main():
     coeff_gbs = coeff_by / (coeff_ms * 1e-3) / 1e9
 
     line = {
-        "metric": f"OMM CG iterations/s (FP64, n={nb}, m={no}); time-to-converge |dE|/E<1e-10 with --ttc",
+        "metric": (f"OMM time-to-converge (|\u0394E|/E<1e-10) & CG iters/s, n={nb}, mixed vs FP64; value = FP64 CG "
+                   f"iters/s, time-to-converge in ttc_*"),
         "value": round(ips, 4),
         "unit": "CG iters/s",
         "n_gpus": 1,
@@ -315,11 +352,29 @@ def main():
 
     if "mixed_lowp_ips" in res:
         line["mixed_low_precision_iters_per_s"] = round(res["mixed_lowp_ips"], 4)
-    if args.ttc:
-        line["time_to_converge"] = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log,
-                                                    variants=("mixed",)
-                                                    + (("mixed_fp32_gradient",) if args.ttc_fp32_gradient else ())
-                                                    + (("mixed_state_bf16x3",) if args.ttc_state_bf16x3 else ()))
+    if not args.no_ttc:
+        tt = time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, log_path=args.ttc_log,
+                              variants=("mixed",)
+                              + (("mixed_fp32_gradient",) if args.ttc_fp32_gradient else ())
+                              + (("mixed_state_bf16x3",) if args.ttc_state_bf16x3 else ()))
+        line["time_to_converge"] = tt
+        line.update(ttc_summary(tt, ""))
+        gexact = exact_eigs(nb, f"theta0_L{L}") if isinstance(L, int) else None
+        if gexact is not None and not args.no_gapped:
+            # the same solves on the parameterised lattice at theta0 (one shared hopping block,
+            # onsite levels -6, -6, 6, 6): the gap at half filling is 3.9 instead of 3e-4, so both
+            # reach the exact eigenvalue sum
+            geta = gexact["eta_mid"]
+            cpg, rig, hg = oh.lattice_bsr_theta(L, oh.lattice_theta0())
+            assert np.array_equal(cpg, Sb[0]) and np.array_equal(rig, Sb[1])
+            Hg = F.Mat.from_bsr(nb, cpg, rig, hg - geta * Sb[2])
+            log(f"bench: gapped lattice theta0 eta={geta:.6f} gap={gexact['gap_at_m']:.3f}")
+            tg = time_to_converge(F, oh, ctxh, (Hg, S, Cm), nb, no, geta, C0, exact_key=f"theta0_L{L}")
+            tg["workload"] = (f"omm flavour 0, parameterised {L}x{L}x{L} lattice (theta0: shared hopping block, "
+                              f"onsite -6,-6,6,6), n={nb}, m={no}, eta={geta:.6f} (mid-gap)")
+            line["time_to_converge_gapped"] = tg
+            line.update(ttc_summary(tg, "gapped_"))
+            Hg.free()
 
     if not args.no_cpu:
         t = time.time()
@@ -329,6 +384,21 @@ def main():
         M.free()
     print(json.dumps(line), flush=True)
     return 0
+
+
+def ttc_summary(tt, prefix):
+    """Flat time-to-converge figures of one FP64/mixed pair for the bench line."""
+    mx = tt["mixed"]
+    out = {f"{prefix}ttc_fp64_s": round(tt["fp64"]["seconds"], 3), f"{prefix}ttc_mixed_s": round(mx["seconds"], 3),
+           f"{prefix}ttc_speedup": round(mx["speedup_vs_fp64"], 3),
+           f"{prefix}ttc_icg_fp64": tt["fp64"]["icg"], f"{prefix}ttc_icg_mixed": mx["icg"],
+           f"{prefix}mixed_energy_rel_err_vs_fp64": mx["energy_rel_err_vs_fp64"],
+           f"{prefix}mixed_density_rel_fro_err_vs_fp64": mx["density_rel_fro_err_vs_fp64"],
+           f"{prefix}ttc_converged": bool(tt["fp64"]["converged"] and mx["converged"])}
+    if "exact" in tt:
+        out[f"{prefix}fp64_energy_rel_err_vs_exact"] = tt["exact"]["fp64_energy_rel_err"]
+        out[f"{prefix}mixed_energy_rel_err_vs_exact"] = mx["energy_rel_err_vs_exact"]
+    return out
 
 
 EIGS_FILE = "tests/golden/lattice_eigs.json"
@@ -346,18 +416,23 @@ def profiled_traffic(rel):
     return None
 
 
-def exact_eigs(nb):
-    """Tabulated exact eigenvalue data of the seeded lattice with nb basis functions, or None."""
+def exact_eigs(nb, key=None):
+    """Tabulated exact eigenvalue data of the seeded lattice with nb basis functions (or the entry
+    `key`, e.g. 'theta0_L16' for the parameterised lattice), or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), EIGS_FILE)
     if not os.path.exists(path):
         return None
-    for v in json.load(open(path)).values():
-        if v.get("n") == nb:
+    tab = json.load(open(path))
+    if key is not None:
+        return tab.get(key)
+    for k, v in tab.items():
+        if v.get("n") == nb and not k.startswith("theta"):
             return v
     return None
 
 
-def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None, variants=("mixed",)):
+def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50, log_path=None, variants=("mixed",),
+                     exact_key=None):
     """FP64 omm() and the mixed-precision variants ("mixed": the default BF16x3 gradient,
     "mixed_fp32_gradient": FP32 gradient, "mixed_state_bf16x3": BF16x3 periodic state recomputation)
     from the same C0 to the reference's e_diff <= tol;
@@ -378,7 +453,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
     p1, ld = Dref.dptr()
     fro_ref = ctypes.c_double(0)
     oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p1, ld, p1, ld, ctypes.byref(fro_ref)), "fro")
-    exact = exact_eigs(nb)
+    exact = exact_eigs(nb, exact_key)
     if exact is not None:
         out["exact"] = {"sum_lowest_m": exact["sum_lowest_m"], "gap_at_m": exact["gap_at_m"], "source": EIGS_FILE,
                         "fp64_energy_rel_err": abs(r["e_min"] - exact["sum_lowest_m"]) / abs(exact["sum_lowest_m"])}
@@ -402,6 +477,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
         oh.check(oh.lib().omx_mm_trace(ctxh, oh.OMX_F64, nb, nb, p2, ld, p2, ld, ctypes.byref(fro)), "fro")
         v = {"seconds": tmx, "icg": rm["icg"], "icg_low_precision": rm["icg_lowp"],
              "icg_fp64": rm["icg"] - rm["icg_lowp"], "e_min": rm["e_min"], "converged": rm["converged"],
+             "refine_gradient_start": rm["refine_g0"], "refine_gradient_end": rm["refine_gend"],
              "gradient": "fp32" if name == "mixed_fp32_gradient" else "bf16x3",
              "state_recompute": "bf16x3" if name == "mixed_state_bf16x3" else "fp32",
              "energy_rel_err_vs_fp64": abs(rm["e_min"] - r["e_min"]) / abs(r["e_min"]),

