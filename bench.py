@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--eta", type=float, default=None)
     ap.add_argument("--ttc", action="store_true", help="(default) time-to-converge of the FP64 and mixed solves")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge solves")
+    ap.add_argument("--c0-seed", type=int, default=None,
+                    help="seed of the initial coefficients (default: the generator's default seed)")
     ap.add_argument("--no-gapped", action="store_true",
                     help="skip the time-to-converge pair on the gapped parameterised lattice of the same size")
     ap.add_argument("--no-cpu", action="store_true")
@@ -215,6 +217,13 @@ def main():
     ap.add_argument("--group-steps", type=int, default=3, help="CG steps timed in --group mode")
     ap.add_argument("--group-count", type=int, default=24, help="systems in --group mode")
     ap.add_argument("--dims", default=None, help="non-cubic lattice Lx,Ly,Lz (overrides --L), e.g. 32,32,16")
+    ap.add_argument("--kernel", choices=["gram", "spmm", "pair"], default=None,
+                    help="counter mode: --launches launches of one kernel at the workload's shape (the FP64 Gram "
+                         "HW = HC*C^T, the BSR SpMM Y = X*H, the calc_G pair), then 3 plain copies of a no x n matrix "
+                         "that calibrate FETCH_SIZE / WRITE_SIZE against known bytes")
+    ap.add_argument("--launches", type=int, default=3, help="launches in --kernel mode")
+    ap.add_argument("--kernel-prec", choices=["f64", "f32", "bf16", "bf16x3"], default="f64",
+                    help="product precision in --kernel mode (all but f64 on FP32 storage)")
     ap.add_argument("--mixed-steps", type=int, default=0,
                     help="also time this many low-precision steps of the mixed (hmx) solve, FP32 storage")
     args = ap.parse_args()
@@ -240,6 +249,9 @@ def main():
     if args.la:
         print(json.dumps(la_bench(oh, args.L)), flush=True)
         return 0
+    if args.kernel:
+        print(json.dumps(kernel_mode(oh, args.kernel, args.L, args.launches, args.kernel_prec)), flush=True)
+        return 0
 
     if args.dims:
         dims = tuple(int(x) for x in args.dims.split(","))
@@ -260,7 +272,7 @@ def main():
     # H - eta*S on the union pattern (same pattern: onsite + 6 neighbour blocks)
     assert np.array_equal(Hb[0], Sb[0]) and np.array_equal(Hb[1], Sb[1])
     Hs = (Hb[0], Hb[1], Hb[2] - eta * Sb[2])
-    C0 = oh.random_c0(no, nb)
+    C0 = oh.random_c0(no, nb) if args.c0_seed is None else oh.random_c0(no, nb, args.c0_seed)
     log(f"bench: inputs generated in {time.time() - t:.1f}s")
 
     F.setup(0)
@@ -323,7 +335,8 @@ def main():
         "data": "synthetic (seeded lattice generator)",
         "config": {"workload": f"omm flavour 0, {'x'.join(map(str, dims))} lattice, n={nb}, m={no}, eta={eta:.6f}"
                                + (" (mid-gap)" if L in ETA_MID_GAP else ""),
-                   "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip"},
+                   "n": nb, "m": no, "nnz_H": 16 * nnzb, "storage": "gdden/gdcsc", "operation": "hip",
+                   "c0_seed": args.c0_seed if args.c0_seed is not None else oh.DEFAULT_SEED},
         "roofline": {"bound": "mfma",
                      "kernel": "FP64 Gram and state products (k_dgemm_dl direct-to-LDS; the symmetric calc_AW Grams on their "
                                "lower-triangle tiles; v_mfma_f64_16x16x4f64, 128x128 tiles)",
@@ -501,6 +514,58 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
 
 
 GROUP_ETA = 0.0517
+
+
+def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3):
+    """Fixed launch sequences of one library kernel at the L^3 workload's shape (X, Y no x n),
+    followed by `copies` device copies of a no x n FP64 matrix (no*n*8 bytes read and written per
+    copy): under rocprofv3 --pmc the copies' counter totals give the scale between a counter and
+    bytes, the kernel's dispatches then convert to HBM bytes per launch.  prec other than f64 runs
+    the product on FP32 storage (f32, bf16 or bf16x3 products)."""
+    nb = 4 * L ** 3
+    no = nb // 2
+    dt = oh.OMX_F64 if prec == "f64" else oh.OMX_F32
+    pr = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3}[prec]
+    es = 8 if prec == "f64" else 4
+    ctx = oh.Ctx(0)
+    mats = []
+    rng = np.random.default_rng(0)
+    if kernel == "spmm":
+        cp, ri, bv = oh.lattice_bsr(L, 0)
+        H = oh.Bsr(ctx, nb, cp, ri, bv)
+        X, Y = oh.DMat(ctx, no, nb, dt), oh.DMat(ctx, no, nb, dt)
+        mats += [X, Y]
+        run = lambda: oh.spmm(ctx, X, H, Y)  # noqa: E731
+        work = {"bytes_per_launch": 2.0 * no * nb * es + len(ri) * (128 + 4) + 4 * (nb // 4 + 1)}
+    elif kernel == "gram":
+        A, B, G = oh.DMat(ctx, no, nb, dt), oh.DMat(ctx, no, nb, dt), oh.DMat(ctx, no, no, dt)
+        mats += [A, B, G]
+        run = lambda: oh.gemm(ctx, "n", "t", A, B, G, prec=pr)  # noqa: E731
+        work = {"flops_per_launch": 2.0 * no * no * nb, "bytes_per_launch": 2.0 * no * nb * es + no * no * es}
+    else:
+        W1, W2 = oh.DMat(ctx, no, no, dt), oh.DMat(ctx, no, no, dt)
+        HC, SC, G = (oh.DMat(ctx, no, nb, dt) for _ in range(3))
+        mats += [W1, W2, HC, SC, G]
+        run = lambda: oh.omm_calc_G(ctx, W1, W2, HC, SC, G, prec=pr)  # noqa: E731
+        work = {"flops_per_launch": 4.0 * no * no * nb}
+    for M in mats:  # finite operands (the values do not change the launch sequence)
+        M.upload(np.asfortranarray(rng.standard_normal((M.rows, M.cols)) * 1e-2))
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(launches):
+        run()
+    ctx.sync()
+    dt = (time.perf_counter() - t) / launches
+    Sx, Dx = oh.DMat(ctx, no, nb), oh.DMat(ctx, no, nb)
+    for _ in range(copies):
+        oh.add(ctx, "n", Sx, Dx, 1.0, 0.0)
+    ctx.sync()
+    out = {"mode": "kernel", "kernel": kernel, "prec": prec, "L": L, "n": nb, "m": no, "launches": launches,
+           "wall_ms_per_launch": dt * 1e3, "copy_bytes_read": no * nb * 8, "copy_bytes_written": no * nb * 8}
+    out.update(work)
+    for M in mats + [Sx, Dx]:
+        M.free()
+    return out
 
 
 def group_bench(oh, F, steps, count, seed=None, stream=6):

@@ -122,6 +122,12 @@ int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, 
  */
 int omx_bsr_from_csc(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
                      int rowbase, omx_bsr *out);
+/* The host half of omx_bsr_from_csc / omx_bsr_from_csc_z (no device, no context): the BSR-4 block
+ * columns of a CSC matrix.  colptr must start at ptrbase and never decrease (checked before any
+ * entry is read).  Call with null outputs to get *nnzb, then with bcolptr[ceil(nb/4)+1],
+ * browidx[*nnzb] and bval[*nnzb * (is_complex ? 32 : 16)]. */
+int omx_csc_to_bsr(int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase, int rowbase,
+                   int is_complex, int64_t *nnzb, int *bcolptr, int *browidx, double *bval);
 /* Complex (k-point) operators: 32 doubles per block, (re, im) interleaved, column-major inside
  * the block; CSC values (re, im) interleaved per entry.  Same conventions as the real calls
  * (szcsc / pzcsc storage, MatrixSwitch_m_copy.F90, pspMat.F90). */

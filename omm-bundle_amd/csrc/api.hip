@@ -378,37 +378,61 @@ int omx_bsr_create_z(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx
     return bsr_create_common(ctx, nb, bcolptr, browidx, bval, 1, out);
 }
 
-int omx_bsr_from_csc_z(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
-                       int rowbase, omx_bsr *out) {
-    OMX_REQUIRE(ctx && out && colptr && nb > 0, "omx_bsr_from_csc_z: bad arguments");
-    OMX_REQUIRE((ptrbase == 0 || ptrbase == 1) && (rowbase == 0 || rowbase == 1), "omx_bsr_from_csc_z: bad base");
-    int nbc = (nb + 3) / 4;
-    int64_t nnz = (int64_t)colptr[nb] - ptrbase;
-    OMX_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val)), "omx_bsr_from_csc_z: bad nnz");
-    std::vector<int> cp(nbc + 1, 0), ri;
-    std::vector<double> bv;
+// CSC (colptr with base ptrbase, row indices with base rowbase; values (re, im) interleaved when
+// cplx) to BSR-4 block columns: block rows ascending, duplicates added, empty blocks not stored.
+// The column pointers are checked before any index or value is read: colptr[0] == ptrbase and
+// non-decreasing, so every entry offset lies in [0, nnz).
+static int csc_to_bsr(int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase, int rowbase,
+                      bool cplx, std::vector<int> &cp, std::vector<int> &ri, std::vector<double> &bv,
+                      const char *who) {
+    OMX_REQUIRE(colptr && nb > 0, "%s: bad arguments", who);
+    OMX_REQUIRE((ptrbase == 0 || ptrbase == 1) && (rowbase == 0 || rowbase == 1), "%s: bad base", who);
+    OMX_REQUIRE(colptr[0] == ptrbase, "%s: colptr[0] is %d, expected %d", who, colptr[0], ptrbase);
+    for (int j = 0; j < nb; ++j)
+        OMX_REQUIRE(colptr[j + 1] >= colptr[j], "%s: colptr decreases at column %d", who, j);
+    const int64_t nnz = (int64_t)colptr[nb] - ptrbase;
+    OMX_REQUIRE(nnz == 0 || (rowidx && val), "%s: null index or value array", who);
+    const int nbc = (nb + 3) / 4;
+    const int bs = cplx ? 32 : 16;
+    cp.assign(nbc + 1, 0);
+    ri.clear();
+    bv.clear();
     for (int J = 0; J < nbc; ++J) {
         std::map<int, std::vector<std::pair<int, int64_t>>> blocks;  // I -> (r + 4c, entry)
         for (int c = 0; c < 4; ++c) {
-            int j = 4 * J + c;
+            const int j = 4 * J + c;
             if (j >= nb) break;
             for (int64_t l = colptr[j] - ptrbase; l < colptr[j + 1] - ptrbase; ++l) {
-                int i = rowidx[l] - rowbase;
-                OMX_REQUIRE(i >= 0 && i < nb, "omx_bsr_from_csc_z: row index %d out of range", i + rowbase);
+                const int i = rowidx[l] - rowbase;
+                OMX_REQUIRE(i >= 0 && i < nb, "%s: row index %d out of range", who, i + rowbase);
                 blocks[i / 4].push_back({(i % 4) + 4 * c, l});
             }
         }
         for (auto &kv : blocks) {
             ri.push_back(kv.first);
-            size_t base = bv.size();
-            bv.resize(base + 32, 0.0);
+            const size_t base = bv.size();
+            bv.resize(base + bs, 0.0);
             for (auto &e : kv.second) {
-                bv[base + 2 * e.first] += val[2 * e.second];
-                bv[base + 2 * e.first + 1] += val[2 * e.second + 1];
+                if (cplx) {
+                    bv[base + 2 * e.first] += val[2 * e.second];
+                    bv[base + 2 * e.first + 1] += val[2 * e.second + 1];
+                } else {
+                    bv[base + e.first] += val[e.second];
+                }
             }
         }
         cp[J + 1] = (int)ri.size();
     }
+    return OMX_OK;
+}
+
+int omx_bsr_from_csc_z(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
+                       int rowbase, omx_bsr *out) {
+    OMX_REQUIRE(ctx && out, "omx_bsr_from_csc_z: bad arguments");
+    std::vector<int> cp, ri;
+    std::vector<double> bv;
+    int rc = csc_to_bsr(nb, colptr, rowidx, val, ptrbase, rowbase, 1, cp, ri, bv, "omx_bsr_from_csc_z");
+    if (rc) return rc;
     return bsr_upload(ctx, nb, cp, ri, bv.data(), 1, out);
 }
 
@@ -432,34 +456,26 @@ int omx_bsr_create(omx_ctx ctx, int nb, const int *bcolptr, const int *browidx, 
 
 int omx_bsr_from_csc(omx_ctx ctx, int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase,
                      int rowbase, omx_bsr *out) {
-    OMX_REQUIRE(ctx && out && colptr && nb > 0, "omx_bsr_from_csc: bad arguments");
-    OMX_REQUIRE((ptrbase == 0 || ptrbase == 1) && (rowbase == 0 || rowbase == 1), "omx_bsr_from_csc: bad base");
-    int nbc = (nb + 3) / 4;
-    int64_t nnz = (int64_t)colptr[nb] - ptrbase;
-    OMX_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val)), "omx_bsr_from_csc: bad nnz");
-    // Per block column: ordered map block row -> block index.
-    std::vector<int> cp(nbc + 1, 0), ri;
+    OMX_REQUIRE(ctx && out, "omx_bsr_from_csc: bad arguments");
+    std::vector<int> cp, ri;
     std::vector<double> bv;
-    for (int J = 0; J < nbc; ++J) {
-        std::map<int, std::vector<std::pair<int, double>>> blocks;  // I -> (r + 4c, value)
-        for (int c = 0; c < 4; ++c) {
-            int j = 4 * J + c;
-            if (j >= nb) break;
-            for (int64_t l = colptr[j] - ptrbase; l < colptr[j + 1] - ptrbase; ++l) {
-                int i = rowidx[l] - rowbase;
-                OMX_REQUIRE(i >= 0 && i < nb, "omx_bsr_from_csc: row index %d out of range", i + rowbase);
-                blocks[i / 4].push_back({(i % 4) + 4 * c, val[l]});
-            }
-        }
-        for (auto &kv : blocks) {
-            ri.push_back(kv.first);
-            size_t base = bv.size();
-            bv.resize(base + 16, 0.0);
-            for (auto &e : kv.second) bv[base + e.first] += e.second;
-        }
-        cp[J + 1] = (int)ri.size();
-    }
+    int rc = csc_to_bsr(nb, colptr, rowidx, val, ptrbase, rowbase, 0, cp, ri, bv, "omx_bsr_from_csc");
+    if (rc) return rc;
     return bsr_upload(ctx, nb, cp, ri, bv.data(), 0, out);
+}
+
+int omx_csc_to_bsr(int nb, const int *colptr, const int *rowidx, const double *val, int ptrbase, int rowbase,
+                   int is_complex, int64_t *nnzb, int *bcolptr, int *browidx, double *bval) {
+    OMX_REQUIRE(nnzb, "omx_csc_to_bsr: bad arguments");
+    std::vector<int> cp, ri;
+    std::vector<double> bv;
+    int rc = csc_to_bsr(nb, colptr, rowidx, val, ptrbase, rowbase, is_complex != 0, cp, ri, bv, "omx_csc_to_bsr");
+    if (rc) return rc;
+    *nnzb = (int64_t)ri.size();
+    if (bcolptr) std::copy(cp.begin(), cp.end(), bcolptr);
+    if (browidx) std::copy(ri.begin(), ri.end(), browidx);
+    if (bval) std::copy(bv.begin(), bv.end(), bval);
+    return OMX_OK;
 }
 
 int omx_bsr_destroy(omx_bsr a) {

@@ -67,12 +67,16 @@ module MatrixSwitch
   interface ms_upload
     module procedure ms_upload_d, ms_upload_z
   end interface
+  interface m_register_pcsc
+    module procedure m_register_pdcsc
+  end interface
   interface ms_download
     module procedure ms_download_d, ms_download_z
   end interface
 
   public :: m_allocate, m_deallocate, m_copy, mm_multiply, m_add, m_trace, mm_trace, m_scale, m_set
   public :: m_set_element, m_get_element, m_register_sden, m_register_gbsr, m_register_gbsr_z, m_set_walk
+  public :: m_register_pcsc, m_convert
   public :: ms_hip_setup, ms_hip_finalize, ms_hip_sync, ms_hip_ctx, ms_upload, ms_download, ms_convert
   public :: ms_label_prec, die
 
@@ -370,6 +374,64 @@ contains
     call chk(rc, 'm_register_gbsr')
     m_name%is_initialized = .true.
   end subroutine m_register_gbsr
+
+  !> Register a compressed-sparse-column matrix in pspBLAS's convention (MatrixSwitch_m_register.F90:
+  !> 394-443 -> psp_register_dspm, pspMat.F90:87-140): idx1 = row indices (1-based), idx2 = column
+  !> pointers (idx2(1) = 1, idx2(j+1)-idx2(j) entries in column j), val = the values, desc = the BLACS
+  !> descriptor (desc(3), desc(4) = global rows and columns).  As in pspBLAS the data are copied, here
+  !> into HBM as the device operator of 'gdcsc' storage (BSR-4; the pattern's blocks, duplicates
+  !> added).  One process holds the whole matrix, so the local dimensions are the global ones.
+  subroutine m_register_pdcsc(m_name, idx1, idx2, val, desc)
+    type(matrix), intent(inout) :: m_name
+    integer, intent(in), target :: desc(9)
+    integer, intent(in), target :: idx1(:)
+    integer, intent(in), target :: idx2(:)
+    real(dp), intent(in), target :: val(:)
+    integer(c_int), allocatable, target :: cp(:), ri(:)
+    real(dp), allocatable, target :: cv(:)
+    integer :: nb, nnz
+    integer(c_int) :: rc
+    if (desc(3) /= desc(4)) call die('m_register_pcsc: device sparse storage needs a square operator')
+    nb = desc(4)
+    if (size(idx2) < nb + 1) call die('m_register_pcsc: idx2 holds fewer than desc(4)+1 column pointers')
+    nnz = idx2(nb+1) - 1
+    if (nnz < 0 .or. nnz > size(idx1) .or. nnz > size(val)) call die('m_register_pcsc: idx2 does not match idx1/val')
+    allocate(cp(nb+1), ri(max(nnz,1)), cv(max(nnz,1)))
+    cp = int(idx2(1:nb+1), c_int)
+    if (nnz > 0) then
+      ri(1:nnz) = int(idx1(1:nnz), c_int)
+      cv(1:nnz) = val(1:nnz)
+    end if
+    m_name%str_type = 'csc'
+    m_name%is_serial = .true.
+    m_name%is_real = .true.
+    m_name%is_sparse = .true.
+    m_name%is_device = .true.
+    m_name%dtype = OMX_F64
+    m_name%dim1 = desc(3)
+    m_name%dim2 = desc(4)
+    m_name%is_square = .true.
+    rc = omx_bsr_from_csc(ctx(), int(nb, c_int), c_loc(cp), c_loc(ri), c_loc(cv), 1_c_int, 1_c_int, m_name%bsr)
+    call chk(rc, 'm_register_pcsc')
+    m_name%is_initialized = .true.
+  end subroutine m_register_pdcsc
+
+  !> Change a matrix's storage format in place (MatrixSwitch.F90:987-1041): m_copy to a temporary
+  !> with the new label (and threshold), then m_deallocate.  The conversions are those of m_copy:
+  !> dense host <-> device, FP64 <-> FP32 on the device, dense -> sparse ('sdcsc', 'gdcsc') with
+  !> |a| > threshold.  The reference copies the temporary back and frees it; here m_name takes over
+  !> the temporary's storage (its components are pointers and handles), which gives the same result
+  !> without a second copy, also for a device operator, which m_copy cannot take as a source.
+  subroutine m_convert(m_name, label, threshold, threshold_is_soft)
+    type(matrix), intent(inout) :: m_name
+    character(5), intent(in), optional :: label
+    real(dp), intent(in), optional :: threshold
+    logical, intent(in), optional :: threshold_is_soft
+    type(matrix) :: temp_matrix
+    call m_copy(temp_matrix, m_name, label, threshold, threshold_is_soft)
+    call m_deallocate(m_name)
+    m_name = temp_matrix
+  end subroutine m_convert
 
   !> Schedule hint for a real device operator ('gdcsc'): the order in which products visit its
   !> block columns (0-based permutation of the nb/4 block columns), taken `group` at a time with

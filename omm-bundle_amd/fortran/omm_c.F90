@@ -94,6 +94,41 @@ contains
     omm_c_mat_alloc = h
   end function omm_c_mat_alloc
 
+  !> Sparse device operator registered in pspBLAS's convention through m_register_pcsc: 1-based
+  !> row indices idx1(nnz) and column pointers idx2(nb+1) (idx2(1) = 1), values val(nnz).
+  integer(c_int) function omm_c_mat_register_pcsc(nb, nnz, idx1, idx2, val) bind(c, name='omm_c_mat_register_pcsc')
+    integer(c_int), value :: nb, nnz
+    integer(c_int), intent(in) :: idx1(*), idx2(*)
+    real(c_double), intent(in) :: val(*)
+    integer :: h, desc(9)
+    h = new_handle()
+    if (h < 0) then
+      omm_c_mat_register_pcsc = -1
+      return
+    end if
+    ! BLACS descriptor of one process holding the whole nb x nb matrix
+    desc = (/1, 0, int(nb), int(nb), int(nb), int(nb), 0, 0, int(nb)/)
+    call m_register_pcsc(table(h), idx1(1:nnz), idx2(1:nb+1), val(1:nnz), desc)
+    omm_c_mat_register_pcsc = h
+  end function omm_c_mat_register_pcsc
+
+  !> m_convert of a handle's matrix to a new 5-character label (threshold < 0: none).
+  integer(c_int) function omm_c_mat_convert(h, label, threshold) bind(c, name='omm_c_mat_convert')
+    integer(c_int), value :: h
+    character(kind=c_char), intent(in) :: label(*)
+    real(c_double), value :: threshold
+    if (.not. valid(h)) then
+      omm_c_mat_convert = -1
+      return
+    end if
+    if (threshold >= 0.0_dp) then
+      call m_convert(table(h), cstr(label, 5), threshold)
+    else
+      call m_convert(table(h), cstr(label, 5))
+    end if
+    omm_c_mat_convert = 0
+  end function omm_c_mat_convert
+
   !> Sparse operator from a 0-based CSC (colptr nb+1, rowidx, val); label 'gdcsc' or 'sdcsc'.
   integer(c_int) function omm_c_mat_from_csc(nb, colptr, rowidx, val, label) bind(c, name='omm_c_mat_from_csc')
     integer(c_int), value :: nb

@@ -43,6 +43,7 @@ SIGNATURES = {
     "omx_convert": (_i, [_vp, _i, _i, _vp, _i, _i64, _vp, _i, _i64]),
     "omx_bsr_create": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(_vp)]),
     "omx_bsr_from_csc": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, C.POINTER(_vp)]),
+    "omx_csc_to_bsr": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, C.POINTER(_i64), _vp, _vp, _vp]),
     "omx_bsr_destroy": (_i, [_vp]),
     "omx_bsr_info": (_i, [_vp, _pi, C.POINTER(_i64), _pi]),
     "omx_bsr_update_values": (_i, [_vp, _vp]),
@@ -150,6 +151,22 @@ def device_count() -> int:
 
 DEFAULT_SEED = 0x0EEDC0DE
 THETA_COUNT = 20
+
+
+def csc_to_bsr(nb, colptr, rowidx, val, ptrbase=0, rowbase=0, is_complex=False):
+    """BSR-4 block columns (bcolptr, browidx, bval) of a CSC matrix, on the host (omx_csc_to_bsr)."""
+    colptr = np.ascontiguousarray(colptr, np.int32)
+    rowidx = np.ascontiguousarray(rowidx, np.int32)
+    val = np.ascontiguousarray(val, np.complex128 if is_complex else np.float64)
+    nnzb = C.c_int64(0)
+    check(lib().omx_csc_to_bsr(nb, _ptr(colptr), _ptr(rowidx), _ptr(val), ptrbase, rowbase, int(is_complex),
+                               C.byref(nnzb), None, None, None), "omx_csc_to_bsr")
+    cp = np.zeros((nb + 3) // 4 + 1, np.int32)
+    ri = np.zeros(nnzb.value, np.int32)
+    bv = np.zeros((nnzb.value, 32 if is_complex else 16), np.float64)
+    check(lib().omx_csc_to_bsr(nb, _ptr(colptr), _ptr(rowidx), _ptr(val), ptrbase, rowbase, int(is_complex),
+                               C.byref(nnzb), _ptr(cp), _ptr(ri), _ptr(bv)), "omx_csc_to_bsr")
+    return cp, ri, bv
 
 
 def lattice_bsr(L, which: int, seed: int = DEFAULT_SEED):

@@ -31,6 +31,8 @@ def lib():
         L.omm_c_mat_set_walk.argtypes = [_i, _i, _vp, _i]
         L.omm_c_mat_set_walk.restype = _i
         L.omm_c_mat_from_bsr.argtypes = [_i, _vp, _vp, _vp]
+        L.omm_c_mat_register_pcsc.argtypes = [_i, _i, _vp, _vp, _vp]
+        L.omm_c_mat_convert.argtypes = [_i, C.c_char_p, _d]
         L.omm_c_mat_upload.argtypes = [_i, _vp]
         L.omm_c_mat_download.argtypes = [_i, _vp]
         L.omm_c_mat_free.argtypes = [_i]
@@ -82,6 +84,28 @@ class Mat:
         if self.h <= 0:
             raise RuntimeError("omm_c_mat_from_csc failed")
         return self
+
+    @classmethod
+    def register_pcsc(cls, nb, idx1, idx2, val):
+        """Device operator registered through m_register_pcsc from a CSC in pspBLAS's convention:
+        1-based row indices idx1 and column pointers idx2 (idx2[0] == 1)."""
+        self = cls.__new__(cls)
+        self.rows = self.cols = nb
+        self.label = "gdcsc"
+        i1 = np.ascontiguousarray(idx1, np.int32)
+        i2 = np.ascontiguousarray(idx2, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self.h = lib().omm_c_mat_register_pcsc(nb, len(v), _p(i1), _p(i2), _p(v))
+        if self.h <= 0:
+            raise RuntimeError("omm_c_mat_register_pcsc failed")
+        return self
+
+    def convert(self, label, threshold=None):
+        """m_convert to another 5-character label (optionally with a threshold)."""
+        rc = lib().omm_c_mat_convert(self.h, label.encode(), -1.0 if threshold is None else float(threshold))
+        if rc != 0:
+            raise RuntimeError("omm_c_mat_convert failed (bad handle)")
+        self.label = label
 
     @classmethod
     def from_bsr(cls, nb, bcolptr, browidx, bval):

@@ -208,3 +208,58 @@ def test_mixed_precision_bf16x3_gradient(state):
     assert abs(r["e_min"] - exact) <= 1e-10 * abs(exact)
     for M in (Hm, Sm, Cm):
         M.free()
+
+
+def test_pdcsc_registered_operators_match_oracle():
+    """H and S registered through m_register_pcsc in pspBLAS's convention (1-based row indices and
+    column pointers, MatrixSwitch_m_register.F90:394-443, pspMat.F90:87-140), then omm() on device
+    storage against the oracle on the same CSC: the caller of the 1-rank 'pdcsc' path, drop-in."""
+    _note("pdcsc registration")
+    L, eta = 3, 0.2
+    nb, Hb, Sb = lattice(L)
+    no = nb // 2
+    H = oh.bsr_to_dense(nb, *Hb)
+    S = oh.bsr_to_dense(nb, *Sb)
+    Hs = H - eta * S
+    C0 = oh.random_c0(no, nb)
+    Cc = C0.copy(order="F")
+    o = O.Omm()(nb, no, csc_of(Hs), csc_of(S), Cc, init_C=True, eta=eta, cg_tol=1e-9, max_iter=300)
+    hp, sp = csc_of(Hs), csc_of(S)
+    Hm = F.Mat.register_pcsc(nb, hp[1] + 1, hp[0] + 1, hp[2])
+    Sm = F.Mat.register_pcsc(nb, sp[1] + 1, sp[0] + 1, sp[2])
+    for i, j in ((1, 1), (1, 5), (7, 2), (nb, nb)):
+        assert Hm.get_element(i, j) == Hs[i - 1, j - 1]
+    Cm = F.Mat(no, nb, data=C0)
+    F.set_option("max_iter", 300)
+    try:
+        r = F.omm(nb, no, Hm, Sm, Cm, init_C=True, eta=eta, cg_tol=1e-9, dealloc=True)
+    finally:
+        F.set_option("max_iter", 0)
+    assert abs(r["e_min"] - o["e_min"]) <= 1e-10 * abs(o["e_min"])
+    assert abs(r["icg"] - o["icg"]) <= 2
+    for M in (Hm, Sm, Cm):
+        M.free()
+
+
+def test_m_convert_device_storages():
+    """m_convert (MatrixSwitch.F90:987-1041) between host and device storage: 'sdden' -> 'gdden'
+    (values unchanged), 'gdden' -> 'gsden' (FP32), and 'gdden' -> 'gdcsc' with a threshold used as
+    the SpMM operator of mm_multiply against the oracle's CSC product."""
+    _note("m_convert")
+    nb, no = 24, 8
+    r = np.random.default_rng(4)
+    A = r.standard_normal((nb, nb)) * (r.random((nb, nb)) < 0.4)
+    M = F.Mat(nb, nb, "sdden", data=A)
+    M.convert("gdden")
+    assert np.array_equal(M.download(), A)
+    M.convert("gsden")
+    assert np.array_equal(M.download(), A.astype(np.float32).astype(np.float64))
+    M.free()
+    M = F.Mat(nb, nb, "sdden", data=A)
+    M.convert("gdden")
+    M.convert("gdcsc", threshold=0.5)
+    At = np.where(np.abs(A) > 0.5, A, 0.0)
+    for i in range(1, nb + 1, 5):
+        for j in range(1, nb + 1, 3):
+            assert M.get_element(i, j) == At[i - 1, j - 1]
+    M.free()
