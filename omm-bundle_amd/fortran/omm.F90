@@ -226,7 +226,7 @@ subroutine omm(m, n, H, S, new_S, e_min, D_min, calc_ED, eta, C_min, init_C, T, 
     call point_state()
     call alloc_locals(m_storage)
     call initial_energy('hip', .true.)
-    call cg('hip', cg_tol_internal, opt_max_iter, .false., opt_refine_grad)
+    call cg('hip', cg_tol_internal, opt_max_iter, .false., opt_refine_min, opt_refine_grad)
   else
     call point_state()
     call alloc_locals(m_storage)
@@ -366,18 +366,21 @@ contains
   !> Stops when e_diff <= tol, or after max_steps steps of this phase (0 = reference caps only).
   !> lowp: the low-precision phase of 'hmx' (periodic recomputation of the state from C, and the
   !> stop test on the decrease predicted by the line search).
-  !> gtol (the FP64 refinement of 'hmx'): e_diff <= tol ends the loop only when the gradient of the
-  !> step, ||G_p||, is at most gtol times the gradient at the start of the loop.  The refinement
-  !> starts from the low-precision phase's precision floor, where the first FP64 steps (restarted
-  !> CG) can lower the energy by less than cg_tol while C, and so the density matrix, still carries
-  !> that phase's error; the energy is quadratic in that error, the gradient linear.
-  subroutine cg(lab, tol, max_steps, lowp, gtol)
+  !> min_steps, gtol (the FP64 refinement of 'hmx'): e_diff <= tol ends the loop only from the
+  !> min_steps-th step on, and, with gtol > 0, only when the step's gradient ||G_p|| is at most gtol
+  !> times the gradient at the start of the loop.  The refinement restarts CG (a steepest-descent
+  !> step first) from the low-precision phase's coefficients; the first steps of a restarted CG
+  !> can lower the energy by less than cg_tol while C, and so the density matrix, still carries that
+  !> phase's error (the energy is quadratic in it, the density matrix linear).  After a few steps
+  !> the per-step decrease again measures the remaining error, as in the FP64 solve.
+  subroutine cg(lab, tol, max_steps, lowp, min_steps, gtol)
     character(3), intent(in) :: lab
     real(dp), intent(in) :: tol
     integer, intent(in) :: max_steps
     logical, intent(in) :: lowp
+    integer, intent(in), optional :: min_steps
     real(dp), intent(in), optional :: gtol
-    integer :: outer, inner, steps, n_below, n_stall
+    integer :: outer, inner, steps, n_below, n_stall, nmin
     logical :: ls_conv, capped, rewin_now
     real(dp) :: e_pred, e_best, g2_start, g2
     conv = .false.
@@ -386,6 +389,8 @@ contains
     n_below = 0
     n_stall = 0
     e_best = huge(1.0_dp)
+    nmin = 0
+    if (present(min_steps)) nmin = min_steps
     g2_start = -1.0_dp
     if (present(gtol)) then
       if (gtol > 0.0_dp) then
@@ -482,7 +487,7 @@ contains
             end if
             if (opt_lowp_stall > 0 .and. n_stall >= opt_lowp_stall) conv = .true.
           end if
-        else if (e_diff <= tol) then
+        else if (e_diff <= tol .and. steps >= nmin) then
           conv = .true.
           if (g2_start > 0.0_dp) then
             call mm_trace(G_p, G_p, g2, scale_label())

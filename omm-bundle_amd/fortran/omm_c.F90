@@ -368,8 +368,8 @@ contains
     omm_c_stats = 0
   end function omm_c_stats
 
-  !> Gradient norms of the last call's FP64 refinement ('hmx'): at its start and at its last stop
-  !> test (both 0 when the call had no refinement with a gradient criterion).
+  !> Gradient norms of the last call's FP64 refinement ('hmx' with refine_grad > 0): at its start
+  !> and at its last stop test (both 0 otherwise).
   integer(c_int) function omm_c_refine_stats(g0, gend) bind(c, name='omm_c_refine_stats')
     real(c_double), intent(out) :: g0, gend
     g0 = stat_g0

@@ -38,14 +38,17 @@ module omm_hip_ops
                                              ! 3 those of 1 and FP32 ('h32') products
   integer, save :: opt_lowp_rewin = 5        ! low-precision phase: recompute HC, SC, HW, SW from C every
                                              ! this many steps (0 = incremental updates only)
-  ! mixed precision: the FP64 refinement stops on e_diff <= cg_tol only once the gradient norm has
-  ! dropped to this fraction of its value at the start of the refinement (<= 0: e_diff alone)
-  real(dp), save :: opt_refine_grad = 0.1_dp
+  ! mixed precision: the FP64 refinement applies the stop test e_diff <= cg_tol from this step on
+  integer, save :: opt_refine_min = 8
+  ! mixed precision: with > 0, the FP64 refinement also needs the gradient norm at this fraction of
+  ! its value at the start of the refinement (0: off)
+  real(dp), save :: opt_refine_grad = 0.0_dp
 
   ! ---- statistics of the last omm() call ----
   integer, save :: stat_icg = 0, stat_conv = 0, stat_nfail = 0, stat_icg_lowp = 0
   real(dp), save :: stat_TrQS = 0.0_dp
-  ! gradient norms of the last FP64 refinement ('hmx'): at its start and at its last stop test
+  ! gradient norms of the last FP64 refinement ('hmx', with refine_grad > 0): at its start and at
+  ! its last stop test
   real(dp), save :: stat_g0 = 0.0_dp, stat_gend = 0.0_dp
   integer, parameter :: stat_log_cap = 100000
   real(dp), save :: stat_elog(stat_log_cap)
@@ -397,6 +400,8 @@ contains
       opt_lowp_stall = max(0, nint(value))
     case ('refine_grad')
       opt_refine_grad = value
+    case ('refine_min')
+      opt_refine_min = max(0, nint(value))
     case default
       call omm_die('omm_set_option: unknown option '//name)
     end select
