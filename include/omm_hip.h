@@ -275,6 +275,17 @@ int omx_omm_calc_G(omx_ctx ctx, int prec, int dt, int n, int m, const void *HW, 
 /* out[0] = <G, G - G_p>, out[1] = <G_p, G_p> in FP64 (omm.F90:536-546, Polak-Ribiere). Synchronises. */
 int omx_omm_pr_dots(omx_ctx ctx, int dt, int n, int m, const void *G, const void *G_p, int64_t ld, double *out);
 
+/* omx_omm_pr_dots without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums live in a file-scope device array of 32 KB (2048 workgroups'
+ * partials and the 2 final sums) that each GPU holds from omx_init until the process ends; inputs
+ * of any size use at most 2048 workgroups, the same launch as omx_omm_pr_dots, so the two return
+ * the same bits.  Every call on one GPU uses that GPU's single array: calls that overlap on one GPU
+ * (other host threads, other contexts on the device) race and must be serialised by the caller.
+ * Synchronises. */
+int omx_omm_pr_dots_noalloc(omx_ctx ctx, int dt, int n, int m, const void *G, const void *G_p, int64_t ld,
+                            double *out);
+
 /* D := G + lambda*D and G_p := G in one pass (omm.F90:440-445). */
 int omx_omm_cg_direction(omx_ctx ctx, int dt, int n, int m, double lambda, const void *G, void *D, void *G_p,
                          int64_t ld);

@@ -215,6 +215,11 @@ int omx_init(int device, omx_ctx *out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    const int rc = noalloc_setup(device);
+    if (rc) {
+        (void)omx_finalize(ctx);
+        return rc;
+    }
     *out = ctx;
     return OMX_OK;
 }

@@ -5,6 +5,7 @@
 // tile range into the context workspace, then one workgroup adds the partials in workgroup
 // order and writes the result to the context's scalar slots (read back through pinned memory).
 #include <algorithm>
+#include <mutex>
 
 #include "omx_internal.h"
 
@@ -403,6 +404,38 @@ static int finish_reduce(omx_ctx ctx, int blocks, double *partial) {
     return read_scalars(ctx, NV);
 }
 
+// ---- scratch of the allocation-free Polak-Ribiere dots (omx_omm_pr_dots_noalloc) ---------------
+// The partial sums of at most kPrBlocks workgroups (2 each), then the 2 final sums: 32 KB per GPU,
+// held until the process ends.  In host code `scratch` only names the symbol; the device address
+// of each GPU's copy comes from hipGetSymbolAddress (noalloc_setup) and is what kernels receive.
+constexpr int kPrBlocks = 2048;
+constexpr int kScratchDoubles = 2 * kPrBlocks + 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (called by omx_init): look up this GPU's copy.
+int noalloc_setup(int device) {
+    OMX_REQUIRE(device >= 0 && device < kMaxDevices, "omx_init: device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return OMX_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error("omx_init: hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+        return OMX_ERR_HIP;
+    }
+    g_scratch_ptr[device] = (double *)p;
+    return OMX_OK;
+}
+
+static double *noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace omx
 
 using namespace omx;
@@ -598,6 +631,33 @@ int omx_omm_pr_dots(omx_ctx ctx, int dt, int n, int m, const void *G, const void
     if (rc) return rc;
     out[0] = ctx->h_scalars[0];
     out[1] = ctx->h_scalars[1];
+    return OMX_OK;
+}
+
+int omx_omm_pr_dots_noalloc(omx_ctx ctx, int dt, int n, int m, const void *G, const void *G_p, int64_t ld,
+                            double *out) {
+    OMX_REQUIRE(ctx && out && n > 0 && m > 0 && ld >= n && G && G_p, "omx_omm_pr_dots_noalloc: bad arguments");
+    double *sp = noalloc_scratch(ctx->device);
+    OMX_REQUIRE(sp, "omx_omm_pr_dots_noalloc: no scratch on device %d (context not initialised)", ctx->device);
+    const int64_t total = (int64_t)n * m;
+    // the same workgroup count as omx_omm_pr_dots (so the same sums, bit for bit); it never exceeds
+    // the scratch's kPrBlocks partial slots, whatever the size
+    const int blocks = std::min(2048, grid_for(total));
+    OMX_REQUIRE(blocks >= 1 && blocks <= kPrBlocks,
+                "omx_omm_pr_dots_noalloc: %d workgroups exceed the scratch's %d partial slots", blocks, kPrBlocks);
+    const bool ok = with_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        hipLaunchKernelGGL(k_pr_partial<T>, dim3(blocks), dim3(EW_NT), 0, ctx->stream, total, n, (const T *)G,
+                           (const T *)G_p, ld, sp);
+    });
+    if (!ok) {
+        set_error("omx_omm_pr_dots_noalloc: unsupported element type %d", dt);
+        return OMX_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((k_reduce_partials<2>), dim3(1), dim3(EW_NT), 0, ctx->stream, blocks, sp, sp + 2 * kPrBlocks);
+    OMX_CHECK_HIP(hipGetLastError());
+    OMX_CHECK_HIP(hipMemcpyAsync(out, sp + 2 * kPrBlocks, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    OMX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return OMX_OK;
 }
 

@@ -142,6 +142,10 @@ struct El<double2> {
 // Ensure ctx->ws holds at least `bytes`.  Returns OMX_OK or an error.
 int ensure_ws(omx_ctx ctx, size_t bytes);
 
+// Look up this device's copy of the static scratch of omx_omm_pr_dots_noalloc (once per device,
+// with the device current).  Returns OMX_OK or an error.
+int noalloc_setup(int device);
+
 // Begin / end a timed launch of kernel class `kclass` (no-ops unless profiling is on).
 void prof_begin(omx_ctx ctx, int kclass, hipEvent_t *ev_start);
 void prof_end(omx_ctx ctx, int kclass, hipEvent_t ev_start);

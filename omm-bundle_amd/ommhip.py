@@ -93,6 +93,7 @@ SIGNATURES = {
     "omx_omm_coeff": (_i, [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "omx_omm_calc_G": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _vp, _i64]),
     "omx_omm_pr_dots": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp]),
+    "omx_omm_pr_dots_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp]),
     "omx_omm_cg_direction": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp, _i64]),
     "omx_omm_update_w": (_i, [_vp, _i, _i, _i64, _d, _vp, _vp, _vp]),
     "omx_prof_enable": (_i, [_vp, _i]),
