@@ -399,6 +399,15 @@ contains
     omm_c_tlog = k + 1
   end function omm_c_tlog
 
+  !> The C-ABI operator (omx_bsr) behind a device sparse matrix ('gdcsc', 'gzcsc'), null otherwise:
+  !> for omx_bsr_get_values / omx_bsr_get_pattern from the caller.
+  type(c_ptr) function omm_c_mat_bsr(h) bind(c, name='omm_c_mat_bsr')
+    integer(c_int), value :: h
+    omm_c_mat_bsr = c_null_ptr
+    if (.not. valid(h)) return
+    if (table(h)%is_device .and. table(h)%is_sparse) omm_c_mat_bsr = table(h)%bsr
+  end function omm_c_mat_bsr
+
   !> The libommhip context used by the 'g' storage (for event timing from the caller).
   type(c_ptr) function omm_c_ctx() bind(c, name='omm_c_ctx')
     omm_c_ctx = ms_hip_ctx()

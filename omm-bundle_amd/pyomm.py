@@ -33,6 +33,8 @@ def lib():
         L.omm_c_mat_from_bsr.argtypes = [_i, _vp, _vp, _vp]
         L.omm_c_mat_register_pcsc.argtypes = [_i, _i, _vp, _vp, _vp]
         L.omm_c_mat_convert.argtypes = [_i, C.c_char_p, _d]
+        L.omm_c_mat_bsr.argtypes = [_i]
+        L.omm_c_mat_bsr.restype = _vp
         L.omm_c_mat_upload.argtypes = [_i, _vp]
         L.omm_c_mat_download.argtypes = [_i, _vp]
         L.omm_c_mat_free.argtypes = [_i]
@@ -99,6 +101,19 @@ class Mat:
         if self.h <= 0:
             raise RuntimeError("omm_c_mat_register_pcsc failed")
         return self
+
+    def bsr_values(self):
+        """Block values (nnzb x 16, column-major blocks) of a real device sparse matrix."""
+        import ommhip as oh
+
+        b = lib().omm_c_mat_bsr(self.h)
+        if not b:
+            raise RuntimeError("not a device sparse matrix")
+        nnzb = C.c_int64(0)
+        oh.check(oh.lib().omx_bsr_info(b, None, C.byref(nnzb), None), "bsr_info")
+        out = np.zeros((nnzb.value, 16))
+        oh.check(oh.lib().omx_bsr_get_values(b, _p(out)), "bsr_get_values")
+        return out
 
     def convert(self, label, threshold=None):
         """m_convert to another 5-character label (optionally with a threshold)."""
