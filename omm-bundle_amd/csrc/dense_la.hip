@@ -169,6 +169,16 @@ __global__ __launch_bounds__(LA_NT) void k_mirror_upper(int n, double *__restric
     }
 }
 
+// Strict lower triangle of src (n x n) into dst, the upper triangle untouched.
+__global__ __launch_bounds__(LA_NT) void k_copy_lower(int n, const double *__restrict__ src, int64_t lds,
+                                                      double *__restrict__ dst, int64_t ldd) {
+    const int64_t total = (int64_t)n * n;
+    for (int64_t q = (int64_t)blockIdx.x * LA_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * LA_NT) {
+        const int64_t i = q % n, j = q / n;
+        if (i > j) dst[i + j * ldd] = src[i + j * lds];
+    }
+}
+
 __global__ void k_set_identity(int n, double *__restrict__ A, int64_t lda) {
     const int64_t total = (int64_t)n * n;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
@@ -338,7 +348,29 @@ int omx_potrf(omx_ctx ctx, int dt, int n, void *A, int64_t lda, int *info) {
     *info = 0;
     if (n == 0) return OMX_OK;
     OMX_REQUIRE(A, "omx_potrf: null operand");
-    return potrf_impl(ctx, n, (double *)A, lda, info);
+    // The trailing updates write both triangles; like dpotrf 'u' the call leaves the strict lower
+    // triangle as it was, so it is saved here and put back afterwards.
+    double *save = nullptr;
+    if (n > 1) {
+        if (hipMalloc(&save, sizeof(double) * (size_t)n * n) != hipSuccess) {
+            set_error("omx_potrf: device allocation of %d x %d failed", n, n);
+            return OMX_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_copy_lower, dim3(la_grid((int64_t)n * n)), dim3(LA_NT), 0, ctx->stream, n,
+                           (const double *)A, lda, save, (int64_t)n);
+    }
+    int rc = potrf_impl(ctx, n, (double *)A, lda, info);
+    if (save) {
+        hipLaunchKernelGGL(k_copy_lower, dim3(la_grid((int64_t)n * n)), dim3(LA_NT), 0, ctx->stream, n,
+                           (const double *)save, (int64_t)n, (double *)A, lda);
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        (void)hipFree(save);
+        if (!rc && e != hipSuccess) {
+            set_error("omx_potrf: %s", hipGetErrorString(e));
+            rc = OMX_ERR_HIP;
+        }
+    }
+    return rc;
 }
 
 int omx_trsm(omx_ctx ctx, int dt, char side, char trans, int m, int n, double alpha, const void *U, int64_t ldu,

@@ -33,6 +33,8 @@ def test_potrf(ctx, n):
     dS2 = oh.DMat.from_host(ctx, S2)
     oh.check(oh.lib().omx_potrf(ctx.h, oh.OMX_F64, n, dS2.p, dS2.ld, C.byref(info)), "potrf")
     assert np.array_equal(np.triu(dS2.download()), U)
+    # and, as with dpotrf 'u', it is left as it was
+    assert np.array_equal(np.tril(dS2.download(), -1), np.tril(S2, -1))
 
 
 @pytest.mark.parametrize("n,bad", [(40, 35), (600, 400)])

@@ -263,3 +263,39 @@ def test_m_convert_device_storages():
         for j in range(1, nb + 1, 3):
             assert M.get_element(i, j) == At[i - 1, j - 1]
     M.free()
+
+
+@pytest.mark.parametrize("flavour", [1, 2, 3])
+def test_flavours_reach_the_closed_form(flavour):
+    """The Cholesky flavours 1 (S factored by omm()) and 2 (S given as its upper Cholesky factor U,
+    omm.F90:133-136, 305-309) and the preconditioned flavour 3 (P = S^-1, omm.F90:379-403) on dense
+    device H and S: the gapped 4x4x4 lattice (onsite levels -6, -6, 6, 6, eta mid-gap) solved to
+    cg_tol 1e-11; the energy agrees with the sum of the occupied generalised eigenvalues to 1e-10
+    relative and D_min with the S-orthonormal projector to 1e-6, as the flavour-0 solve does."""
+    import scipy.linalg as sla
+
+    _note("flavour", flavour)
+    L = 4
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    cp, ri, h = oh.lattice_bsr_theta(L, oh.lattice_theta0())
+    Sb = oh.lattice_bsr(L, 1)
+    H, S = oh.bsr_to_dense(nb, cp, ri, h), oh.bsr_to_dense(nb, *Sb)
+    w, Dx = _exact_density(H, S, no)
+    eta = 0.5 * (w[no - 1] + w[no])
+    exact = float(np.sum(w[:no]))
+    Sgiven = sla.cholesky(S, lower=False) if flavour == 2 else S
+    Hm, Sm = F.Mat(nb, nb, data=H - eta * S), F.Mat(nb, nb, data=Sgiven)
+    C0 = oh.random_c0(no, nb)
+    Cm, Dm = F.Mat(no, nb, data=C0), F.Mat(nb, nb)
+    r = F.omm(nb, no, Hm, Sm, Cm, D_min=Dm, init_C=True, eta=eta, cg_tol=1e-11, flavour=flavour, dealloc=True)
+    rel = abs(r["e_min"] - exact) / abs(exact)
+    derr = np.max(np.abs(Dm.download() - Dx))
+    _note("flavour", flavour, "icg", r["icg"], "energy err", rel, "D err", derr)
+    assert r["converged"] and rel <= 1e-10 and derr < 1e-6
+    if flavour == 1:
+        # S now holds U in its upper triangle; its strict lower triangle is the caller's S
+        Sout = Sm.download()
+        assert np.max(np.abs(np.triu(Sout) - sla.cholesky(S, lower=False))) < 1e-12
+        assert np.array_equal(np.tril(Sout, -1), np.tril(S, -1))
+    for M in (Hm, Sm, Cm, Dm):
+        M.free()
