@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--launches", type=int, default=3, help="launches in --kernel mode")
     ap.add_argument("--kernel-prec", choices=["f64", "f32", "bf16", "bf16x3"], default="f64",
                     help="product precision in --kernel mode (all but f64 on FP32 storage)")
+    ap.add_argument("--spmm-stencil", action="store_true",
+                    help="run lattice operators through the 2.5-D stencil SpMM instead of the general BSR kernel")
     ap.add_argument("--mixed-steps", type=int, default=0,
                     help="also time this many low-precision steps of the mixed (hmx) solve, FP32 storage")
     args = ap.parse_args()
@@ -250,7 +252,8 @@ def main():
         print(json.dumps(la_bench(oh, args.L)), flush=True)
         return 0
     if args.kernel:
-        print(json.dumps(kernel_mode(oh, args.kernel, args.L, args.launches, args.kernel_prec)), flush=True)
+        print(json.dumps(kernel_mode(oh, args.kernel, args.L, args.launches, args.kernel_prec,
+                                     stencil=args.spmm_stencil)), flush=True)
         return 0
 
     if args.dims:
@@ -516,7 +519,7 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
 GROUP_ETA = 0.0517
 
 
-def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3):
+def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3, stencil=False):
     """Fixed launch sequences of one library kernel at the L^3 workload's shape (X, Y no x n),
     followed by `copies` device copies of a no x n FP64 matrix (no*n*8 bytes read and written per
     copy): under rocprofv3 --pmc the copies' counter totals give the scale between a counter and
@@ -528,6 +531,7 @@ def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3):
     pr = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3}[prec]
     es = 8 if prec == "f64" else 4
     ctx = oh.Ctx(0)
+    ctx.spmm_stencil(stencil)
     mats = []
     rng = np.random.default_rng(0)
     if kernel == "spmm":
@@ -561,6 +565,7 @@ def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3):
         oh.add(ctx, "n", Sx, Dx, 1.0, 0.0)
     ctx.sync()
     out = {"mode": "kernel", "kernel": kernel, "prec": prec, "L": L, "n": nb, "m": no, "launches": launches,
+           "spmm_kernel": "stencil" if stencil else "general",
            "wall_ms_per_launch": dt * 1e3, "copy_bytes_read": no * nb * 8, "copy_bytes_written": no * nb * 8}
     out.update(work)
     for M in mats + [Sx, Dx]:

@@ -202,6 +202,16 @@ int omx_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, cons
  * rows; 1: each XCD walks one row chunk at a time).  batch = 0 returns to the default shape (or
  * the autotuner's).  Tuning hook; results do not depend on it. */
 int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode);
+/* With enable != 0, real operators whose pattern is the periodic 7-point stencil of an
+ * Lx x Ly x Lz lattice of 4x4 blocks (recognised when the operator is created; Lx, Ly multiples of
+ * 4) run through a 2.5-D kernel that keeps z-planes of X in registers and LDS instead of gathering
+ * each basis column 7 times: 1.2x the algorithmic HBM bytes instead of 2.1x, but slower than the
+ * general kernel at n = 16384 (1.05 ms against 0.74 ms), so off by default.  Tuning hook; the two
+ * kernels give the same bits. */
+int omx_spmm_stencil(omx_ctx ctx, int enable);
+/* Lattice edges (Lx, Ly, Lz) recognised in the operator's pattern, all 0 when it is not such a
+ * stencil. */
+int omx_bsr_lattice(omx_bsr a, int *dims);
 /* Attach to an operator the order in which SpMMs visit its block columns (a permutation of
  * 0..nbc-1, host array copied to the device; NULL clears it).  A schedule hint only: every
  * output column is computed the same way in any order (no counterpart in the reference). */

@@ -330,6 +330,42 @@ int omx_convert(omx_ctx ctx, int m, int n, const void *src, int dt_src, int64_t 
 
 // ---- sparse operators -----------------------------------------------------------------------
 
+// Lx, Ly, Lz when block column J = x + Lx*(y + Ly*z) of an nbc-site operator holds exactly the
+// blocks of J and of its six periodic neighbours x+-1, y+-1, z+-1 (every edge >= 3), else false.
+// Site 0's sorted block rows are 0, 1, Lx-1, Lx, Lx*(Ly-1), Lx*Ly, Lx*Ly*(Lz-1), which give the
+// edges; every column is then checked against them.
+static bool detect_lattice(int nbc, const std::vector<int> &colptr, const std::vector<int> &rowidx, int lat[3]) {
+    lat[0] = lat[1] = lat[2] = 0;
+    if (nbc < 27 || colptr[1] - colptr[0] != 7) return false;
+    const int *r = rowidx.data();
+    const int Lx = r[3];
+    if (Lx < 3 || r[0] != 0 || r[1] != 1 || r[2] != Lx - 1) return false;
+    const int LxLy = r[5];
+    if (LxLy % Lx != 0) return false;
+    const int Ly = LxLy / Lx;
+    if (Ly < 3 || r[4] != Lx * (Ly - 1) || nbc % LxLy != 0) return false;
+    const int Lz = nbc / LxLy;
+    if (Lz < 3 || r[6] != LxLy * (Lz - 1)) return false;
+    for (int J = 0; J < nbc; ++J) {
+        if (colptr[J + 1] - colptr[J] != 7) return false;
+        const int x = J % Lx, y = (J / Lx) % Ly, z = J / LxLy;
+        int want[7] = {J,
+                       (x + 1) % Lx + Lx * (y + Ly * z),
+                       (x + Lx - 1) % Lx + Lx * (y + Ly * z),
+                       x + Lx * ((y + 1) % Ly + Ly * z),
+                       x + Lx * ((y + Ly - 1) % Ly + Ly * z),
+                       x + Lx * (y + Ly * ((z + 1) % Lz)),
+                       x + Lx * (y + Ly * ((z + Lz - 1) % Lz))};
+        std::sort(want, want + 7);
+        for (int q = 0; q < 7; ++q)
+            if (rowidx[colptr[J] + q] != want[q]) return false;
+    }
+    lat[0] = Lx;
+    lat[1] = Ly;
+    lat[2] = Lz;
+    return true;
+}
+
 static int bsr_upload(omx_ctx ctx, int nb, const std::vector<int> &colptr, const std::vector<int> &rowidx,
                       const double *val, int is_complex, omx_bsr *out) {
     omx_bsr a = new omx_bsr_s();
@@ -341,6 +377,7 @@ static int bsr_upload(omx_ctx ctx, int nb, const std::vector<int> &colptr, const
     int mx = 0;
     for (int j = 0; j < a->nbc; ++j) mx = std::max(mx, colptr[j + 1] - colptr[j]);
     a->max_blocks_per_col = mx;
+    if (nb == 4 * a->nbc) (void)detect_lattice(a->nbc, colptr, rowidx, a->lat);
     size_t vbytes = sizeof(double) * 16 * (size_t)a->nnzb * (is_complex ? 2 : 1);
     if (hipMalloc(&a->d_colptr, sizeof(int) * (a->nbc + 1)) != hipSuccess ||
         hipMalloc(&a->d_rowidx, sizeof(int) * std::max<int64_t>(a->nnzb, 1)) != hipSuccess ||
@@ -480,6 +517,12 @@ int omx_csc_to_bsr(int nb, const int *colptr, const int *rowidx, const double *v
     if (bcolptr) std::copy(cp.begin(), cp.end(), bcolptr);
     if (browidx) std::copy(ri.begin(), ri.end(), browidx);
     if (bval) std::copy(bv.begin(), bv.end(), bval);
+    return OMX_OK;
+}
+
+int omx_bsr_lattice(omx_bsr a, int *dims) {
+    OMX_REQUIRE(a && dims, "omx_bsr_lattice: bad arguments");
+    for (int q = 0; q < 3; ++q) dims[q] = a->lat[q];
     return OMX_OK;
 }
 

@@ -77,6 +77,7 @@ struct omx_ctx_s {
     bool gemm_sym = false;  // set by omx_gemm_sym around its product: lower-triangle tiles only
     omx_spmm_shape_s spmm_shape;  // forced SpMM launch shape (when spmm_forced)
     bool spmm_forced = false;
+    bool spmm_stencil = false;  // lattice operators use the 2.5-D stencil SpMM (omx_spmm_stencil)
     // Device flag for the grouped solver: while set, GEMM, SpMM and reduction launches do nothing
     // when *launch_mask == 0, and GEMMs do not split K (no workspace, so launches can be captured).
     const int *launch_mask = nullptr;
@@ -97,6 +98,9 @@ struct omx_bsr_s {
     int *d_rowidx = nullptr;  // nnzb
     double *d_val = nullptr;  // 16*nnzb (x2 if complex)
     int *d_walk = nullptr;    // optional order in which SpMMs visit the block columns (nbc)
+    // Lx, Ly, Lz when the pattern is the periodic 7-point stencil of that lattice of 4x4 blocks
+    // (recognised at upload; 0 otherwise)
+    int lat[3] = {0, 0, 0};
     // LDS-staged gathers: block columns taken `group` at a time in walk order; group g gathers the
     // distinct block rows uidx[gptr[g] .. gptr[g+1]) and block l reads slot[l] of them
     int group = 0, ngroups = 0;

@@ -93,6 +93,8 @@ SIGNATURES = {
     "omx_omm_coeff": (_i, [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "omx_omm_calc_G": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _vp, _i64]),
     "omx_omm_pr_dots": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp]),
+    "omx_spmm_stencil": (_i, [_vp, _i]),
+    "omx_bsr_lattice": (_i, [_vp, _vp]),
     "omx_omm_pr_dots_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp]),
     "omx_omm_cg_direction": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp, _i64]),
     "omx_omm_update_w": (_i, [_vp, _i, _i, _i64, _d, _vp, _vp, _vp]),
@@ -333,6 +335,10 @@ class Ctx:
         lib().omx_autotune_report(self.h, buf, len(buf))
         return buf.value.decode()
 
+    def spmm_stencil(self, enable=True):
+        """Use the 2.5-D stencil SpMM for recognised lattice operators (default off)."""
+        check(lib().omx_spmm_stencil(self.h, int(enable)), "spmm_stencil")
+
     def spmm_shape(self, batch=0, wpg=4, jper=1, mode=0):
         """Force the real SpMM launch shape (batch 0: back to the default or tuned shape)."""
         check(lib().omx_spmm_set_tile(self.h, batch, wpg, jper, mode), "spmm_set_tile")
@@ -467,8 +473,11 @@ class Bsr:
             self.h = None
 
     def __del__(self):
+        # only while the context lives: destroying an operator synchronises its context's stream,
+        # which is gone once the context is finalised (the memory is then released with the process)
         try:
-            self.free()
+            if self.h and self.ctx.h:
+                self.free()
         except Exception:
             pass
 
@@ -607,7 +616,8 @@ class ZBatch:
 
     def __del__(self):
         try:
-            self.free()
+            if self.h and self.ctx.h:
+                self.free()
         except Exception:
             pass
 

@@ -4,6 +4,8 @@ Each test prints the case it runs before the GPU call so that a failing run name
 Tolerances: FP64 paths 1e-12 relative to the operand scale (FMA and summation order differ from
 numpy); FP32 paths 1e-5; BF16 paths 2e-2 (8-bit mantissa operands, FP32 accumulation).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -589,3 +591,59 @@ def test_sddmm_f32(ctx):
     ref = np.where(_pattern_mask(nb, cp, ri), A.astype(np.float64).T @ B.astype(np.float64), 0.0)
     got = oh.bsr_to_dense(nb, cp, ri, P.values())
     assert np.max(np.abs(got - ref)) < 1e-4
+
+
+@pytest.mark.parametrize("dims,no,dt", [((16, 16, 16), 8192, oh.OMX_F64), ((8, 8, 8), 1024, oh.OMX_F64),
+                                        ((4, 8, 3), 200, oh.OMX_F64), ((12, 4, 5), 37, oh.OMX_F64),
+                                        ((16, 16, 16), 8192, oh.OMX_F32), ((8, 4, 6), 300, oh.OMX_F32)])
+def test_spmm_stencil_equals_general_kernel(ctx, dims, no, dt):
+    """The 2.5-D kernel of lattice operators (z-planes in registers and LDS) against the general
+    BSR kernel on the same operator and X: the same bits, including partial row chunks, Lz = 3 and
+    beta != 0; and the lattice edges recognised in the pattern."""
+    _note("spmm stencil", dims, no, dt)
+    cp, ri, bv = oh.lattice_bsr(dims, 0)
+    nb = 4 * int(np.prod(dims))
+    A = oh.Bsr(ctx, nb, cp, ri, bv)
+    lat = np.zeros(3, np.int32)
+    oh.check(oh.lib().omx_bsr_lattice(A.h, lat.ctypes.data_as(C.c_void_p)), "bsr_lattice")
+    assert tuple(lat) == tuple(dims)
+    X = rng.standard_normal((no, nb))
+    Y0 = rng.standard_normal((no, nb))
+    dX = oh.DMat.from_host(ctx, X, dt)
+    out = {}
+    try:
+        for stencil in (True, False):
+            ctx.spmm_stencil(stencil)
+            for beta in (-0.5, 0.0):
+                dY = oh.DMat.from_host(ctx, Y0, dt)
+                oh.spmm(ctx, dX, A, dY, alpha=1.5, beta=beta)
+                out[stencil, beta] = dY.download()
+                dY.free()
+    finally:
+        ctx.spmm_stencil(False)
+        dX.free()
+        A.free()
+    for beta in (-0.5, 0.0):
+        assert np.array_equal(out[True, beta], out[False, beta])
+    H = oh.bsr_to_dense(nb, cp, ri, bv)
+    tol = 1e-12 if dt == oh.OMX_F64 else 1e-4
+    assert np.max(np.abs(out[True, -0.5] - (1.5 * X @ H - 0.5 * Y0))) < tol * 30
+
+
+def test_non_lattice_pattern_is_not_taken_for_a_lattice(ctx):
+    # a 7-block pattern that is not the periodic stencil (one block row moved) stays on the general
+    # kernel: no lattice recognised
+    cp, ri, bv = oh.lattice_bsr((4, 4, 4), 0)
+    ri = ri.copy()
+    J = 5
+    rows = ri[cp[J]:cp[J + 1]]
+    free = [I for I in range(64) if I not in rows]
+    rows[-1] = free[-1]
+    ri[cp[J]:cp[J + 1]] = np.sort(rows)
+    A = oh.Bsr(ctx, 256, cp, ri, bv)
+    lat = np.zeros(3, np.int32)
+    try:
+        oh.check(oh.lib().omx_bsr_lattice(A.h, lat.ctypes.data_as(C.c_void_p)), "bsr_lattice")
+    finally:
+        A.free()
+    assert tuple(lat) == (0, 0, 0)
