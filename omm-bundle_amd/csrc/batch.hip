@@ -644,9 +644,12 @@ static int group_calc_G(omx_group g, int s, const int *mask) {
     return rc;
 }
 
-// Line-search products X*A^T (no x no) of system s, masked.
-static int group_gram(omx_group g, int s, const double *X, const double *Y, double *W) {
+// Line-search products X*Y^T (no x no) of system s, masked; sym: the product is symmetric (X = Y*A
+// with A = H or S, calc_AW, omm_ops.F90:44-64), computed on its lower-triangle tiles and mirrored,
+// as omm()'s symmetric Grams.
+static int group_gram(omx_group g, int s, const double *X, const double *Y, double *W, bool sym = false) {
     const int no = g->no[s], nb = g->nb[s];
+    if (sym) return launch_gemm_sym(g->ctx, OMX_PREC_F64, OMX_F64, no, nb, 1.0, X, no, Y, no, W, no);
     return launch_gemm(g->ctx, OMX_PREC_F64, OMX_F64, false, true, no, no, nb, 1.0, X, no, Y, no, 0.0, W, no);
 }
 
@@ -667,8 +670,8 @@ static int group_sys_step(omx_group g, int s) {
     if (!rc) rc = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_D], no, g->S[s], 0.0, M[M_SG], no);
     if (!rc) rc = group_gram(g, s, M[M_HC], M[M_D], M[M_HWD]);
     if (!rc) rc = group_gram(g, s, M[M_SC], M[M_D], M[M_SWD]);
-    if (!rc) rc = group_gram(g, s, M[M_HG], M[M_D], M[M_HWDD]);
-    if (!rc) rc = group_gram(g, s, M[M_SG], M[M_D], M[M_SWDD]);
+    if (!rc) rc = group_gram(g, s, M[M_HG], M[M_D], M[M_HWDD], true);
+    if (!rc) rc = group_gram(g, s, M[M_SG], M[M_D], M[M_SWDD], true);
     if (!rc)
         rc = launch_coeff_batched(ctx, OMX_F64, no, no, 0, M[M_HW], M[M_SW], M[M_HWD], M[M_SWD], M[M_HWDD],
                                   M[M_SWDD], 1, g->partial[s], g->coeff + 18 * s, act);
@@ -843,8 +846,8 @@ int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps
         ctx->launch_mask = gflag(g, F_ACTIVE, s);
         int r = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_C], no, g->H[s], 0.0, M[M_HC], no);
         if (!r) r = omx_spmm(ctx, OMX_F64, no, nb, 1.0, M[M_C], no, g->S[s], 0.0, M[M_SC], no);
-        if (!r) r = group_gram(g, s, M[M_HC], M[M_C], M[M_HW]);
-        if (!r) r = group_gram(g, s, M[M_SC], M[M_C], M[M_SW]);
+        if (!r) r = group_gram(g, s, M[M_HC], M[M_C], M[M_HW], true);
+        if (!r) r = group_gram(g, s, M[M_SC], M[M_C], M[M_SW], true);
         ctx->launch_mask = nullptr;
         if (!r) r = group_calc_G(g, s, gflag(g, F_ACTIVE, s));
         return r;

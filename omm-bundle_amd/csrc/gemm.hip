@@ -431,8 +431,10 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
                                                                double beta0, const double *__restrict__ C0,
                                                                double beta, double *__restrict__ C, int64_t ldc,
                                                                int sym, int tbase, int nsplit,
-                                                               double *__restrict__ partial) {
+                                                               double *__restrict__ partial,
+                                                               const int *__restrict__ mask) {
     static_assert(CF::BM == 128 && CF::BN == 128, "one 1 KB row per wave load");
+    if (mask && !*mask) return;  // system masked off (grouped solver; whole workgroup, before any barrier)
     constexpr int BK = Prec<OMX_PREC_F64>::BK, NW = CF::WGM * CF::WGN, INS = 2 * BK, LPW = INS / NW;
     constexpr int STAGE = INS * 128;  // doubles: A rows (BK x 128), then B (BK x 128 or 128 x BK)
     static_assert(LPW * NW == INS, "loads per wave");
@@ -565,8 +567,7 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     const bool sym = ctx->gemm_sym && m == n && NPAIR == 1;
     const int tiles = sym ? (m / 128) * (m / 128 + 1) / 2 : (m / 128) * (n / 128);
     if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F64>::BK || lda % 2 || ldb % 2 ||
-        !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || ctx->launch_mask ||
-        tiles < 2 * ctx->num_cus)
+        !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || tiles < 2 * ctx->num_cus)
         return OMX_ERR_UNSUPPORTED;
     const int cls = NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM;
     hipEvent_t ev;
@@ -577,7 +578,7 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     const int slots = 2 * ctx->num_cus;
     const int whole = sym ? tiles / slots * slots : tiles, rest = tiles - whole;
     int nsplit = 1;
-    if (rest > 0) {
+    if (rest > 0 && !ctx->launch_mask) {  // masked (grouped, captured) launches use no workspace
         nsplit = std::max(1, std::min(16, slots / rest));
         while (nsplit > 1 && (k / Prec<OMX_PREC_F64>::BK) % nsplit) --nsplit;
     }
@@ -585,15 +586,17 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
         int rc = ensure_ws(ctx, sizeof(double) * (size_t)rest * nsplit * 128 * 128);
         if (rc) return rc;
         hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(whole), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1,
-                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr);
+                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr,
+                           (const int *)nullptr);
         hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(rest * nsplit), dim3(CF::NT), 0, ctx->stream, m, n, k,
                            alpha, A1, A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, whole, nsplit,
-                           (double *)ctx->ws);
+                           (double *)ctx->ws, (const int *)nullptr);
         hipLaunchKernelGGL(k_dl_split_reduce, dim3(rest, 128 * 128 / 1024), dim3(256), 0, ctx->stream, m, whole, nsplit,
                            (const double *)ctx->ws, alpha, beta0, C0, beta, C, ldc);
     } else {
         hipLaunchKernelGGL((k_dgemm_dl<CF, TRB, NPAIR>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A1,
-                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr);
+                           A2, lda, B1, B2, ldb, beta0, C0, beta, C, ldc, (int)sym, 0, 1, (double *)nullptr,
+                           ctx->launch_mask);
     }
     prof_end(ctx, cls, ev);
     OMX_CHECK_HIP(hipGetLastError());
@@ -1245,9 +1248,11 @@ int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double 
 // 64 x 64 tile pair per workgroup (the upper tile and the lower one it mirrors, the diagonal tiles
 // on themselves), through an LDS transpose so that reads and writes run along columns.
 template <typename TS>
-__global__ __launch_bounds__(256) void k_sym_mirror(int n, TS *__restrict__ C, int64_t ldc) {
+__global__ __launch_bounds__(256) void k_sym_mirror(int n, TS *__restrict__ C, int64_t ldc,
+                                                    const int *__restrict__ mask) {
     constexpr int T = 64;
     __shared__ TS tile[T][T + 1];
+    if (mask && !*mask) return;  // system masked off (whole workgroup, before the barrier)
     int a, b;  // lower tile (a, b), a >= b; it fills the upper tile (b, a)
     tile_of_lower(blockIdx.x, (n + T - 1) / T, a, b);
     const int r0 = a * T, c0 = b * T;  // lower tile origin (rows r0.., columns c0..)
@@ -1276,9 +1281,11 @@ int launch_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, c
     const int nt = (n + 63) / 64;
     const unsigned grid = (unsigned)(nt * (nt + 1) / 2);
     if (dt == OMX_F64)
-        hipLaunchKernelGGL(k_sym_mirror<double>, dim3(grid), dim3(256), 0, ctx->stream, n, (double *)C, ldc);
+        hipLaunchKernelGGL(k_sym_mirror<double>, dim3(grid), dim3(256), 0, ctx->stream, n, (double *)C, ldc,
+                           ctx->launch_mask);
     else
-        hipLaunchKernelGGL(k_sym_mirror<float>, dim3(grid), dim3(256), 0, ctx->stream, n, (float *)C, ldc);
+        hipLaunchKernelGGL(k_sym_mirror<float>, dim3(grid), dim3(256), 0, ctx->stream, n, (float *)C, ldc,
+                           ctx->launch_mask);
     OMX_CHECK_HIP(hipGetLastError());
     return OMX_OK;
 }

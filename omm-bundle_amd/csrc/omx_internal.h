@@ -211,6 +211,11 @@ __device__ inline void tile_of_lower(int w, int nt, int &tm, int &tn) {
 // Launch helpers implemented in the .hip files.
 int launch_gemm(omx_ctx ctx, int prec, int dt, bool trA, bool trB, int m, int n, int k, double alpha,
                 const void *A, int64_t lda, const void *B, int64_t ldb, double beta, void *C, int64_t ldc);
+// C := alpha*A*B^T known to be symmetric (n x n, 'n','t' operands of k columns): the lower-triangle
+// tiles where the kernels allow it, then a mirror of the lower triangle onto the upper (both honour
+// ctx->launch_mask).
+int launch_gemm_sym(omx_ctx ctx, int prec, int dt, int n, int k, double alpha, const void *A, int64_t lda,
+                    const void *B, int64_t ldb, void *C, int64_t ldc);
 int launch_gemm_pair(omx_ctx ctx, int prec, int dt, int m, int n, int k, double alpha, const void *A1,
                      const void *A2, int64_t lda, const void *B1, const void *B2, int64_t ldb, double beta0,
                      const void *C0, double beta, void *C, int64_t ldc);
