@@ -222,7 +222,7 @@ def main():
                          "HW = HC*C^T, the BSR SpMM Y = X*H, the calc_G pair), then 3 plain copies of a no x n matrix "
                          "that calibrate FETCH_SIZE / WRITE_SIZE against known bytes")
     ap.add_argument("--launches", type=int, default=3, help="launches in --kernel mode")
-    ap.add_argument("--kernel-prec", choices=["f64", "f32", "bf16", "bf16x3"], default="f64",
+    ap.add_argument("--kernel-prec", choices=["f64", "f32", "bf16", "bf16x3", "bf16x6"], default="f64",
                     help="product precision in --kernel mode (all but f64 on FP32 storage)")
     ap.add_argument("--spmm-stencil", action="store_true",
                     help="run lattice operators through the 2.5-D stencil SpMM instead of the general BSR kernel")
@@ -524,11 +524,12 @@ def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3, stencil=False):
     followed by `copies` device copies of a no x n FP64 matrix (no*n*8 bytes read and written per
     copy): under rocprofv3 --pmc the copies' counter totals give the scale between a counter and
     bytes, the kernel's dispatches then convert to HBM bytes per launch.  prec other than f64 runs
-    the product on FP32 storage (f32, bf16 or bf16x3 products)."""
+    the product on FP32 storage (f32, bf16, bf16x3 or bf16x6 products)."""
     nb = 4 * L ** 3
     no = nb // 2
     dt = oh.OMX_F64 if prec == "f64" else oh.OMX_F32
-    pr = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3}[prec]
+    pr = {"f64": oh.PREC_F64, "f32": oh.PREC_F32, "bf16": oh.PREC_BF16, "bf16x3": oh.PREC_BF16X3,
+          "bf16x6": oh.PREC_BF16X6}[prec]
     es = 8 if prec == "f64" else 4
     ctx = oh.Ctx(0)
     ctx.spmm_stencil(stencil)

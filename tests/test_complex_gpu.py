@@ -1,15 +1,22 @@
 """Complex (k-point) path on the device: the complex MFMA products, the complex BSR SpMM, the
-complex element-wise and reduction kernels against numpy complex128 references, and complex omm()
-solves against the closed form (sum of the occupied eigenvalues of the Hermitian generalised
-problem H(k) x = lambda S(k) x) and against the real solve at k = 0.
+complex element-wise and reduction kernels against numpy complex128 references; complex omm()
+trajectories against the oracle's complex ('szden') table, itself bit-identical to the reference
+build on the same inputs (the reference's example_kpoints data, a Bloch lattice with S and eta, the
+no-S branch); and complex omm() solves against the closed form (sum of the occupied eigenvalues of
+the Hermitian generalised problem H(k) x = lambda S(k) x) and against the real solve at k = 0.
 
 Tolerances: complex128 paths 1e-12 relative to the operand scale (FMA/summation order); complex64
-paths 1e-4; converged energies 1e-10 relative to the real solve at k = 0 and to the closed form."""
+paths 1e-4; against the oracle (as for the real solves) converged energies 1e-10 relative, per-step
+energies 1e-9 relative, density matrices 1e-8 absolute, CG step counts within 2; converged energies
+1e-10 relative to the real solve at k = 0 and to the closed form."""
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
 
 import ommhip as oh
+import oracle as O
 import pyomm as F
 
 pytestmark = pytest.mark.gpu
@@ -173,6 +180,100 @@ def test_zcoeff(ctx, n):
 def fsetup():
     F.setup(0)
     yield
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _same_trajectory(r, o, steps=20):
+    assert abs(r["e_min"] - o["e_min"]) <= 1e-10 * abs(o["e_min"]), (r["e_min"], o["e_min"])
+    assert abs(r["icg"] - o["icg"]) <= 2, (r["icg"], o["icg"])
+    k = min(len(r["e_log"]), len(o["e_log"]), steps)
+    assert k > 0 and np.allclose(r["e_log"][:k], o["e_log"][:k], rtol=1e-9, atol=0)
+
+
+def test_example_kpoints_device_matches_oracle(fsetup):
+    """example_kpoints.F90:154-305 (flavour 0) on 'gzden': two k-points with their own state
+    (np = 2), the LCG start, the perturbed second SCF step from the previous C, then the
+    energy-weighted density matrices; every call against the oracle's complex table."""
+    _note("example_kpoints device vs oracle")
+    m, n, ks = O.read_example_kpoints(os.path.join(GOLD, "example_gamma.dat"), os.path.join(GOLD, "example_kpoints.dat"))
+    H2s = []
+    for H, _ in ks:
+        H2 = H.copy()
+        H2[np.diag_indices(m)] += -0.001
+        H2s.append(H2)
+    om = O.Omm(2)
+    Co = [np.zeros((n, m), np.complex128, order="F") for _ in ks]
+    Hm = [F.Mat(m, m, "gzden", data=H) for H, _ in ks]
+    Sm = [F.Mat(m, m, "gzden", data=S) for _, S in ks]
+    Cm = [F.Mat(n, m, "gzden") for _ in ks]
+    Dm = [F.Mat(m, m, "gzden") for _ in ks]
+    for step in range(2):
+        for k, (H, S) in enumerate(ks):
+            Hh = H if step == 0 else H2s[k]
+            if step == 1:
+                Hm[k].upload(Hh)
+            o = om(m, n, Hh, S, Co[k], new_S=(step == 0), init_C=(step == 1), ip=k + 1)
+            r = F.omm(m, n, Hm[k], Sm[k], Cm[k], new_S=(step == 0), D_min=Dm[k], init_C=(step == 1), np_=2,
+                      ip=k + 1, storage="gzden")
+            _note("step", step, "k", k, "device", r["e_min"], r["icg"], "oracle", o["e_min"], o["icg"])
+            _same_trajectory(r, o)
+            assert np.max(np.abs(Dm[k].download() - o["D"])) < 1e-8
+            assert np.max(np.abs(Cm[k].download() - Co[k])) < 1e-7
+    for k, (H, S) in enumerate(ks):
+        o = om(m, n, H2s[k], S, Co[k], new_S=False, init_C=True, calc_ED=True, ip=k + 1, dealloc=(k == 1))
+        F.omm(m, n, Hm[k], Sm[k], Cm[k], new_S=False, D_min=Dm[k], calc_ED=True, init_C=True, np_=2, ip=k + 1,
+              dealloc=(k == 1), storage="gzden")
+        assert np.max(np.abs(Dm[k].download() - o["D"])) < 1e-8
+    for M in Hm + Sm + Cm + Dm:
+        M.free()
+
+
+@pytest.mark.parametrize("L,phi,eta", [(3, (0.25, 0.5, 0.75), 0.2), (4, (0.125, 0.0, 0.375), 0.0)])
+def test_complex_lattice_device_matches_oracle(fsetup, L, phi, eta):
+    """Bloch operators H(k), S(k) of the L^3 cell on the device's complex BSR path ('gzcsc') from a
+    complex start, against the oracle's complex trajectory on the same dense operators."""
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    _note("complex lattice device vs oracle", L, phi, eta)
+    Hb, Sb = oh.lattice_bsr_k(L, 0, phi), oh.lattice_bsr_k(L, 1, phi)
+    Hs = (Hb[0], Hb[1], Hb[2] - eta * Sb[2])
+    r0 = np.random.default_rng(L)
+    C0 = np.asfortranarray((r0.uniform(-0.5, 0.5, (no, nb)) + 1j * r0.uniform(-0.5, 0.5, (no, nb))) / np.sqrt(nb))
+    Co = C0.copy(order="F")
+    o = O.Omm()(nb, no, oh.bsr_to_dense(nb, *Hs), oh.bsr_to_dense(nb, *Sb), Co, init_C=True, eta=eta, cg_tol=1e-9,
+                max_iter=300)
+    Hm, Sm = F.Mat.from_bsr_z(nb, *Hs), F.Mat.from_bsr_z(nb, *Sb)
+    Cm, Dm = F.Mat(no, nb, "gzden", data=C0), F.Mat(nb, nb, "gzden")
+    F.set_option("max_iter", 300)
+    try:
+        r = F.omm(nb, no, Hm, Sm, Cm, D_min=Dm, init_C=True, eta=eta, cg_tol=1e-9, dealloc=True, storage="gzden")
+    finally:
+        F.set_option("max_iter", 0)
+    _note("device", r["e_min"], r["icg"], "oracle", o["e_min"], o["icg"])
+    _same_trajectory(r, o, 30)
+    assert np.max(np.abs(Dm.download() - o["D"])) < 1e-7
+    assert abs(r["TrQS"] - o["TrQS"]) < 1e-7
+    for M in (Hm, Sm, Cm, Dm):
+        M.free()
+
+
+def test_complex_no_S_device_matches_oracle(fsetup):
+    L, no = 3, 40
+    nb = 4 * L ** 3
+    _note("complex no_S device vs oracle")
+    H = oh.bsr_to_dense(nb, *oh.lattice_bsr_k(L, 0, (0.25, 0.5, 0.0))) - 0.5 * np.eye(nb)
+    r0 = np.random.default_rng(5)
+    C0 = np.asfortranarray((r0.uniform(-0.5, 0.5, (no, nb)) + 1j * r0.uniform(-0.5, 0.5, (no, nb))) / np.sqrt(nb))
+    Co = C0.copy(order="F")
+    o = O.Omm()(nb, no, H, None, Co, init_C=True)
+    Hm, Cm = F.Mat.from_dense_z(H), F.Mat(no, nb, "gzden", data=C0)
+    r = F.omm(nb, no, Hm, None, Cm, init_C=True, dealloc=True, storage="gzden")
+    _note("device", r["e_min"], r["icg"], "oracle", o["e_min"], o["icg"])
+    _same_trajectory(r, o)
+    assert np.max(np.abs(Cm.download() - Co)) < 1e-7
+    Hm.free()
+    Cm.free()
 
 
 def _exact(Hd, Sd, no, eta):

@@ -7,6 +7,9 @@ small cases, run once in the build container).
     D_min to 1e-8 (its diagonal, its first 8 columns, and 8 +-1 probes scaled by 1/sqrt(n)).
   * 16x16x16 lattice (n=16384, m=8192): the first three CG steps (the initial products, the
     gradient, two rounds of the four Gram products and two line searches) agree to 1e-11 relative.
+  * One k-point of the 10x10x10 cell's 4x4x4 grid, k = (1/4, 1/2, 3/4) (complex128, n=4000,
+    m=2000, eta 0.05): the first three CG steps on the complex path ('gzden' state, complex BSR
+    operators, Hermitian Grams) against the oracle's complex table, to 1e-11 relative.
 
 The inputs are rebuilt from the seeded generator; their sums are checked against the fixture's so
 a generator change cannot pass silently.
@@ -89,5 +92,37 @@ def test_cfg1_16cubed_first_steps_match_oracle():
     assert np.max(np.abs(r["e_log"] - ref_log) / np.abs(ref_log)) <= 1e-11
     assert abs(r["e_min"] - float(fx["e_min"])) <= 1e-11 * abs(float(fx["e_min"]))
     assert abs(r["TrQS"] - float(fx["TrQS"])) <= 1e-7
+    for M in (H, S, Cm):
+        M.free()
+
+
+def test_cfg3_kpoint_of_the_10cubed_cell_first_steps_match_oracle():
+    fx = np.load(os.path.join(GOLD, "omm_cfg3_L10k_3steps.npz"))
+    L = int(fx["L"])
+    nb, no = 4 * L ** 3, 2 * L ** 3
+    phi = tuple(float(x) for x in fx["phi"])
+    eta = float(fx["eta"])
+    _note("cfg3 k-point", phi, "of the 10x10x10 cell, n =", nb, "first", int(fx["max_iter"]), "steps")
+    Hb, Sb = oh.lattice_bsr_k(L, 0, phi), oh.lattice_bsr_k(L, 1, phi)
+    hv = Hb[2] - eta * Sb[2]
+    assert complex(hv.sum()) == complex(fx["h_sum"])
+    C0 = oh.random_c0(no, nb).astype(np.complex128)
+    assert complex(C0.sum()) == complex(fx["c0_sum"])
+    H, S = F.Mat.from_bsr_z(nb, Hb[0], Hb[1], hv), F.Mat.from_bsr_z(nb, *Sb)
+    Cm = F.Mat(no, nb, "gzden", data=C0)
+    F.set_option("max_iter", int(fx["max_iter"]))
+    try:
+        r = F.omm(nb, no, H, S, Cm, init_C=True, eta=eta, cg_tol=float(fx["cg_tol"]), dealloc=True, storage="gzden")
+    finally:
+        F.set_option("max_iter", 0)
+    ref_log = fx["e_log"]
+    _note("cfg3 device", r["e_log"].tolist(), "oracle", ref_log.tolist())
+    assert r["icg"] == int(fx["icg"]) == len(ref_log)
+    assert np.max(np.abs(r["e_log"] - ref_log) / np.abs(ref_log)) <= 1e-11
+    assert abs(r["e_min"] - float(fx["e_min"])) <= 1e-11 * abs(float(fx["e_min"]))
+    assert abs(r["TrQS"] - float(fx["TrQS"])) <= 1e-7
+    P = np.where(np.random.default_rng(7).random((nb, 8)) < 0.5, -1.0, 1.0)
+    Cp = Cm.download() @ P
+    assert np.max(np.abs(Cp - fx["C_probe"])) / np.sqrt(nb) < 1e-9
     for M in (H, S, Cm):
         M.free()
