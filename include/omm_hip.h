@@ -361,8 +361,9 @@ int omx_group_solve(omx_group g, const double *eta, double cg_tol, int max_steps
 /* All FP64 column-major; U is upper triangular and only its upper triangle is read. */
 
 /* A = U^T U in place, upper (m_factorize, libOMM/src/omm_ops.F90:246-315, dpotrf 'u').  *info > 0
- * names the first non-positive pivot (1-based).  As with dpotrf 'u', the strictly lower triangle
- * is left as it was (the library saves it in a temporary n x n device buffer and restores it).
+ * names the first non-positive pivot (1-based).  As with dpotrf 'u', only the upper triangle is read
+ * and written; the strictly lower triangle is left as it was (the factorisation runs on a transposed
+ * copy of the upper triangle in a temporary device buffer of order n rounded up to 128).
  * Synchronises. */
 int omx_potrf(omx_ctx ctx, int dt, int n, void *A, int64_t lda, int *info);
 /* B := alpha*op(U)^-1*B (side 'l') or alpha*B*op(U)^-1 (side 'r'), op 'n' or 't' (dtrsm; used by

@@ -405,7 +405,7 @@ static double sym_area(int n, int bm) {
 // ---------------------------------------------------------------------------------------------
 // FP64 products with direct-to-LDS staging (configuration 7): C := alpha*sum_p A_p*op(B_p) + beta0*C0
 // + beta*C over NPAIR products back to back along K (the Gram: NPAIR = 1, op 't'; calc_G's pair:
-// NPAIR = 2, op 'n').  A is m x k with the row index contiguous, so every k-row of a 128-wide tile
+// NPAIR = 2, op 'n'; single 'n','n' products: NPAIR = 1, op 'n').  A is m x k with the row index contiguous, so every k-row of a 128-wide tile
 // is 1 KB, which one wave moves with one global_load_lds_dwordx4 (16 bytes per lane, lane l landing
 // at byte 16*l of the destination, checked on the device) straight into the k-major LDS tile.
 //   B 't' (n x k, row index contiguous): the same k-major rows.
@@ -656,12 +656,12 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
     return OMX_OK;
 }
 
-// Tile configurations that apply to a product: the FP64 Gram and gradient products carry 0-3 and
-// 5-7 (7 on FP64 storage); the FP32 and BF16 ones on FP32 storage 0 and 4; every other product
-// uses 0.
+// Tile configurations that apply to a product: FP64 products with A not transposed ('n','t' Grams,
+// 'n','n' products and the calc_G pair) carry 0-3 and 5-7 (7 on FP64 storage); the FP32 and BF16
+// ones on FP32 storage 0 and 4; every other product uses 0.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
 struct CfgSet {
-    static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA && (TRB || NPAIR == 2);
+    static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA;
     static constexpr bool dl = f64 && std::is_same<TS, double>::value;
     static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
     static std::vector<int> ids() {

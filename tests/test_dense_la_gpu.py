@@ -1,6 +1,7 @@
 """Dense factorisations of the Cholesky and preconditioned flavours (omx_potrf, omx_trsm, omx_trmm,
-omx_sygst, omx_inverse_spd) against numpy/scipy FP64, on sizes that exercise partial 32-blocks.
-Tolerance 1e-11 relative to the operand scale (blocked FMA order against LAPACK's)."""
+omx_sygst, omx_inverse_spd) against numpy/scipy FP64, on orders that exercise partial 64- and
+128-wide diagonal blocks and more than one 512-wide panel.  Tolerance 1e-11 relative to the
+operand scale (blocked order and block inverses against LAPACK's substitution)."""
 import ctypes as C
 
 import numpy as np
@@ -18,7 +19,7 @@ def _spd(n):
     return A @ A.T / n + np.eye(n)
 
 
-@pytest.mark.parametrize("n", [1, 31, 64, 100, 257, 600])
+@pytest.mark.parametrize("n", [1, 31, 64, 100, 257, 600, 1100])
 def test_potrf(ctx, n):
     print("CASE potrf", n, flush=True)
     S = _spd(n)
@@ -37,10 +38,11 @@ def test_potrf(ctx, n):
     assert np.array_equal(np.tril(dS2.download(), -1), np.tril(S2, -1))
 
 
-@pytest.mark.parametrize("n,bad", [(40, 35), (600, 400)])
+@pytest.mark.parametrize("n,bad", [(40, 35), (600, 400), (700, 600)])
 def test_potrf_not_spd(ctx, n, bad):
     """A negative diagonal entry makes that pivot the first non-positive one (earlier pivots do not
-    depend on it): info names it, 1-based, also past the first 256-wide outer block."""
+    depend on it): info names it, 1-based, also past the first 64-wide diagonal blocks and past the
+    first 512-wide panel."""
     print("CASE potrf not spd", n, bad, flush=True)
     S = _spd(n)
     S[bad, bad] = -1.0
@@ -52,7 +54,7 @@ def test_potrf_not_spd(ctx, n, bad):
 
 @pytest.mark.parametrize("side", ["l", "r"])
 @pytest.mark.parametrize("trans", ["n", "t"])
-@pytest.mark.parametrize("nu", [75, 600])
+@pytest.mark.parametrize("nu", [75, 600, 1100])
 def test_trsm_trmm(ctx, side, trans, nu):
     other = 50
     print("CASE trsm", side, trans, nu, flush=True)
@@ -73,7 +75,7 @@ def test_trsm_trmm(ctx, side, trans, nu):
     assert np.max(np.abs(dB.download() - ref)) < 1e-12 * np.max(np.abs(ref)) * nu
 
 
-@pytest.mark.parametrize("n", [90, 600])
+@pytest.mark.parametrize("n", [90, 600, 1100])
 def test_sygst_and_inverse(ctx, n):
     print("CASE sygst/inverse", n, flush=True)
     S = _spd(n)
