@@ -19,7 +19,7 @@ def _spd(n):
     return A @ A.T / n + np.eye(n)
 
 
-@pytest.mark.parametrize("n", [1, 31, 64, 100, 257, 600, 1100])
+@pytest.mark.parametrize("n", [1, 31, 64, 100, 257, 600, 1100, 1600])
 def test_potrf(ctx, n):
     print("CASE potrf", n, flush=True)
     S = _spd(n)
@@ -38,7 +38,7 @@ def test_potrf(ctx, n):
     assert np.array_equal(np.tril(dS2.download(), -1), np.tril(S2, -1))
 
 
-@pytest.mark.parametrize("n,bad", [(40, 35), (600, 400), (700, 600)])
+@pytest.mark.parametrize("n,bad", [(40, 35), (600, 400), (700, 600), (1600, 1300)])
 def test_potrf_not_spd(ctx, n, bad):
     """A negative diagonal entry makes that pivot the first non-positive one (earlier pivots do not
     depend on it): info names it, 1-based, also past the first 64-wide diagonal blocks and past the
