@@ -36,14 +36,13 @@ constexpr int LB_NT = 1024;   // threads of the diagonal-block kernels (a 32 x 3
 constexpr int LA_NT = 256;    // threads of the element-wise kernels
 constexpr int TT = 64;        // transpose tile
 
-// Register layout of the diagonal-block kernels (order DB = 64 or 128): thread (tx, ty) of a TG x TG
-// grid owns the elements of rows tx + TG a and columns ty + TG b (a, b < DB / TG) of the block: four
-// waves per SIMD, so the short dependent chains of one column step overlap across waves.  Row or
-// column groups that a step cannot touch are skipped by uniform branches.  The column steps are
-// sequential (latency-bound), so potrf uses the 64-wide form (DC) and the triangular solves the
-// 128-wide one (LB).
+// Register layout of the block inverse of the triangular solves (order DB = LB): thread (tx, ty) of a
+// TG x TG grid owns the elements of rows tx + TG a and columns ty + TG b (a, b < DB / TG): four waves
+// per SIMD, so the short dependent chains of one row step overlap across waves.  Row or column
+// groups that a step cannot touch are skipped by uniform branches.
 constexpr int TG = 32;
-constexpr int DC = 64;
+constexpr int DC = 64;  // potrf's diagonal block: one wave, one row per lane
+static_assert(DC == 64, "k_chol_diag_wave maps one row to each lane of a 64-wide wave");
 
 // X = L^-1 for the lower-triangular LB x LB matrix L in LDS (a, column-major, stride LB_S; rdiag[k] =
 // 1 / L(k, k)), by right-looking forward substitution of L X = I in registers.  At row k the owners
@@ -103,100 +102,65 @@ __device__ void reg_lower_inverse(const double *a, const double *rdiag, double (
     }
 }
 
-// Lower Cholesky of the DB x DB diagonal block of W at (k0, k0), in place (lower triangle), and its
-// inverse into LI (DB x DB, zeros above the diagonal).  W is padded with the identity beyond n, so
-// the block is always whole.  *info := global index + 1 of the first non-positive pivot below n (a
-// non-positive pivot is replaced by 1 so the remaining blocks stay finite).
-// Right-looking in registers: at column k the owners of column k publish it to col[k & 1] and the
-// owner of the pivot publishes sqrt and reciprocal; after one barrier every thread scales what it
-// needs and updates its own elements, and the owners keep the scaled column.  A col buffer is
-// rewritten two columns later, after every thread has passed the barrier that follows its last reads.
-template <int DB>
-__global__ __launch_bounds__(LB_NT) void k_chol_diag(double *__restrict__ W, int64_t ldw, int k0, int n,
-                                                     double *__restrict__ LI, int *__restrict__ info) {
-    constexpr int RG = DB / TG, DS = DB + 1;
-    __shared__ double a[DB * DS];
-    __shared__ double col[2][DB];
-    __shared__ double piv[2][2];
-    __shared__ double rdiag[DB];
-    __shared__ int bad;
-    const int t = threadIdx.x, tx = t % TG, ty = t / TG;
+// v broadcast from lane `lane` (a compile-time constant in the callers) through two scalar reads.
+__device__ inline double bcast_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The DC x DC diagonal block of W at (k0, k0) by one wave, without barriers: lane i keeps row i of
+// the block in registers.  Lower Cholesky, right-looking, with the pivot and L(j, k) broadcast from
+// the lanes that hold them; then X = L^-1 by forward substitution of L X = I, row k of X broadcast
+// from lane k, with L's columns read back from LDS.  L goes to W's lower triangle, X into LI (DC x DC,
+// zeros above the diagonal); *info := global index + 1 of the first non-positive pivot below n (a
+// non-positive pivot is replaced by 1 so the remaining blocks stay finite).  W is padded with the
+// identity beyond n, so the block is always whole.
+// Elements above the diagonal take harmless updates and are never written out.
+__global__ __launch_bounds__(64) void k_chol_diag_wave(double *__restrict__ W, int64_t ldw, int k0, int n,
+                                                       double *__restrict__ LI, int *__restrict__ info) {
+    constexpr int DS = DC + 1;
+    __shared__ double a[DC * DS];
+    const int i = threadIdx.x;
     double *Wkk = W + k0 + (int64_t)k0 * ldw;
-    double r[RG][RG];
+    double r[DC];
 #pragma unroll
-    for (int ai = 0; ai < RG; ++ai)
+    for (int j = 0; j < DC; ++j) r[j] = (i >= j) ? Wkk[i + (int64_t)j * ldw] : 0.0;
+    int bad = 0;
 #pragma unroll
-        for (int bj = 0; bj < RG; ++bj) {
-            const int i = tx + TG * ai, j = ty + TG * bj;
-            r[ai][bj] = (i >= j) ? Wkk[i + (int64_t)j * ldw] : 0.0;
-        }
-    if (t == 0) bad = 0;
-#pragma unroll 1
-    for (int k = 0; k < DB; ++k) {
-        const int kg = k / TG, kr = k % TG;
-        double *ck = col[k & 1];
-        if (ty == kr) {
+    for (int k = 0; k < DC; ++k) {
+        const double d = bcast_lane(r[k], k);
+        if (!(d > 0.0) && !bad && k0 + k < n) bad = k0 + k + 1;
+        const double dk = sqrt(d > 0.0 ? d : 1.0), rdk = 1.0 / dk;
+        const double li = r[k] * rdk;  // L(i, k) for i > k
+        r[k] = (i == k) ? dk : (i > k ? li : r[k]);
 #pragma unroll
-            for (int bj = 0; bj < RG; ++bj) {
-                if (bj != kg) continue;
+        for (int j = k + 1; j < DC; ++j) r[j] = fma(-li, bcast_lane(li, j), r[j]);
+    }
 #pragma unroll
-                for (int ai = 0; ai < RG; ++ai) {
-                    const int i = tx + TG * ai;
-                    if (i >= k) ck[i] = r[ai][bj];
-                    if (i == k) {  // the pivot's owner
-                        const double d = r[ai][bj];
-                        if (!(d > 0.0) && !bad && k0 + k < n) bad = k0 + k + 1;
-                        const double dk = sqrt(d > 0.0 ? d : 1.0);
-                        piv[k & 1][0] = dk;
-                        piv[k & 1][1] = 1.0 / dk;
-                        rdiag[k] = 1.0 / dk;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // all reads first (LDS, always in bounds); the rank-1 update then runs unpredicated with
-        // multipliers that are zero for rows or columns <= k (r - 0*y = r exactly; elements above
-        // the diagonal are never read)
-        const double dk = piv[k & 1][0], rdk = piv[k & 1][1];
-        double li[RG], lj[RG];
+    for (int j = 0; j < DC; ++j) {
+        if (i >= j) Wkk[i + (int64_t)j * ldw] = r[j];
+        a[j * DS + i] = (i >= j) ? r[j] : 0.0;  // column j of L
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes are done
+    __builtin_amdgcn_wave_barrier();
+    double x[DC];
 #pragma unroll
-        for (int q = 0; q < RG; ++q) {
-            const int i = tx + TG * q, j = ty + TG * q;
-            const double ci = ck[i], cj = ck[j];
-            li[q] = (i > k) ? ci * rdk : 0.0;
-            lj[q] = (j > k) ? cj * rdk : 0.0;
-        }
+    for (int j = 0; j < DC; ++j) x[j] = (i == j) ? 1.0 : 0.0;
 #pragma unroll
-        for (int bj = 0; bj < RG; ++bj) {
-            if (TG * bj + TG - 1 < k) continue;  // column groups left of k are final (uniform)
-            const int j = ty + TG * bj;
+    for (int k = 0; k < DC; ++k) {
+        const double lik = (i > k) ? a[k * DS + i] : 0.0;  // L(i, k) below the diagonal
+        const double s = (i == k) ? 1.0 / a[k * DS + k] : 1.0;
 #pragma unroll
-            for (int ai = 0; ai < RG; ++ai) {
-                if (TG * ai + TG - 1 < k) continue;  // row groups above k are never read again (uniform)
-                const int i = tx + TG * ai;
-                const double upd = fma(-li[ai], lj[bj], r[ai][bj]);
-                r[ai][bj] = (j == k) ? ((i == k) ? dk : li[ai]) : upd;
-            }
+        for (int j = 0; j <= k; ++j) {
+            x[j] *= s;  // lane k finishes row k of X
+            x[j] = fma(-lik, bcast_lane(x[j], k), x[j]);
         }
     }
 #pragma unroll
-    for (int ai = 0; ai < RG; ++ai)
-#pragma unroll
-        for (int bj = 0; bj < RG; ++bj) {
-            const int i = tx + TG * ai, j = ty + TG * bj;
-            if (i >= j) {
-                Wkk[i + (int64_t)j * ldw] = r[ai][bj];
-                a[j * DS + i] = r[ai][bj];
-            }
-        }
-    __syncthreads();
-    reg_lower_inverse<DB>(a, rdiag, col, r);
-#pragma unroll
-    for (int ai = 0; ai < RG; ++ai)
-#pragma unroll
-        for (int bj = 0; bj < RG; ++bj) LI[(tx + TG * ai) + (ty + TG * bj) * DB] = r[ai][bj];
-    if (t == 0 && bad && *info == 0) *info = bad;
+    for (int j = 0; j < DC; ++j) LI[i + j * DC] = x[j];
+    if (i == 0 && bad && *info == 0) *info = bad;
 }
 
 // Inverses of the diagonal blocks of the upper-triangular U (order nu): block b (rows and columns
@@ -481,7 +445,7 @@ static int potrf_impl(omx_ctx ctx, int n, double *A, int64_t lda, int *info) {
     for (int o0 = 0; o0 < N; o0 += LB_OUT) {
         const int end = std::min(N, o0 + LB_OUT);
         for (int k0 = o0; k0 < end; k0 += DC) {
-            hipLaunchKernelGGL(k_chol_diag<DC>, dim3(1), dim3(LB_NT), 0, ctx->stream, W.p, (int64_t)N, k0, n, LI.p,
+            hipLaunchKernelGGL(k_chol_diag_wave, dim3(1), dim3(64), 0, ctx->stream, W.p, (int64_t)N, k0, n, LI.p,
                                d_info);
             const int rest = N - k0 - DC;
             if (rest == 0) break;
