@@ -1005,8 +1005,8 @@ extern "C" int omx_spmm(omx_ctx ctx, int dt, int no, int nb, double alpha, const
 // Mapping: a workgroup owns a chunk of SD_NT*VEC*SD_STR states and one block column J; it holds
 // B's four columns 4J..4J+3 of the chunk in registers and, per stored block, reads A's four
 // columns, accumulates the 16 products per thread over its SD_STR strides and reduces them over
-// the workgroup once.  Per-chunk partial blocks go to the
-// workspace and a second kernel adds them in chunk order (deterministic).
+// each wave (a reduce-scatter, no barrier).  Per-wave partial blocks go to the workspace and a
+// second kernel adds them in (chunk, wave) order (deterministic).
 // =============================================================================================
 namespace omx {
 
@@ -1022,7 +1022,6 @@ __global__ __launch_bounds__(SD_NT) void k_sddmm_bsr4_partial(int no, int nb, co
                                                               int64_t nnzb, double *__restrict__ partial) {
     constexpr int ROWS = SD_NT * VEC * SD_STR;
     const int rc = blockIdx.x % rchunks, J = blockIdx.x / rchunks;
-    __shared__ double red[16][SD_NT / 64];
     double bcol[SD_STR][4][VEC];
 #pragma unroll
     for (int st = 0; st < SD_STR; ++st) {
@@ -1060,30 +1059,42 @@ __global__ __launch_bounds__(SD_NT) void k_sddmm_bsr4_partial(int no, int nb, co
 #pragma unroll
                     for (int q = 0; q < VEC; ++q) s[r + 4 * c] = fma(acol[r][q], bcol[st][c][q], s[r + 4 * c]);
         }
+        // Reduce-scatter of the 16 sums over the wave by recursive halving: at offset 32 a lane keeps
+        // half of its values and adds its partner's copy of that half (8 shuffles), then 4, 2 and 1;
+        // lane l ends with output (l >> 2) & 15, and two more steps add the four lanes that share
+        // it.  17 shuffles instead of 16 x 6; the order is fixed, and each wave writes its own
+        // partial block, so no barrier is needed.
+        double h8[8], h4[4], h2[2];
+        const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            double t = s[e];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-            if (lane == 0) red[e][wave] = t;
+        for (int e = 0; e < 8; ++e) {
+            const double keep = b5 ? s[e + 8] : s[e], send = b5 ? s[e] : s[e + 8];
+            h8[e] = keep + __shfl_xor(send, 32, 64);
         }
-        __syncthreads();
-        if (threadIdx.x < 16) {
-            double t = 0.0;
 #pragma unroll
-            for (int w = 0; w < SD_NT / 64; ++w) t += red[threadIdx.x][w];
-            partial[((int64_t)rc * nnzb + l) * 16 + threadIdx.x] = t;
+        for (int e = 0; e < 4; ++e) {
+            const double keep = b4 ? h8[e + 4] : h8[e], send = b4 ? h8[e] : h8[e + 4];
+            h4[e] = keep + __shfl_xor(send, 16, 64);
         }
-        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const double keep = b3 ? h4[e + 2] : h4[e], send = b3 ? h4[e] : h4[e + 2];
+            h2[e] = keep + __shfl_xor(send, 8, 64);
+        }
+        double t = (b2 ? h2[1] : h2[0]) + __shfl_xor(b2 ? h2[0] : h2[1], 4, 64);
+        t += __shfl_xor(t, 1, 64);
+        t += __shfl_xor(t, 2, 64);
+        if ((lane & 3) == 0)
+            partial[(((int64_t)rc * (SD_NT / 64) + wave) * nnzb + l) * 16 + ((lane >> 2) & 15)] = t;
     }
 }
 
-__global__ void k_sddmm_bsr4_final(int64_t nnzb, int rchunks, const double *__restrict__ partial, double alpha,
+__global__ void k_sddmm_bsr4_final(int64_t nnzb, int nparts, const double *__restrict__ partial, double alpha,
                                    double beta, double *__restrict__ vals) {
     const int64_t total = nnzb * 16;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (int rc = 0; rc < rchunks; ++rc) s += partial[(int64_t)rc * total + q];
+        for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * total + q];
         vals[q] = (beta == 0.0) ? alpha * s : alpha * s + beta * vals[q];
     }
 }
@@ -1095,7 +1106,8 @@ static int run_sddmm(omx_ctx ctx, int no, int nb, double alpha, const T *A, int6
     const int rchunks = (no + ROWS - 1) / ROWS;
     const int64_t grid = (int64_t)rchunks * C->nbc;
     OMX_REQUIRE(grid < (1LL << 31), "omx_sddmm_bsr: grid too large");
-    const size_t bytes = sizeof(double) * 16 * (size_t)C->nnzb * rchunks;
+    const int nparts = rchunks * (SD_NT / 64);  // one partial block per wave of every chunk
+    const size_t bytes = sizeof(double) * 16 * (size_t)C->nnzb * nparts;
     int rc = ensure_ws(ctx, bytes);
     if (rc) return rc;
     double *partial = (double *)ctx->ws;
@@ -1106,7 +1118,7 @@ static int run_sddmm(omx_ctx ctx, int no, int nb, double alpha, const T *A, int6
                        lda, B, ldb, C->d_colptr, C->d_rowidx, rchunks, C->nnzb, partial);
     const int64_t total = C->nnzb * 16;
     const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_sddmm_bsr4_final, dim3(g2), dim3(256), 0, ctx->stream, C->nnzb, rchunks, partial, alpha, beta,
+    hipLaunchKernelGGL(k_sddmm_bsr4_final, dim3(g2), dim3(256), 0, ctx->stream, C->nnzb, nparts, partial, alpha, beta,
                        C->d_val);
     prof_end(ctx, OMX_K_SPMM, ev);
     OMX_CHECK_HIP(hipGetLastError());
