@@ -26,6 +26,14 @@ for s in "${SRC[@]}"; do
 done
 for p in "${pids[@]}"; do wait "$p"; done
 "$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$HERE/libommhip.so.tmp" "${objs[@]}"
+# A kernel whose host launch stub the compiler failed to emit leaves an undefined
+# __device_stub__ symbol that only shows when the library is loaded: refuse such a build.
+if nm -u "$HERE/libommhip.so.tmp" | grep -q "__device_stub__"; then
+  echo "build.sh: undefined kernel launch stubs in libommhip.so:" >&2
+  nm -u -C "$HERE/libommhip.so.tmp" | grep "__device_stub__" >&2
+  rm -f "$HERE/libommhip.so.tmp"
+  exit 1
+fi
 mv "$HERE/libommhip.so.tmp" "$HERE/libommhip.so"
 echo "built $HERE/libommhip.so"
 

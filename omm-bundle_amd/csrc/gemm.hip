@@ -405,13 +405,19 @@ static double sym_area(int n, int bm) {
 // ---------------------------------------------------------------------------------------------
 // FP64 products with direct-to-LDS staging (configuration 7): C := alpha*sum_p A_p*op(B_p) + beta0*C0
 // + beta*C over NPAIR products back to back along K (the Gram: NPAIR = 1, op 't'; calc_G's pair:
-// NPAIR = 2, op 'n'; single 'n','n' products: NPAIR = 1, op 'n').  A is m x k with the row index contiguous, so every k-row of a 128-wide tile
-// is 1 KB, which one wave moves with one global_load_lds_dwordx4 (16 bytes per lane, lane l landing
-// at byte 16*l of the destination, checked on the device) straight into the k-major LDS tile.
+// NPAIR = 2, op 'n'; single 'n','n' products: NPAIR = 1, op 'n').  A is m x k with the row index
+// contiguous, so every k-row of a 128-wide tile is 1 KB, which one wave moves with one
+// buffer_load_dwordx4 ... lds (16 bytes per lane, lane l landing at byte 16*l of the destination)
+// straight into the k-major LDS tile.  The descriptors are rebased at each slice, so the lanes'
+// byte offsets are small and fixed; the buffer form keeps the compiler's LDS waits counted (see
+// k_bgemm_tn), and the fragments of k step ks + 4 are read before the MFMAs of step ks.
 //   B 't' (n x k, row index contiguous): the same k-major rows.
 //   B 'n' (k x n, k contiguous): a j-major tile, one 128-byte row of 16 k values per column j, 8
 //   columns per wave load.  Logical 16-byte chunk c of column j is stored at physical chunk
-//   c ^ ((j >> 1) & 7), so the 16 columns one fragment read touches fall in distinct bank groups.
+//   c ^ ((j >> 1) & 7).  A fragment read takes one k of 16 columns, 8 bytes at the same place in
+//   16 different 16-byte chunks, so it spans at most 8 of the 16-byte bank groups: two-way bank
+//   conflicts that no placement of whole chunks avoids (measured: conflict cycles on 40 % of the
+//   LDS-active cycles with this swizzle and with c ^ (j & 7); none on the 't' path).
 // No staging registers and no LDS write instructions; two LDS stages, the loads of slice s+1 in
 // flight while slice s feeds the MFMAs.  Tiles, waves and the MFMA chain per element are those of
 // the staged kernel with Cfg5, so the results are bitwise the same.  Requires m, n % 128 == 0,
@@ -421,6 +427,10 @@ typedef __attribute__((address_space(3))) void lds_void;
 // s_waitcnt immediate (gfx9 encoding): vmcnt(vm), expcnt and lgkmcnt left at their maxima
 constexpr int waitcnt_vmcnt(int vm) { return (vm & 0xF) | ((vm >> 4) << 14) | (7 << 4) | (0xF << 8); }
 __device__ inline int dl_swz(int j) { return (j >> 1) & 7; }
+// Raw buffer descriptor at p (gfx9 descriptor word 3); the lanes' offsets stay below 2^31.
+__device__ inline __amdgpu_buffer_rsrc_t dl_rsrc(const double *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), (short)0, 0x7fffffff, 0x00020000);
+}
 
 template <class CF, bool TRB, int NPAIR>
 __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int k, double alpha,
@@ -455,6 +465,23 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
 #pragma unroll
         for (int b = 0; b < CF::FBN; ++b) acc[a][b] = d4v{0, 0, 0, 0};
     const int kpart = NPAIR * k / nsplit, s0 = z * (kpart / BK);  // this part's first slice
+    // Each lane's byte offset from its slice's descriptor base (A + i0 + kl*lda; B 't': B + j0 +
+    // kl*ldb; B 'n': B + j0*ldb + kl), the same for every slice.
+    const int uwave = __builtin_amdgcn_readfirstlane(wave);
+    int voff[LPW];
+#pragma unroll
+    for (int q = 0; q < LPW; ++q) {
+        const int ins = uwave * LPW + q;  // < BK: row k of A; else the B part
+        if (ins < BK) {
+            voff[q] = 8 * (2 * lane + ins * (int)lda);
+        } else if (TRB) {
+            voff[q] = 8 * (2 * lane + (ins - BK) * (int)ldb);
+        } else {
+            const int j = (ins - BK) * 8 + (lane >> 3);  // column of the tile
+            const int c = (lane & 7) ^ dl_swz(j);         // logical chunk this lane holds
+            voff[q] = 8 * (j * (int)ldb + 2 * c);
+        }
+    }
     auto issue = [&](int s) {
         double *st = smem + (s & 1) * STAGE;
         int kl = (s0 + s) * BK;
@@ -464,20 +491,16 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
             B = B2;
             kl -= k;
         }
+        const __amdgpu_buffer_rsrc_t ra = dl_rsrc(A + i0 + (int64_t)kl * lda);
+        const __amdgpu_buffer_rsrc_t rb = dl_rsrc(TRB ? B + j0 + (int64_t)kl * ldb : B + (int64_t)j0 * ldb + kl);
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
-            const int ins = wave * LPW + q;  // < BK: row k of A; else the B part
-            const double *src;
-            if (ins < BK) {
-                src = A + (int64_t)(i0 + 2 * lane) + (int64_t)(kl + ins) * lda;
-            } else if (TRB) {
-                src = B + (int64_t)(j0 + 2 * lane) + (int64_t)(kl + ins - BK) * ldb;
-            } else {
-                const int j = (ins - BK) * 8 + (lane >> 3);  // column of the tile
-                const int c = (lane & 7) ^ dl_swz(j);         // logical chunk this lane holds
-                src = B + (int64_t)(j0 + j) * ldb + kl + 2 * c;
-            }
-            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + ins * 128), 16, 0, 0);
+            const int ins = uwave * LPW + q;
+            const int vo = voff[q];  // (a local: see k_bgemm_tn)
+            if (ins < BK)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(st + ins * 128), 16, vo, 0, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void *)(st + ins * 128), 16, vo, 0, 0, 0);
         }
     };
     const int li = lane & 15, lk = lane >> 4;
@@ -488,21 +511,29 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
         __syncthreads();                               // everyone's have; slice s-1 is no longer read
         if (s + 1 < ns) issue(s + 1);
         const double *As = smem + (s & 1) * STAGE, *Bs = As + BK * 128;
+        // the fragments of k step ks + 4 are read before the MFMAs of step ks
+        double a[2][CF::FBM], b[2][CF::FBN];
+        auto frag = [&](int ks, double (&fa)[CF::FBM], double (&fb)[CF::FBN]) {
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 4) {
-            double a[CF::FBM], b[CF::FBN];
-#pragma unroll
-            for (int f = 0; f < CF::FBM; ++f) a[f] = As[(ks + lk) * 128 + wm * CF::WM + f * 16 + li];
+            for (int f = 0; f < CF::FBM; ++f) fa[f] = As[(ks + lk) * 128 + wm * CF::WM + f * 16 + li];
 #pragma unroll
             for (int f = 0; f < CF::FBN; ++f) {
                 const int j = wn * CF::WN + f * 16 + li, kk = ks + lk;
-                b[f] = TRB ? Bs[kk * 128 + j] : Bs[j * BK + ((kk >> 1) ^ dl_swz(j)) * 2 + (kk & 1)];
+                fb[f] = TRB ? Bs[kk * 128 + j] : Bs[j * BK + ((kk >> 1) ^ dl_swz(j)) * 2 + (kk & 1)];
             }
+        };
+        frag(0, a[0], b[0]);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int cur = (ks / 4) & 1;
+            if (ks + 4 < BK) frag(ks + 4, a[cur ^ 1], b[cur ^ 1]);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int fm = 0; fm < CF::FBM; ++fm)
 #pragma unroll
                 for (int fn = 0; fn < CF::FBN; ++fn)
-                    acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][fm], b[cur][fn], acc[fm][fn], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     const int lq = lane >> 4;
@@ -567,7 +598,8 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     const bool sym = ctx->gemm_sym && m == n && NPAIR == 1;
     const int tiles = sym ? (m / 128) * (m / 128 + 1) / 2 : (m / 128) * (n / 128);
     if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F64>::BK || lda % 2 || ldb % 2 ||
-        !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || tiles < 2 * ctx->num_cus)
+        !aligned16(A1) || !aligned16(A2) || !aligned16(B1) || !aligned16(B2) || tiles < 2 * ctx->num_cus ||
+        lda > (1 << 20) || ldb > (1 << 20))  // 32-bit lane offsets from the slice descriptors
         return OMX_ERR_UNSUPPORTED;
     const int cls = NPAIR == 2 ? OMX_K_GEMM_PAIR : OMX_K_GEMM;
     hipEvent_t ev;
@@ -931,6 +963,12 @@ __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *_
 // distinct 16-byte bank groups, so the read is free of bank conflicts.
 __device__ inline int bg_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
 
+// Raw buffer descriptor over `elems` BF16 values from p (gfx9 descriptor word 3: 32-bit data,
+// no swizzle).  Reads past the range return zero instead of touching memory.
+__device__ inline __amdgpu_buffer_rsrc_t bg_rsrc(const __bf16 *p, int64_t elems) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16 *>(p), (short)0, (int)(2 * elems), 0x00020000);
+}
+
 // One BK = 64 slice of the BF16 panel product, software-pipelined: the wave's MFMAs run in groups
 // (one A fragment x the FBN B fragments of a 32-wide k step), and the LDS reads for group g + LEAD
 // are issued before the MFMAs of group g, so that LEAD groups of reads are in flight behind the
@@ -973,10 +1011,15 @@ __device__ inline void mma_slice_bf16_pipe(const __bf16 *As, const __bf16 *Bs, f
 // lda, ldb multiples of 8, 16-byte aligned panels), FP32 accumulation; C, C0 of type TO (the
 // storage type, FP32 or FP64).
 //
-// Operand rows go straight from the panels into LDS: one global_load_lds_dwordx4 per wave moves 8
-// rows of a slice (1 KB, lane l landing at byte 16*l), each lane fetching the logical chunk that
+// Operand rows go straight from the panels into LDS: one buffer_load_dwordx4 ... lds per wave moves
+// 8 rows of a slice (1 KB, lane l landing at byte 16*l), each lane fetching the logical chunk that
 // belongs at its physical position under bg_swz.  No staging registers and no LDS write
 // instructions; two LDS stages, the loads of slice s+1 in flight while slice s feeds the MFMAs.
+// The buffer form rather than global_load_lds: while a FLAT-encoded load into LDS is in flight the
+// compiler's wait insertion cannot count LDS reads in order and drains every one of them
+// (lgkmcnt(0)) before each use, which serialises the fragment reads the slice pipeline issues ahead;
+// with the buffer form the waits count only the reads a group needs.  Requires BM * K * 2 bytes
+// below 2^31 (byte offsets inside the tile's rows are 32-bit).
 // Rows past m or n are fetched from the last valid row and their results are not stored.
 template <class CF, typename TO>
 __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int K, double alpha,
@@ -1008,23 +1051,36 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
 #pragma unroll
         for (int b = 0; b < CF::FBN; ++b) acc[a][b] = f4v{0, 0, 0, 0};
 
+    // Buffer descriptors over the tile's A rows and B rows; each lane's byte offset of its chunk
+    // in slice 0 is fixed for the whole K loop (slice s adds 2*s*BK).
+    const __amdgpu_buffer_rsrc_t ra = bg_rsrc(At + (int64_t)i0 * lda, (int64_t)min(CF::BM, m - i0) * lda);
+    const __amdgpu_buffer_rsrc_t rb = bg_rsrc(Bt + (int64_t)j0 * ldb, (int64_t)min(CF::BN, n - j0) * ldb);
+    const int uwave = __builtin_amdgcn_readfirstlane(wave);
+    int voff[LPW];
+#pragma unroll
+    for (int q = 0; q < LPW; ++q) {
+        const int ins = uwave * LPW + q;       // A rows first, then B rows
+        const int rr = ins * 8 + (lane >> 3);  // tile row this lane fills
+        const int p = lane & 7;                // physical chunk it fills
+        if (ins * 8 < CF::BM)
+            voff[q] = 2 * (int)(min(rr, m - 1 - i0) * lda + (p ^ bg_swz(rr)) * 8);
+        else
+            voff[q] = 2 * (int)(min(rr - CF::BM, n - 1 - j0) * ldb + (p ^ bg_swz(rr - CF::BM)) * 8);
+    }
     auto issue = [&](int s) {
         __bf16 *st = smem + (s & 1) * STAGE;
-        const int k0 = s * BK;
+        const int kb = 2 * s * BK;
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
-            const int ins = wave * LPW + q;        // A rows first, then B rows
-            const int rr = ins * 8 + (lane >> 3);  // tile row this lane fills
-            const int p = lane & 7;                // physical chunk it fills
-            const __bf16 *src;
-            if (rr < CF::BM) {
-                const int gi = min(i0 + rr, m - 1);
-                src = At + (int64_t)gi * lda + k0 + (p ^ bg_swz(rr)) * 8;
-            } else {
-                const int r = rr - CF::BM, gj = min(j0 + r, n - 1);
-                src = Bt + (int64_t)gj * ldb + k0 + (p ^ bg_swz(r)) * 8;
-            }
-            __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(st + ins * 8 * BK), 16, 0, 0);
+            const int ins = uwave * LPW + q;
+            const int vo = voff[q] + kb;
+            // An if over two calls, and the offset in a local: with a conditional expression over the
+            // two descriptors, or an array element in the argument, the host compilation pass drops
+            // the kernel's launch stub without a diagnostic (build.sh checks for that).
+            if (ins * 8 < CF::BM)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(st + ins * 8 * BK), 16, vo, 0, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void *)(st + ins * 8 * BK), 16, vo, 0, 0, 0);
         }
     };
 
@@ -1109,7 +1165,7 @@ static int bgemm_panels(omx_ctx ctx, int m, int n, int k, int nterm, const BgTer
     OMX_REQUIRE(nsplit >= 1 && nsplit <= 3, "bf16 panels: split %d", nsplit);
     const int reps = nblk[nsplit - 1];
     const int64_t K = (int64_t)k * nterm * reps;
-    OMX_REQUIRE(K < (1LL << 31), "bf16 panels: K too large");
+    if ((int64_t)Cfg4::BM * K * (int64_t)sizeof(__bf16) >= (1LL << 31)) return OMX_ERR_UNSUPPORTED;  // 32-bit offsets
     const size_t bytes_a = sizeof(__bf16) * (size_t)K * m, bytes_b = sizeof(__bf16) * (size_t)K * n;
     int rc = ensure_ws(ctx, bytes_a + bytes_b + 256);
     if (rc) return rc;
