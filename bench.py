@@ -226,6 +226,9 @@ def main():
                     help="product precision in --kernel mode (all but f64 on FP32 storage)")
     ap.add_argument("--spmm-stencil", action="store_true",
                     help="run lattice operators through the 2.5-D stencil SpMM instead of the general BSR kernel")
+    ap.add_argument("--blaslt", action="store_true",
+                    help="time the vendor BF16 GEMM (torch.mm, hipBLASLt) at the BF16 Gram and pair shapes, "
+                         "as a reference point for k_bgemm_tn")
     ap.add_argument("--mixed-steps", type=int, default=0,
                     help="also time this many low-precision steps of the mixed (hmx) solve, FP32 storage")
     args = ap.parse_args()
@@ -250,6 +253,9 @@ def main():
         return 0
     if args.la:
         print(json.dumps(la_bench(oh, args.L)), flush=True)
+        return 0
+    if args.blaslt:
+        print(json.dumps(blaslt_bench(args.L, args.launches)), flush=True)
         return 0
     if args.kernel:
         print(json.dumps(kernel_mode(oh, args.kernel, args.L, args.launches, args.kernel_prec,
@@ -517,6 +523,36 @@ def time_to_converge(F, oh, ctxh, mats, nb, no, eta, C0, tol=1e-10, progress=50,
 
 
 GROUP_ETA = 0.0517
+
+
+def blaslt_bench(L, launches):
+    """torch.mm in BF16 with an FP32 result (hipBLASLt on this stack) at the shapes k_bgemm_tn runs
+    for the workload: the Gram (m x m, K = n) and the BF16x3 gradient pair (m x n, K = 6 m).  A
+    reference point for the library's own BF16 kernel, not a path of the library."""
+    import torch
+    n = 4 * L ** 3
+    m = n // 2
+    dev = "cuda:0"
+    g = torch.Generator(device=dev).manual_seed(0)
+    out = {"mode": "blaslt", "L": L, "launches": launches}
+    for name, (M, N, K) in (("gram", (m, m, n)), ("pair", (m, n, 6 * m))):
+        a = (torch.randn(M, K, device=dev, generator=g) * 1e-2).to(torch.bfloat16)
+        b = (torch.randn(N, K, device=dev, generator=g) * 1e-2).to(torch.bfloat16)
+        torch.mm(a, b.t(), out_dtype=torch.float32)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(launches):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            torch.mm(a, b.t(), out_dtype=torch.float32)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        out[name] = {"shape": [M, N, K], "ms": [round(t, 4) for t in ts],
+                     "avg_PFs": round(2.0 * M * N * K / (sum(ts) / len(ts) * 1e-3) / 1e15, 4)}
+        del a, b
+        torch.cuda.empty_cache()
+    return out
 
 
 def kernel_mode(oh, kernel, L, launches, prec="f64", copies=3, stencil=False):

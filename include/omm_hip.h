@@ -206,7 +206,7 @@ int omx_spmm_set_tile(omx_ctx ctx, int batch, int wpg, int jper, int mode);
  * Lx x Ly x Lz lattice of 4x4 blocks (recognised when the operator is created; Lx, Ly multiples of
  * 4) run through a 2.5-D kernel that keeps z-planes of X in registers and LDS instead of gathering
  * each basis column 7 times: 1.2x the algorithmic HBM bytes instead of 2.1x, but slower than the
- * general kernel at n = 16384 (1.05 ms against 0.74 ms), so off by default.  Tuning hook; the two
+ * general kernel at n = 16384 (0.89 ms against 0.83 ms), so off by default.  Tuning hook; the two
  * kernels give the same bits. */
 int omx_spmm_stencil(omx_ctx ctx, int enable);
 /* Lattice edges (Lx, Ly, Lz) recognised in the operator's pattern, all 0 when it is not such a

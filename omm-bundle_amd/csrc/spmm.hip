@@ -197,9 +197,10 @@ __global__ __launch_bounds__(64 * WPG) void k_spmm_bsr4(int no, int nb, int nbc,
 // halo) instead of 7 times.  The 7 blocks of a column are taken in stored order with the FMA
 // sequence of k_spmm_bsr4, so the results are the same bits.
 // Measured at n = 16384 (FP64): 1.2x the algorithmic HBM bytes (2.6 GB per launch against 4.4 GB
-// for k_spmm_bsr4), but 1.05 ms against 0.74 ms: the block values reach each lane through
-// v_readlane (2.6x the VALU instructions of k_spmm_bsr4) and every plane ends in a workgroup
-// barrier.  It is therefore opt-in (omx_spmm_stencil).
+// for k_spmm_bsr4), but 0.89 ms against 0.83 ms in the same run (FP32: 0.57 against 0.47 ms): one
+// workgroup per CU (128 KB of LDS) and a barrier per plane.  The block values come through the
+// scalar cache (broadcast with v_readlane they cost 2.6x the VALU instructions of k_spmm_bsr4 and
+// the kernel 1.08 ms).  It is therefore opt-in (omx_spmm_stencil).
 constexpr int ST_TX = 4, ST_TY = 4, ST_WAVES = ST_TX * ST_TY, ST_SLOTS = ST_WAVES + 2 * (ST_TX + ST_TY);
 constexpr int ST_VBYTES = 8;  // bytes per lane and basis column: 64 FP64 or 128 FP32 states per wave
 
@@ -212,14 +213,6 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_s_waitcnt(0xc07f);  // gfx9 encoding: vmcnt 63, expcnt 7, lgkmcnt 0
     __builtin_amdgcn_s_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
-// The double in lane `lane` of v, as a wave-uniform (scalar) value.
-__device__ inline double readlane_d(double v, int lane) {
-    const long long u = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sorts the 7 (key, tag) pairs by key: a 16-comparator network, checked on all 0/1 inputs.
@@ -317,7 +310,7 @@ __global__ __launch_bounds__(64 * ST_WAVES, MINB) void k_spmm_stencil(int no, in
 #pragma unroll
         for (int r = 0; r < 4; ++r) vstore<T, VEC>(&lds[buf][slot][r][lane * VEC], x[r]);
     };
-    auto fma_block = [&](const T (&x)[4][VEC], const double (&v)[16], T (&acc)[4][VEC]) {
+    auto fma_block = [&](const T (&x)[4][VEC], const double *__restrict__ v, T (&acc)[4][VEC]) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -331,31 +324,19 @@ __global__ __launch_bounds__(64 * ST_WAVES, MINB) void k_spmm_stencil(int no, in
     // z+1 and buffer z&1 plane z of the tile and halo.  The step loads the own plane z+2 into xn and
     // the halo of plane z+2 into hn (a step ahead of their use), computes plane z, then stages plane
     // z+1 (xp, hc) into the other buffer.
-    // The 7 blocks of block column J are stored at 7J..7J+6 (every column of a recognised lattice
-    // holds exactly 7), so their 112 values are contiguous at val + 112 J: lane q holds values q and
-    // 64 + q of them (vals[0], vals[1]), and a block value is read out as a scalar with readlane.
-    auto load_vals = [&](int J, double (&v)[2]) {
-        const double *src = val + 112 * (int64_t)J;
-        v[0] = src[lane];
-        v[1] = (lane < 48) ? src[64 + lane] : 0.0;
-    };
     // Register rings, slots fixed at compile time (plane p of the own site in own[p % 5], halo plane
-    // p in halo[p % 3], the values of plane p in vals[p % 3]): the step of plane z computes from
-    // planes z-1, z, z+1 and the values of plane z, stages plane z+1 and the halo of plane z+1 into
-    // the other LDS buffer, and loads the own plane z+3, the halo of plane z+3 and the values of
-    // plane z+2, two steps before their use.
+    // p in halo[p % 3]): the step of plane z computes from planes z-1, z, z+1, stages plane z+1 and
+    // the halo of plane z+1 into the other LDS buffer, and loads the own plane z+3 and the halo of
+    // plane z+3, two steps before their use.
     T own[5][4][VEC], halo[3][4][VEC];
-    double vals[3][2];
     auto step = [&](int z, auto K) {
         constexpr int k = decltype(K)::value;  // z % 15
         T (&xm)[4][VEC] = own[(k + 4) % 5];
         T (&xc)[4][VEC] = own[k % 5];
         T (&xp)[4][VEC] = own[(k + 1) % 5];
         T (&hc)[4][VEC] = halo[(k + 1) % 3];
-        const double (&vc)[2] = vals[k % 3];
         const bool more = (z + 1 < Lz);
         if (z + 2 < Lz) load(own_xy + plane * ((z + 3) % Lz), own[(k + 3) % 5]);
-        if (z + 2 < Lz) load_vals(own_xy + plane * (z + 2), vals[(k + 2) % 3]);
         if (z + 3 < Lz) load(halo_xy + plane * (z + 3), halo[k % 3]);
         const int buf = z & 1;
         const int zo = plane * z;
@@ -375,15 +356,14 @@ __global__ __launch_bounds__(64 * ST_WAVES, MINB) void k_spmm_stencil(int no, in
         int dpack = 0;
 #pragma unroll
         for (int b = 0; b < 7; ++b) dpack |= dir[b] << (3 * b);
-        // a loop, not unrolled: one block's 16 values (32 scalar registers) live at a time
+        // The 7 blocks of block column J are stored at 7J..7J+6 (every column of a recognised
+        // lattice holds exactly 7), so their 112 values are contiguous at val + 112 J; J is
+        // wave-uniform, so a block's 16 values come through the scalar cache.  A loop, not
+        // unrolled: one block's values (32 scalar registers) live at a time.
+        const double *vals = val + 112 * (int64_t)J;
 #pragma unroll 1
         for (int b = 0; b < 7; ++b) {
-            double a[16];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int idx = 16 * b + e;
-                a[e] = readlane_d((idx >> 6) ? vc[1] : vc[0], idx & 63);
-            }
+            const double *a = vals + 16 * b;
             const int d = (dpack >> (3 * b)) & 7;
             if (d == 0) {
                 fma_block(xc, a, acc);
@@ -431,8 +411,6 @@ __global__ __launch_bounds__(64 * ST_WAVES, MINB) void k_spmm_stencil(int no, in
         lds_barrier();
     };
 
-    load_vals(own_xy, vals[0]);
-    load_vals(own_xy + plane, vals[1]);
     load(own_xy + plane * (Lz - 1), own[4]);
     load(own_xy, own[0]);
     load(own_xy + plane, own[1]);

@@ -595,11 +595,14 @@ def test_sddmm_f32(ctx):
 
 @pytest.mark.parametrize("dims,no,dt", [((16, 16, 16), 8192, oh.OMX_F64), ((8, 8, 8), 1024, oh.OMX_F64),
                                         ((4, 8, 3), 200, oh.OMX_F64), ((12, 4, 5), 37, oh.OMX_F64),
-                                        ((16, 16, 16), 8192, oh.OMX_F32), ((8, 4, 6), 300, oh.OMX_F32)])
+                                        ((5, 3, 7), 130, oh.OMX_F64), ((3, 9, 4), 64, oh.OMX_F64),
+                                        ((16, 16, 16), 8192, oh.OMX_F32), ((8, 4, 6), 300, oh.OMX_F32),
+                                        ((7, 5, 3), 258, oh.OMX_F32)])
 def test_spmm_stencil_equals_general_kernel(ctx, dims, no, dt):
     """The 2.5-D kernel of lattice operators (z-planes in registers and LDS) against the general
-    BSR kernel on the same operator and X: the same bits, including partial row chunks, Lz = 3 and
-    beta != 0; and the lattice edges recognised in the pattern."""
+    BSR kernel on the same operator and X: the same bits, including partial row chunks, Lz = 3,
+    beta != 0 and edges that are not multiples of 4 (where the stencil kernel falls back to the
+    general one); and the lattice edges recognised in the pattern."""
     _note("spmm stencil", dims, no, dt)
     cp, ri, bv = oh.lattice_bsr(dims, 0)
     nb = 4 * int(np.prod(dims))
@@ -612,22 +615,22 @@ def test_spmm_stencil_equals_general_kernel(ctx, dims, no, dt):
     dX = oh.DMat.from_host(ctx, X, dt)
     out = {}
     try:
-        for stencil in (True, False):
-            ctx.spmm_stencil(stencil)
+        for kern in (1, 0):
+            ctx.spmm_stencil(kern)
             for beta in (-0.5, 0.0):
                 dY = oh.DMat.from_host(ctx, Y0, dt)
                 oh.spmm(ctx, dX, A, dY, alpha=1.5, beta=beta)
-                out[stencil, beta] = dY.download()
+                out[kern, beta] = dY.download()
                 dY.free()
     finally:
-        ctx.spmm_stencil(False)
+        ctx.spmm_stencil(0)
         dX.free()
         A.free()
     for beta in (-0.5, 0.0):
-        assert np.array_equal(out[True, beta], out[False, beta])
+        assert np.array_equal(out[1, beta], out[0, beta])
     H = oh.bsr_to_dense(nb, cp, ri, bv)
     tol = 1e-12 if dt == oh.OMX_F64 else 1e-4
-    assert np.max(np.abs(out[True, -0.5] - (1.5 * X @ H - 0.5 * Y0))) < tol * 30
+    assert np.max(np.abs(out[1, -0.5] - (1.5 * X @ H - 0.5 * Y0))) < tol * 30
 
 
 def test_non_lattice_pattern_is_not_taken_for_a_lattice(ctx):
