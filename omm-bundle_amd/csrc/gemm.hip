@@ -635,6 +635,148 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
     return OMX_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP32 Gram C := alpha*A*B^T + beta0*C0 + beta*C with direct-to-LDS staging (configuration 7 on
+// FP32 storage): A (m x k) and B (n x k) with the row index contiguous, 128x128 tiles, 8 waves of
+// 32x64, v_mfma_f32_16x16x4f32, BK = 32.  A k-row of a tile is 512 B, so one
+// buffer_load_dwordx4 ... lds per wave moves two consecutive k-rows (lanes 0-31 the first, 32-63
+// the second), through descriptors rebased at every slice.  An MFMA step's fragment read takes 16
+// consecutive i from each of four consecutive k-rows, which lie 512 B apart, on the same banks:
+// 16-byte chunk c of k-row r is therefore stored at chunk c ^ (4 (r mod 4)), which places the four
+// 64-byte runs in four different quarters of 256 bytes.  The fragments of k step ks + 4 are read
+// before the MFMAs of step ks.  The MFMA chain per element (k in order) and the epilogue are the
+// staged kernel's, so the results are bitwise the same.  Requires m, n % 128 == 0, k % 32 == 0,
+// leading dimensions that are multiples of 4 and 16-byte aligned operands.
+// ---------------------------------------------------------------------------------------------
+__device__ inline __amdgpu_buffer_rsrc_t sl_rsrc(const float *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ inline int sl_swz(int r) { return 4 * (r & 3); }
+
+template <class CF>
+__global__ __launch_bounds__(CF::NT, CF::MINW) void k_sgemm_dl(int m, int n, int k, double alpha,
+                                                               const float *__restrict__ A, int64_t lda,
+                                                               const float *__restrict__ B, int64_t ldb, double beta0,
+                                                               const float *__restrict__ C0, double beta,
+                                                               float *__restrict__ C, int64_t ldc,
+                                                               const int *__restrict__ mask) {
+    static_assert(CF::BM == 128 && CF::BN == 128, "two 512-byte rows per wave load");
+    if (mask && !*mask) return;  // system masked off (whole workgroup, before any barrier)
+    constexpr int BK = Prec<OMX_PREC_F32>::BK, NW = CF::WGM * CF::WGN, INS = BK, LPW = INS / NW;
+    constexpr int STAGE = 2 * BK * 128;  // floats: A rows (BK x 128), then B rows
+    static_assert(LPW * NW == INS, "loads per wave");
+    __shared__ float smem[2 * STAGE];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave / CF::WGN, wn = wave % CF::WGN;
+    int tm, tn;
+    tile_of((int)blockIdx.x, m / 128, n / 128, tm, tn);
+    const int i0 = tm * 128, j0 = tn * 128;
+    f4v acc[CF::FBM][CF::FBN];
+#pragma unroll
+    for (int a = 0; a < CF::FBM; ++a)
+#pragma unroll
+        for (int b = 0; b < CF::FBN; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+    // wave load `ins` < BK/2 fills A rows 2 ins, 2 ins + 1; the others B rows; lane l the physical
+    // chunk l % 32 of row 2 ins + l / 32 (of its operand)
+    const int uwave = __builtin_amdgcn_readfirstlane(wave);
+    int voff[LPW];
+#pragma unroll
+    for (int q = 0; q < LPW; ++q) {
+        const int ins = uwave * LPW + q;
+        const int r = 2 * (ins % (BK / 2)) + (lane >> 5);
+        const int c = (lane & 31) ^ sl_swz(r);  // logical chunk: i = 4c .. 4c+3
+        voff[q] = 4 * (4 * c + r * (int)(ins < BK / 2 ? lda : ldb));
+    }
+    auto issue = [&](int s) {
+        float *st = smem + (s & 1) * STAGE;
+        const int kl = s * BK;
+        const __amdgpu_buffer_rsrc_t ra = sl_rsrc(A + i0 + (int64_t)kl * lda);
+        const __amdgpu_buffer_rsrc_t rb = sl_rsrc(B + j0 + (int64_t)kl * ldb);
+#pragma unroll
+        for (int q = 0; q < LPW; ++q) {
+            const int ins = uwave * LPW + q;
+            const int vo = voff[q];  // (a local: see k_bgemm_tn)
+            if (ins < BK / 2)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(st + ins * 256), 16, vo, 0, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void *)(st + ins * 256), 16, vo, 0, 0, 0);
+        }
+    };
+    const int li = lane & 15, lk = lane >> 4;
+    const int ns = k / BK;
+    issue(0);
+    for (int s = 0; s < ns; ++s) {
+        __builtin_amdgcn_s_waitcnt(waitcnt_vmcnt(0));  // this wave's rows of slice s have landed
+        __syncthreads();                               // everyone's have; slice s-1 is no longer read
+        if (s + 1 < ns) issue(s + 1);
+        const float *As = smem + (s & 1) * STAGE, *Bs = As + BK * 128;
+        float a[2][CF::FBM], b[2][CF::FBN];
+        auto frag = [&](int ks, float (&fa)[CF::FBM], float (&fb)[CF::FBN]) {
+            const int r = ks + lk;
+#pragma unroll
+            for (int f = 0; f < CF::FBM; ++f) {
+                const int i = wm * CF::WM + f * 16 + li;
+                fa[f] = As[r * 128 + (((i >> 2) ^ sl_swz(r)) << 2) + (i & 3)];
+            }
+#pragma unroll
+            for (int f = 0; f < CF::FBN; ++f) {
+                const int j = wn * CF::WN + f * 16 + li;
+                fb[f] = Bs[r * 128 + (((j >> 2) ^ sl_swz(r)) << 2) + (j & 3)];
+            }
+        };
+        frag(0, a[0], b[0]);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int cur = (ks / 4) & 1;
+            if (ks + 4 < BK) frag(ks + 4, a[cur ^ 1], b[cur ^ 1]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < CF::FBN; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[cur][fm], b[cur][fn], acc[fm][fn], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    const int lq = lane >> 4;
+#pragma unroll
+    for (int fm = 0; fm < CF::FBM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < CF::FBN; ++fn) {
+            const int j = j0 + wn * CF::WN + fn * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wm * CF::WM + fm * 16 + Prec<OMX_PREC_F32>::out_row(lq, v);
+                const float val = acc[fm][fn][v];
+                double r = alpha * (double)val;
+                if (C0) r += beta0 * (double)C0[(int64_t)i + (int64_t)j * ldc];
+                if (beta != 0.0) r += beta * (double)C[(int64_t)i + (int64_t)j * ldc];
+                C[(int64_t)i + (int64_t)j * ldc] = (float)r;
+            }
+        }
+}
+
+// Launches the FP32 direct-to-LDS Gram when the product qualifies (full products whose tiles fill
+// the chip; symmetric and small products take the staged kernel); OMX_ERR_UNSUPPORTED otherwise.
+static int run_sgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const float *A, int64_t lda, const float *B,
+                        int64_t ldb, double beta0, const float *C0, double beta, float *C, int64_t ldc) {
+    typedef Cfg5 CF;
+    const int tiles = (m / 128) * (n / 128);
+    if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % Prec<OMX_PREC_F32>::BK || lda % 4 || ldb % 4 ||
+        !aligned16(A) || !aligned16(B) || tiles < 2 * ctx->num_cus || ctx->gemm_sym || lda > (1 << 20) ||
+        ldb > (1 << 20))  // 32-bit lane offsets from the slice descriptors
+        return OMX_ERR_UNSUPPORTED;
+    hipEvent_t ev;
+    prof_begin(ctx, OMX_K_GEMM, &ev);
+    prof_work(ctx, OMX_K_GEMM, 2.0 * (double)m * n * k,
+              sizeof(float) * ((double)m * k + (double)k * n + (double)m * n * (1 + (C0 ? 1 : 0) + (beta != 0.0 ? 1 : 0))));
+    hipLaunchKernelGGL((k_sgemm_dl<CF>), dim3(tiles), dim3(CF::NT), 0, ctx->stream, m, n, k, alpha, A, lda, B, ldb,
+                       beta0, C0, beta, C, ldc, ctx->launch_mask);
+    prof_end(ctx, OMX_K_GEMM, ev);
+    OMX_CHECK_HIP(hipGetLastError());
+    return OMX_OK;
+}
+
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR, class CF>
 static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1, const TS *A2, int64_t lda,
                    const TS *B1, const TS *B2, int64_t ldb, double beta0, const TS *C0, double beta, TS *C,
@@ -690,15 +832,17 @@ static int run_cfg(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1,
 
 // Tile configurations that apply to a product: FP64 products with A not transposed ('n','t' Grams,
 // 'n','n' products and the calc_G pair) carry 0-3 and 5-7 (7 on FP64 storage); the FP32 and BF16
-// ones on FP32 storage 0 and 4; every other product uses 0.
+// ones on FP32 storage 0 and 4, and the FP32 'n','t' Gram also 7; every other product uses 0.
 template <typename TS, int PREC, bool TRA, bool TRB, int NPAIR>
 struct CfgSet {
     static constexpr bool f64 = (PREC == OMX_PREC_F64) && !TRA;
     static constexpr bool dl = f64 && std::is_same<TS, double>::value;
     static constexpr bool lowp = (PREC != OMX_PREC_F64) && std::is_same<TS, float>::value && !TRA && (TRB || NPAIR == 2);
+    static constexpr bool sdl = lowp && PREC == OMX_PREC_F32 && TRB && NPAIR == 1;
     static std::vector<int> ids() {
         if (dl) return {7, 0, 1, 2, 3, 5, 6};
         if (f64) return {0, 1, 2, 3, 5, 6};
+        if (sdl) return {7, 0, 4};
         if (lowp) return {0, 4};
         return {0};
     }
@@ -735,6 +879,13 @@ static int run_cfg_id(int id, omx_ctx ctx, int m, int n, int k, double alpha, co
                                                                 C0, beta, C, ldc);
             default:
                 break;
+        }
+    }
+    if constexpr (CS::sdl) {
+        if (id == 7) {
+            int rc = run_sgemm_dl(ctx, m, n, k, alpha, A1, lda, B1, ldb, beta0, C0, beta, C, ldc);
+            if (rc != OMX_ERR_UNSUPPORTED) return rc;
+            id = 0;  // the staged kernel covers every other shape
         }
     }
     if constexpr (CS::lowp) {
@@ -810,9 +961,10 @@ static int run_gemm(omx_ctx ctx, int m, int n, int k, double alpha, const TS *A1
                     int64_t ldc) {
     // default: 128x128 with 8 waves and two workgroups per CU for the FP64 Gram and gradient products
     // (3-4 % ahead of the 4-wave tile at n=16384, bitwise the same result), with direct-to-LDS
-    // staging where it applies (bitwise the same); 0 otherwise
+    // staging where it applies (bitwise the same); the FP32 Gram's direct-to-LDS kernel where it
+    // applies; 0 otherwise
     typedef CfgSet<TS, PREC, TRA, TRB, NPAIR> CS;
-    int id = CS::dl ? 7 : CS::f64 ? 5 : 0;
+    int id = (CS::dl || CS::sdl) ? 7 : CS::f64 ? 5 : 0;
     if (ctx->gemm_cfg >= 0) {
         id = ctx->gemm_cfg;  // forced (omx_gemm_set_tile); a configuration that does not apply falls back to 0
     } else if (ctx->autotune) {

@@ -368,6 +368,35 @@ def test_gemm_f64_gram_direct_lds(shape):
     assert np.max(np.abs(out[0] - ref)) <= 1e-12 * np.sqrt(k) * 4
 
 
+@pytest.mark.parametrize("shape", [(4096, 2048, 288), (4096 + 64, 2048, 288), (4096, 2048, 280), (256, 256, 64)])
+def test_gemm_f32_gram_direct_lds(shape):
+    """FP32 Gram C = -2 A B^T + 0.5 C0 (op 'n','t', FP32 products on FP32 storage): the default
+    configuration 7 loads operand rows straight into LDS (two k-rows per wave load, chunks swizzled
+    by k-row) where m, n % 128 == 0, k % 32 == 0 and the tiles fill the chip (the first shape); the
+    others fall back to the staged kernel (configuration 0).  Results are bitwise those of
+    configuration 0 and within the FP32 bar of numpy."""
+    m, n, k = shape
+    _note("gemm_f32 gram direct-to-LDS", shape)
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    ref = -2.0 * (A.astype(np.float64) @ B.astype(np.float64).T) + 0.5 * C0
+    with oh.Ctx(0) as c:
+        dA, dB = oh.DMat.from_host(c, A, oh.OMX_F32), oh.DMat.from_host(c, B, oh.OMX_F32)
+        out = []
+        for cfg in (-1, 0):
+            oh.check(oh.lib().omx_gemm_set_tile(c.h, cfg), "gemm_set_tile")
+            dC = oh.DMat.from_host(c, C0, oh.OMX_F32)
+            oh.gemm(c, "n", "t", dA, dB, dC, alpha=-2.0, beta=0.5, prec=oh.PREC_F32)
+            out.append(dC.download())
+            dC.free()
+        oh.check(oh.lib().omx_gemm_set_tile(c.h, -1), "gemm_set_tile")
+        dA.free()
+        dB.free()
+    assert np.array_equal(out[0], out[1])
+    assert np.max(np.abs(out[0] - ref)) <= 1e-5 * np.sqrt(k) * 4
+
+
 @pytest.mark.parametrize("dt,prec,no,nb", [(oh.OMX_F64, oh.PREC_F64, 4096, 1024),
                                            (oh.OMX_F64, oh.PREC_F64, 300, 512),
                                            (oh.OMX_F64, oh.PREC_F64, 4096 + 64, 1024),
