@@ -227,9 +227,9 @@ int omx_bsr_set_walk_groups(omx_bsr A, const int *order, int group);
  * default); 6 = 128x128 with 8 waves of 64x32 (FP64 products); 4 = 256x256 with 8 waves (FP32 and
  * BF16 products on FP32 storage); 7 = the tiles of 5 with the operands loaded straight into LDS
  * (FP64 Gram and gradient products with m, n % 128 == 0 and k % 16 == 0, the default there; other
- * shapes take 5).  A
- * configuration that does not apply to a product falls back to 0.  Used by the tile autotuner;
- * results do not depend on it. */
+ * shapes take 5; and FP32 'n','t' Grams on FP32 storage with m, n % 128 == 0 and k % 32 == 0 whose
+ * tiles fill the chip, the default there; other shapes take 0).  A configuration that does not
+ * apply to a product falls back to 0.  Used by the tile autotuner; results do not depend on it. */
 int omx_gemm_set_tile(omx_ctx ctx, int cfg);
 /* Autotuner (project-written; no counterpart in the reference): with enable != 0, the first
  * product or SpMM of each shape on this context times every applicable tile configuration / SpMM

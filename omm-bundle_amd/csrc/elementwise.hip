@@ -58,13 +58,30 @@ __global__ __launch_bounds__(EW_NT) void k_reduce_partials(int nblocks, const do
 
 // ---- m_add / m_scale / m_set -----------------------------------------------------------------
 
+// Row and column (i, j) of flat index q in an m-row column-major walk, advanced by a fixed grid
+// stride with one division per thread instead of a 64-bit division per element.
+struct ColWalk {
+    int64_t i, j, si, sj, m;
+    __device__ ColWalk(int64_t q, int64_t stride, int64_t rows)
+        : i(q % rows), j(q / rows), si(stride % rows), sj(stride / rows), m(rows) {}
+    __device__ void next() {
+        i += si;
+        j += sj;
+        if (i >= m) {
+            i -= m;
+            ++j;
+        }
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_add_n(int m, int n, double alpha, const T *__restrict__ A, int64_t lda,
                                                  double beta, T *__restrict__ C, int64_t ldc) {
     typedef El<T> E;
-    int64_t total = (int64_t)m * n;
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m;
+    const int64_t total = (int64_t)m * n, stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += stride, w.next()) {
+        const int64_t i = w.i, j = w.j;
         const T a = A[i + j * lda];
         T *p = C + i + j * ldc;
         double ar = alpha * E::re(a), ai = 0.0;
@@ -115,9 +132,10 @@ __global__ __launch_bounds__(EW_NT) void k_add_t(int m, int n, double alpha, con
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_scale(int m, int n, double beta, T *__restrict__ C, int64_t ldc) {
     typedef El<T> E;
-    int64_t total = (int64_t)m * n;
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m;
+    const int64_t total = (int64_t)m * n, stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += stride, w.next()) {
+        const int64_t i = w.i, j = w.j;
         const T c = C[i + j * ldc];
         C[i + j * ldc] = E::make(beta * E::re(c), beta * E::im(c));
     }
@@ -126,9 +144,10 @@ __global__ __launch_bounds__(EW_NT) void k_scale(int m, int n, double beta, T *_
 template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_set(int lu, int m, int n, double alpha, double beta, T *__restrict__ C,
                                                int64_t ldc) {
-    int64_t total = (int64_t)m * n;
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m;
+    const int64_t total = (int64_t)m * n, stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += stride, w.next()) {
+        const int64_t i = w.i, j = w.j;
         if (lu == 1 && j < i) continue;
         if (lu == 2 && j > i) continue;
         C[i + j * ldc] = El<T>::make(i != j ? alpha : beta, 0.0);
@@ -157,9 +176,10 @@ __global__ __launch_bounds__(EW_NT) void k_dot_partial(int m, int n, const T *__
                                                        double *__restrict__ partial) {
     typedef El<T> E;
     double v[2] = {0.0, 0.0};
-    int64_t total = (int64_t)m * n;
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m;
+    const int64_t total = (int64_t)m * n, stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += stride, w.next()) {
+        const int64_t i = w.i, j = w.j;
         const T a = A[i + j * lda], b = B[i + j * ldb];
         if constexpr (E::cplx) {
             v[0] += E::re(a) * E::re(b) + E::im(a) * E::im(b);
@@ -185,8 +205,10 @@ __global__ __launch_bounds__(EW_NT) void k_pr_partial(int64_t total_m, int m, co
     Gp += blockIdx.y * sG;
     partial += 2 * (int64_t)blockIdx.y * gridDim.x;
     double v[2] = {0.0, 0.0};
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total_m; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m;
+    const int64_t stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total_m; q += stride, w.next()) {
+        const int64_t i = w.i, j = w.j;
         const T gt = G[i + j * ld], pt = Gp[i + j * ld];
         const double g = E::re(gt), p = E::re(pt);
         const T wt = E::make(g - p, E::im(gt) - E::im(pt));
@@ -314,9 +336,10 @@ template <typename T>
 __global__ __launch_bounds__(EW_NT) void k_cg_direction(int m, int n, double lambda, const T *__restrict__ G,
                                                         T *__restrict__ D, T *__restrict__ Gp, int64_t ld) {
     typedef El<T> E;
-    int64_t total = (int64_t)m * n;
-    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * EW_NT) {
-        int64_t i = q % m, j = q / m, o = i + j * ld;
+    const int64_t total = (int64_t)m * n, stride = (int64_t)gridDim.x * EW_NT;
+    ColWalk w((int64_t)blockIdx.x * EW_NT + threadIdx.x, stride, m);
+    for (int64_t q = (int64_t)blockIdx.x * EW_NT + threadIdx.x; q < total; q += stride, w.next()) {
+        const int64_t o = w.i + w.j * ld;
         const T gt = G[o];
         const double g = E::re(gt), gi = E::im(gt);
         double d, di = 0.0;
