@@ -342,8 +342,9 @@ int omx_zbatch_solve(omx_zbatch b, double eta, double cg_tol, int max_steps, int
  * libOMM/src/omm.F90:34-35, 96-106), for real flavour-0 systems of different sizes: system s has
  * no[s] states and the operators H[s] (already shifted by -eta_s S as omm() expects) and S[s]
  * (created with omx_bsr_create on ctx; the group keeps references, not copies).  Every CG
- * decision is taken on the device; one step of all systems is captured as one graph with a
- * parallel branch per system.  With omx_autotune on, each system's shapes are tuned in the eager
+ * decision is taken on the device; one step of all systems is captured as one graph: the systems
+ * whose products fill the GPU on their own in sequence, each other system on a parallel branch
+ * beside them.  With omx_autotune on, each system's shapes are tuned in the eager
  * initial pass of the first solve, before the capture. */
 typedef struct omx_group_s *omx_group;
 int omx_group_create(omx_ctx ctx, int nsys, const int *no, const omx_bsr *H, const omx_bsr *S, omx_group *out);

@@ -21,9 +21,11 @@
 //
 // Uneven-size group (omx_group_*): real systems of different sizes, each with its own H and S,
 // advanced by the same device-side step.  A system's step is its own sequence of library launches
-// (SpMM with its shape, GEMMs with its tuned tiles, fused reductions) masked by its flags; the
-// systems' sequences go to separate streams forked from the library stream, so the captured graph
-// holds one parallel branch per system and a single graph launch advances all of them.
+// (SpMM with its shape, GEMMs with its tuned tiles, fused reductions) masked by its flags.  The
+// systems whose products fill the chip on their own run one after another on the library stream;
+// the others go to separate streams forked from it before that chain and joined after it, so the
+// captured graph holds the chain plus one parallel branch per small system, and a single graph
+// launch advances all of them.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -612,6 +614,7 @@ struct omx_group_s {
     int *flags = nullptr;  // F_COUNT arrays of nsys
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> joins;
+    std::vector<char> chain;  // system runs in the serial chain on the library stream (fills the chip)
     hipEvent_t fork = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -704,7 +707,10 @@ static int group_sys_step(omx_group g, int s) {
     return OMX_OK;
 }
 
-// Runs f(s) for every system on its own stream, forked from and joined back to the library stream.
+// Runs f(s) for every system: the systems whose products fill the chip on their own one after
+// another on the library stream (concurrent, they would only contend for the same CUs and L2), the
+// others each on its own stream, forked from the library stream before the chain and joined back
+// after it, so they overlap the chain.
 template <typename F>
 static int group_fanout(omx_group g, F &&f) {
     omx_ctx ctx = g->ctx;
@@ -714,6 +720,7 @@ static int group_fanout(omx_group g, F &&f) {
     int rc = OMX_OK;
     if (hipEventRecord(g->fork, main) != hipSuccess) rc = OMX_ERR_HIP;
     for (int s = 0; s < g->nsys && !rc; ++s) {
+        if (g->chain[s]) continue;
         if (hipStreamWaitEvent(g->streams[s], g->fork, 0) != hipSuccess) {
             rc = OMX_ERR_HIP;
             break;
@@ -722,9 +729,12 @@ static int group_fanout(omx_group g, F &&f) {
         rc = f(s);
         ctx->stream = main;
         if (!rc && hipEventRecord(g->joins[s], g->streams[s]) != hipSuccess) rc = OMX_ERR_HIP;
-        if (!rc && hipStreamWaitEvent(main, g->joins[s], 0) != hipSuccess) rc = OMX_ERR_HIP;
     }
+    for (int s = 0; s < g->nsys && !rc; ++s)
+        if (g->chain[s]) rc = f(s);
     ctx->stream = main;
+    for (int s = 0; s < g->nsys && !rc; ++s)
+        if (!g->chain[s] && hipStreamWaitEvent(main, g->joins[s], 0) != hipSuccess) rc = OMX_ERR_HIP;
     ctx->launch_mask = nullptr;
     ctx->prof_on = prof;
     if (rc == OMX_ERR_HIP) set_error("omx_group: stream fork/join failed");
@@ -776,6 +786,11 @@ int omx_group_create(omx_ctx ctx, int nsys, const int *no, const omx_bsr *H, con
     g->partial2.assign(nsys, nullptr);
     g->streams.assign(nsys, nullptr);
     g->joins.assign(nsys, nullptr);
+    g->chain.assign(nsys, 0);
+    for (int s = 0; s < nsys; ++s) {
+        const int64_t t = (no[s] + 127) / 128;
+        g->chain[s] = (t * t >= 2 * ctx->num_cus);  // its Gram alone fills every workgroup slot
+    }
     bool ok = true;
     for (int s = 0; s < nsys && ok; ++s) {
         const int n = H[s]->nb, o = no[s];
