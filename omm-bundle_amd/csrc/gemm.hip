@@ -87,7 +87,7 @@ typedef Cfg<128, 256, 2, 2, 1> Cfg3;  // 4 waves, 64x128 per wave
 typedef Cfg<256, 256, 2, 4, 1> Cfg4;  // 8 waves, 128x64 per wave (BF16 / FP32 products)
 typedef Cfg<128, 128, 4, 2, 2> Cfg5;  // 8 waves, 32x64 per wave, two workgroups per CU (FP64)
 typedef Cfg<128, 128, 2, 4, 2> Cfg6;  // 8 waves, 64x32 per wave, two workgroups per CU (FP64)
-constexpr int kNumCfg = 8;  // 7: Cfg5 tiles with direct-to-LDS staging (FP64 A*B^T products)
+constexpr int kNumCfg = 8;  // 7: Cfg5 tiles with direct-to-LDS staging (FP64 products, FP32 Grams)
 
 // LDS layouts.  FP64/FP32: k-major ([k][i], i contiguous): one element per lane per MFMA step,
 // lanes 0..15 read 16 consecutive i.  BF16: i-major ([i][k], k contiguous): each lane feeds 8
