@@ -679,3 +679,51 @@ def test_non_lattice_pattern_is_not_taken_for_a_lattice(ctx):
     finally:
         A.free()
     assert tuple(lat) == (0, 0, 0)
+
+
+@pytest.mark.parametrize("dt", [oh.OMX_F64, oh.OMX_F32])
+def test_elementwise_past_one_grid_stride(ctx, dt):
+    """The element-wise kernels walk (row, column) by their grid stride (up to 65536 x 256 threads,
+    2048 x 256 for the Polak-Ribiere dots).  A 3000 x 6000 matrix with ld 3010 has 18M elements,
+    more than one stride, and 3000 divides neither stride, so each thread's walk wraps columns at
+    varying rows: add, scale, set (lower), dot, Polak-Ribiere dots and the CG direction against
+    numpy."""
+    m, n, ld = 3000, 6000, 3010
+    _note("elementwise wrap", dt, m, n, ld)
+    npdt = np.float64 if dt == oh.OMX_F64 else np.float32
+    tol = 1e-13 if dt == oh.OMX_F64 else 1e-5
+    A = rng.standard_normal((m, n)).astype(npdt)
+    B = rng.standard_normal((m, n)).astype(npdt)
+    Cm = rng.standard_normal((m, n)).astype(npdt)
+    dA, dB = oh.DMat.from_host(ctx, A, dt, ld=ld), oh.DMat.from_host(ctx, B, dt, ld=ld)
+    dC = oh.DMat.from_host(ctx, Cm, dt, ld=ld)
+    try:
+        oh.add(ctx, "n", dA, dC, 0.75, -1.25)
+        ref = (-1.25 * Cm.astype(np.float64) + 0.75 * A.astype(np.float64))
+        assert np.max(np.abs(dC.download() - ref)) <= tol * 4
+        oh.scale(ctx, dC, -2.0)
+        ref = -2.0 * ref.astype(npdt).astype(np.float64)
+        assert np.max(np.abs(dC.download() - ref)) <= tol * 8
+        before = dC.download()
+        oh.mset(ctx, "l", dC, 0.5, 3.0)
+        got = dC.download()
+        il = np.tril_indices(m, 0, n)
+        lower = np.zeros((m, n), bool)
+        lower[il] = True
+        assert np.all(got[lower & (np.arange(m)[:, None] != np.arange(n)[None, :])] == 0.5)
+        assert np.all(np.diag(got) == 3.0)
+        assert np.array_equal(got[~lower], before[~lower])
+        dot = oh.mm_trace(ctx, dA, dB)
+        assert abs(dot - np.sum(A.astype(np.float64) * B)) <= tol * 1e3 * np.sqrt(m * n)
+        pr = oh.omm_pr_dots(ctx, dA, dB)
+        W = (A - B).astype(npdt).astype(np.float64)
+        assert abs(pr[0] - np.sum(A.astype(np.float64) * W)) <= tol * 1e3 * np.sqrt(m * n)
+        assert abs(pr[1] - np.sum(B.astype(np.float64) ** 2)) <= tol * 1e3 * np.sqrt(m * n)
+        dGp = oh.DMat(ctx, m, n, dt, ld=ld)
+        oh.omm_cg_direction(ctx, 0.375, dA, dB, dGp)
+        assert np.max(np.abs(dB.download() - (A.astype(np.float64) + 0.375 * B))) <= tol * 4
+        assert np.array_equal(dGp.download(), A)
+        dGp.free()
+    finally:
+        for M in (dA, dB, dC):
+            M.free()
