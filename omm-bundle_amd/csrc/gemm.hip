@@ -427,9 +427,12 @@ typedef __attribute__((address_space(3))) void lds_void;
 // s_waitcnt immediate (gfx9 encoding): vmcnt(vm), expcnt and lgkmcnt left at their maxima
 constexpr int waitcnt_vmcnt(int vm) { return (vm & 0xF) | ((vm >> 4) << 14) | (7 << 4) | (0xF << 8); }
 __device__ inline int dl_swz(int j) { return (j >> 1) & 7; }
-// Raw buffer descriptor at p (gfx9 descriptor word 3); the lanes' offsets stay below 2^31.
-__device__ inline __amdgpu_buffer_rsrc_t dl_rsrc(const double *p) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), (short)0, 0x7fffffff, 0x00020000);
+// Raw (unstrided) buffer descriptor over `bytes` bytes from p, for the buffer loads into LDS of
+// the direct-to-LDS products; word 3 is gfx9's for 32-bit data without swizzling.  Reads at or
+// past `bytes` return zero instead of touching memory.
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t raw_rsrc(const T *p, int bytes = 0x7fffffff) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0, bytes, 0x00020000);
 }
 
 template <class CF, bool TRB, int NPAIR>
@@ -491,8 +494,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_dgemm_dl(int m, int n, int
             B = B2;
             kl -= k;
         }
-        const __amdgpu_buffer_rsrc_t ra = dl_rsrc(A + i0 + (int64_t)kl * lda);
-        const __amdgpu_buffer_rsrc_t rb = dl_rsrc(TRB ? B + j0 + (int64_t)kl * ldb : B + (int64_t)j0 * ldb + kl);
+        const __amdgpu_buffer_rsrc_t ra = raw_rsrc(A + i0 + (int64_t)kl * lda);
+        const __amdgpu_buffer_rsrc_t rb = raw_rsrc(TRB ? B + j0 + (int64_t)kl * ldb : B + (int64_t)j0 * ldb + kl);
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
             const int ins = uwave * LPW + q;
@@ -648,9 +651,6 @@ static int run_dgemm_dl(omx_ctx ctx, int m, int n, int k, double alpha, const do
 // staged kernel's, so the results are bitwise the same.  Requires m, n % 128 == 0, k % 32 == 0,
 // leading dimensions that are multiples of 4 and 16-byte aligned operands.
 // ---------------------------------------------------------------------------------------------
-__device__ inline __amdgpu_buffer_rsrc_t sl_rsrc(const float *p) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, 0x7fffffff, 0x00020000);
-}
 __device__ inline int sl_swz(int r) { return 4 * (r & 3); }
 
 template <class CF>
@@ -690,8 +690,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_sgemm_dl(int m, int n, int
     auto issue = [&](int s) {
         float *st = smem + (s & 1) * STAGE;
         const int kl = s * BK;
-        const __amdgpu_buffer_rsrc_t ra = sl_rsrc(A + i0 + (int64_t)kl * lda);
-        const __amdgpu_buffer_rsrc_t rb = sl_rsrc(B + j0 + (int64_t)kl * ldb);
+        const __amdgpu_buffer_rsrc_t ra = raw_rsrc(A + i0 + (int64_t)kl * lda);
+        const __amdgpu_buffer_rsrc_t rb = raw_rsrc(B + j0 + (int64_t)kl * ldb);
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
             const int ins = uwave * LPW + q;
@@ -1115,11 +1115,6 @@ __global__ __launch_bounds__(256) void k_to_bf16(int rows, int cols, const TS *_
 // distinct 16-byte bank groups, so the read is free of bank conflicts.
 __device__ inline int bg_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
 
-// Raw buffer descriptor over `elems` BF16 values from p (gfx9 descriptor word 3: 32-bit data,
-// no swizzle).  Reads past the range return zero instead of touching memory.
-__device__ inline __amdgpu_buffer_rsrc_t bg_rsrc(const __bf16 *p, int64_t elems) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16 *>(p), (short)0, (int)(2 * elems), 0x00020000);
-}
 
 // One BK = 64 slice of the BF16 panel product, software-pipelined: the wave's MFMAs run in groups
 // (one A fragment x the FBN B fragments of a 32-wide k step), and the LDS reads for group g + LEAD
@@ -1205,8 +1200,8 @@ __global__ __launch_bounds__(CF::NT, CF::MINW) void k_bgemm_tn(int m, int n, int
 
     // Buffer descriptors over the tile's A rows and B rows; each lane's byte offset of its chunk
     // in slice 0 is fixed for the whole K loop (slice s adds 2*s*BK).
-    const __amdgpu_buffer_rsrc_t ra = bg_rsrc(At + (int64_t)i0 * lda, (int64_t)min(CF::BM, m - i0) * lda);
-    const __amdgpu_buffer_rsrc_t rb = bg_rsrc(Bt + (int64_t)j0 * ldb, (int64_t)min(CF::BN, n - j0) * ldb);
+    const __amdgpu_buffer_rsrc_t ra = raw_rsrc(At + (int64_t)i0 * lda, (int)(2 * ((int64_t)min(CF::BM, m - i0) * lda)));
+    const __amdgpu_buffer_rsrc_t rb = raw_rsrc(Bt + (int64_t)j0 * ldb, (int)(2 * ((int64_t)min(CF::BN, n - j0) * ldb)));
     const int uwave = __builtin_amdgcn_readfirstlane(wave);
     int voff[LPW];
 #pragma unroll
